@@ -1,0 +1,160 @@
+/* sgpu.h -- C ABI of the MI355X-native SolarGPU pipeline (libsolargpu.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an sgpu_status;
+ * on failure sgpu_last_error() gives a message for the calling thread.  No
+ * entry point calls exit().
+ *
+ * Reference interfaces these entry points replace (SAMPA-Project/SolarGPU):
+ *   sgpu_uncompressed_irradiance  include/solar_gpu_api.h:37-42 (uncompressed_gpu_irradiance)
+ *   sgpu_compressed_irradiance    include/solar_gpu_api.h:49-54 (compressed_gpu_irradiance)
+ *   sgpu_sun_positions            include/sun_position.h:24-40 (SunPositionCalculator::calc)
+ *   sgpu_sky_bins                 src/sky_bins.cpp:9-52 (compute_sky_bins)
+ *   sgpu_detect_walls             src/voxelization.cpp:10-64 (detect_wall_voxels)
+ *   sgpu_top_slope_aspect         src/voxelization.cpp:66-97 (compute_top_slope_aspect)
+ *   sgpu_assemble_voxels          src/main.cpp:112-127 (voxel array assembly)
+ *   sgpu_viewshed                 include/cuda/viewshed_stage.cuh:36-125 (run_viewshed_out_of_core)
+ *                                 + the CSR consumer (src/cuda/gpu_irradiance_uncompressed.cu:79-95)
+ *                                 + the ba-RLE consumer (src/cuda/gpu_irradiance_compressed.cu:81-100,
+ *                                   src/barle_compression.cpp:189-207)
+ *   sgpu_hour_loop                src/cuda/gpu_irradiance_uncompressed.cu:107-135 and
+ *                                 src/cuda/gpu_irradiance_compressed.cu:116-144 (stage-2 hour loop)
+ *   sgpu_hour_single              one iteration of that loop (src/cuda/kernels/solar_kernel.cu:54-77,
+ *                                 src/cuda/kernels/reflection_kernels.cu:70-183)
+ *   sgpu_barle_encode             src/barle_compression.cpp:156-207 (compress_all_neighbors), on the GPU
+ * Entry points with no reference counterpart are marked "(new)".
+ */
+#ifndef SGPU_H
+#define SGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGPU_HOURS_PER_YEAR 8760
+
+typedef enum sgpu_status {
+    SGPU_OK = 0,
+    SGPU_ERR_ARG = 1,     /* invalid argument / inconsistent sizes */
+    SGPU_ERR_HIP = 2,     /* HIP runtime failure */
+    SGPU_ERR_OOM = 3,     /* device or host allocation failed */
+    SGPU_ERR_STATE = 4,   /* call out of order (e.g. hour loop before viewshed) */
+    SGPU_ERR_IO = 5,      /* file could not be read/written */
+    SGPU_ERR_FORMAT = 6   /* malformed input file or stream */
+} sgpu_status;
+
+/* Neighbour-list storage formats (bit flags for sgpu_viewshed). */
+#define SGPU_FMT_CSR 1u
+#define SGPU_FMT_BARLE 2u
+
+typedef struct sgpu_ctx sgpu_ctx;
+
+/* Run statistics of a context (new). Times are HIP-event milliseconds on the context stream. */
+typedef struct sgpu_info {
+    int32_t grid_width, grid_height;
+    int64_t num_total, num_top;
+    int32_t active_bins;        /* sun-visited upper-hemisphere bins */
+    int32_t svf_bytes_per_voxel;
+    int32_t sunlit_hours;       /* hours with nudged altitude >= 1 deg */
+    int32_t viewshed_chunks;
+    int64_t chunk_voxels;
+    int64_t nnz;                /* CSR neighbour ids */
+    int64_t barle_bytes;        /* ba-RLE stream bytes */
+    double t_trace_ms;          /* DDA trace kernels */
+    double t_lists_ms;          /* dedupe/compaction/encode kernels */
+    double t_viewshed_ms;       /* whole stage 1 */
+    double t_hours_ms;          /* whole stage 2 (last sgpu_hour_loop) */
+    int32_t hours_computed;     /* last sgpu_hour_loop */
+    int32_t hour_passes;        /* kernel passes of the last sgpu_hour_loop */
+    int32_t hours_per_pass;
+    int32_t timed_launches;     /* reflection launches timed individually (sgpu_set_kernel_timing) */
+    double t_refl_kernel_ms;    /* sum of those launch times */
+    double t_solar_kernel_ms;   /* sum of solar-top launch times */
+    int64_t dda_steps;          /* only when step counting is enabled */
+} sgpu_info;
+
+const char* sgpu_version(void);
+const char* sgpu_last_error(void);
+
+/* ---- host-side contract (CPU) ---- */
+int sgpu_sun_positions(float lon, float lat, int tz, float* az_alt_out /* 2*8760 */);
+int sgpu_sky_bins(const float* az_alt /* 2*8760 */, float* nudged_out /* 2*8760 */,
+                  int32_t* full_to_dense /* 72*18 */, int32_t* sun_dense_idx /* 8760 */,
+                  int32_t* active_bins, int32_t* svf_bytes_per_voxel);
+int sgpu_directions(float* cdir /* 3*2592 */, int32_t* is_upper /* 2592 */); /* (new) host Cdir table */
+int sgpu_detect_walls(const float* heights, int w, int h, float voxel_size, int32_t* parent,
+                      float* z, float* aspect, int64_t cap, int64_t* count);
+int sgpu_top_slope_aspect(const float* heights, int w, int h, double scale_x, double scale_y,
+                          float* slope, float* aspect);
+/* Full voxel arrays from a heightmap: n_total/n_top out; arrays sized by the caller with
+   capacity cap (call once with cap = 0 to get n_total). */
+int sgpu_assemble_voxels(const float* heights, int w, int h, double scale_x, double scale_y,
+                         int64_t cap, int64_t* n_total, int32_t* pixel_parent, float* slopes,
+                         float* aspects, float* wall_z);
+/* Seeded synthetic inputs (new). */
+int sgpu_synth_dsm(uint64_t seed, int w, int h, float cell_m, float building_density, int trees,
+                   float* heights_out);
+int sgpu_synth_weather(uint64_t seed, const float* az_alt, float* dni_out, float* dhi_out);
+
+/* ---- device pipeline ---- */
+int sgpu_create(int device, sgpu_ctx** out);
+int sgpu_destroy(sgpu_ctx* ctx);
+int sgpu_get_info(sgpu_ctx* ctx, sgpu_info* out);
+
+/* Upload the scene.  wall_z may be NULL when num_total == num_top. */
+int sgpu_set_scene(sgpu_ctx* ctx, const float* grid, int grid_width, int grid_height, int64_t num_total,
+                   int64_t num_top, const int32_t* pixel_parent, const float* wall_z, float max_height,
+                   const float* slopes, const float* aspects);
+/* Sun positions (2*8760 floats, az/alt degrees, NOT nudged) and TMY DNI/DHI (8760 each). */
+int sgpu_set_sky(sgpu_ctx* ctx, const float* az_alt, const float* dni, const float* dhi);
+/* Override the viewshed chunk (voxels per out-of-core chunk); 0 = from free memory. (new) */
+int sgpu_set_chunk_voxels(sgpu_ctx* ctx, int64_t chunk_voxels);
+/* Stage 1: trace every voxel x 2592 directions; builds the SVF bitmap and the requested formats. */
+int sgpu_viewshed(sgpu_ctx* ctx, unsigned formats);
+int sgpu_get_svf(sgpu_ctx* ctx, uint8_t* out /* svf_bytes_per_voxel * num_total */);
+int sgpu_get_csr(sgpu_ctx* ctx, int64_t* offsets /* num_total+1 */, int32_t* ids /* nnz */);
+int sgpu_get_barle(sgpu_ctx* ctx, int64_t* offsets /* num_total+1 */, uint8_t* bytes);
+/* Raw first-hit ids (id+1, 0 = none) of voxels [v0, v0+count) x 2592 directions (new; tests). */
+int sgpu_trace_raw(sgpu_ctx* ctx, int64_t v0, int64_t count, int32_t* raw_out);
+/* Supply lists/SVF from elsewhere (tests, a persisted cache) instead of tracing. (new) */
+int sgpu_set_svf(sgpu_ctx* ctx, const uint8_t* svf);
+int sgpu_set_csr(sgpu_ctx* ctx, const int64_t* offsets, const int32_t* ids);
+int sgpu_set_barle(sgpu_ctx* ctx, const int64_t* offsets, const uint8_t* bytes);
+/* GPU ba-RLE encoder from the context's CSR lists (device to device). */
+int sgpu_barle_encode(sgpu_ctx* ctx);
+
+/* Stage 2: the annual loop over sunlit hours with format SGPU_FMT_CSR or SGPU_FMT_BARLE.
+   hours_per_pass (1..32) hours share one pass over the neighbour stream; 0 = autotuned.
+   hour_stride > 1 computes every hour_stride-th sunlit hour only (bounded samples).
+   Annual sums are FP32 Kahan-compensated. */
+int sgpu_hour_loop(sgpu_ctx* ctx, unsigned format, int hours_per_pass, int hour_stride);
+/* Record the hour loop as one hipGraph and replay it (new). */
+int sgpu_hour_loop_graph(sgpu_ctx* ctx, unsigned format, int hours_per_pass, int replays);
+int sgpu_get_irradiance(sgpu_ctx* ctx, float* out /* num_total */);
+/* One hour (index day*24+hour) through the solar + reflection kernels; host outputs. */
+int sgpu_hour_single(sgpu_ctx* ctx, unsigned format, int hour_index, float* hourly_out,
+                     float* reflected_out);
+/* Time each reflection / solar launch with HIP events on the context stream (new). */
+int sgpu_set_kernel_timing(sgpu_ctx* ctx, int enable);
+/* Count DDA steps in the trace kernel (slower; new). */
+int sgpu_set_step_counting(sgpu_ctx* ctx, int enable);
+/* Synchronise the context stream. */
+int sgpu_sync(sgpu_ctx* ctx);
+
+/* ---- drop-in whole pipelines (solar_gpu_api.h:37-54) ----
+   sun_az_alt: 2*8760 floats; dni/dhi: 8760 floats each. */
+int sgpu_uncompressed_irradiance(const float* grid_data, int grid_width, int grid_height, int num_total_voxels,
+                                 int num_top_voxels, const int32_t* pixel_parent, const float* wall_z,
+                                 float max_height, const float* sun_az_alt, const float* dni, const float* dhi,
+                                 const float* slopes, const float* aspects, float* irradiance_out);
+int sgpu_compressed_irradiance(const float* grid_data, int grid_width, int grid_height, int num_total_voxels,
+                               int num_top_voxels, const int32_t* pixel_parent, const float* wall_z,
+                               float max_height, const float* sun_az_alt, const float* dni, const float* dhi,
+                               const float* slopes, const float* aspects, float* irradiance_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGPU_H */

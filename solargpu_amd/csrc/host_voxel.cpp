@@ -1,0 +1,70 @@
+/* Voxelization contract (host C++): column-drop wall voxels and top-cell
+ * slope/aspect, as in the reference's voxelization.cpp:10-97, plus the voxel
+ * array assembly of main.cpp:112-127.  Built with -ffp-contract=off. */
+#include <algorithm>
+#include <cmath>
+
+#include "sgpu_internal.h"
+
+namespace sgpu {
+
+/* voxelization.cpp:10-64.  A cell whose drop to its lowest 4-neighbour (an
+ * off-grid side counts as no drop) exceeds one voxel gets floor(drop/voxel)
+ * stacked wall voxels; each faces the side(s) open at its height, where an
+ * off-grid side is fully open (-1e30 sentinel, not clamped). */
+Walls detect_walls(const float* H, int w, int h, float voxel) {
+    Walls out;
+    const float kOpen = -1e30f;
+    for (int gy = 0; gy < h; ++gy) {
+        for (int gx = 0; gx < w; ++gx) {
+            const float z0 = H[(size_t)gy * w + gx];
+            const float nb[4] = {
+                gx + 1 < w ? H[(size_t)gy * w + gx + 1] : kOpen,   // east
+                gx - 1 >= 0 ? H[(size_t)gy * w + gx - 1] : kOpen,  // west
+                gy - 1 >= 0 ? H[(size_t)(gy - 1) * w + gx] : kOpen, // north
+                gy + 1 < h ? H[(size_t)(gy + 1) * w + gx] : kOpen,  // south
+            };
+            float lowest = z0;
+            bool first = true;
+            for (float v : nb) {
+                const float c = (v <= kOpen * 0.5f) ? z0 : v;
+                lowest = first ? c : std::min(lowest, c);
+                first = false;
+            }
+            const float drop = z0 - lowest;
+            if (drop <= voxel) continue;
+            const int count = (int)(drop / voxel);
+            for (int k = 0; k < count; ++k) {
+                const float zz = z0 - (k + 0.5f) * voxel;
+                const float oe = std::max(0.0f, zz - nb[0]), ow = std::max(0.0f, zz - nb[1]);
+                const float on = std::max(0.0f, zz - nb[2]), os = std::max(0.0f, zz - nb[3]);
+                float a = std::atan2(oe - ow, on - os);
+                if (a < 0.0f) a += 2.0f * (float)3.14159265358979323846;
+                out.parent.push_back(gy * w + gx);
+                out.z.push_back(zz);
+                out.aspect.push_back(a);
+            }
+        }
+    }
+    return out;
+}
+
+/* voxelization.cpp:66-97: central differences clamped at the border; slope is
+ * atan(|grad|), aspect atan2(-dx, dy) wrapped to [0, 2pi). */
+void top_slope_aspect(const float* H, int w, int h, double sx, double sy, float* slope, float* aspect) {
+    for (int gy = 0; gy < h; ++gy) {
+        const int up = std::max(gy - 1, 0), dn = std::min(gy + 1, h - 1);
+        for (int gx = 0; gx < w; ++gx) {
+            const int lf = std::max(gx - 1, 0), rt = std::min(gx + 1, w - 1);
+            const float dx = (H[(size_t)gy * w + rt] - H[(size_t)gy * w + lf]) / ((rt - lf) * (float)sx);
+            const float dy = (H[(size_t)dn * w + gx] - H[(size_t)up * w + gx]) / ((dn - up) * (float)sy);
+            const size_t i = (size_t)gy * w + gx;
+            slope[i] = std::atan2(std::sqrt(dx * dx + dy * dy), 1.0f);
+            float a = std::atan2(-dx, dy);
+            if (a < 0.0f) a += 2.0f * (float)3.14159265358979323846;
+            aspect[i] = a;
+        }
+    }
+}
+
+}  // namespace sgpu

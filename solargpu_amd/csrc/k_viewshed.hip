@@ -1,0 +1,270 @@
+/* Stage 1 on gfx950: hemispherical DDA viewshed, SVF bitmap and CSR
+ * neighbour lists.
+ *
+ * k_trace   one lane per voxel, one workgroup per (256-voxel block,
+ *           32-direction group).  Lanes of a wave trace the same direction
+ *           from 64 consecutive voxels, so their height-grid reads walk
+ *           neighbouring cells together.  First-hit ids go to a per-chunk raw
+ *           table raw[voxel][direction] (int32, 0 = no hit).
+ * k_lists   one workgroup per voxel: LDS hash set of the voxel's hit cells
+ *           with the smallest direction index per cell (LDS atomics), then an
+ *           ordered block scan keeps the first occurrence of every cell in
+ *           direction order -- the reference's append-if-absent list
+ *           (viewshed_kernels.cu:50-55) -- and ORs the SVF bits.  Pass 1
+ *           counts, pass 2 writes at scanned offsets.
+ *
+ * The DDA keeps the reference's operation order (viewshed_kernels.cu:27-58);
+ * the library is compiled with -ffp-contract=off and IEEE division. */
+#include <climits>
+
+#include "sgpu_internal.h"
+
+namespace sgpu {
+namespace {
+
+constexpr int kTraceBlock = 256;
+constexpr int kDirsPerGroup = 32;            // 2592 = 81 x 32; 128 B of raw per voxel per group
+constexpr int kListThreads = 256;
+constexpr int kDirsPerThread = (kDirs + kListThreads - 1) / kListThreads;  // 11
+constexpr int kHashSize = 4096;              // > 1.5 x the 2592 possible distinct hits
+
+template <bool COUNT_STEPS>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
+    const int64_t local = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    if (local >= a.count) return;
+    const int64_t v = a.v0 + local;
+    const int top = a.pix_parent[v];
+    const int gx = top % a.W, gy = top / a.W;
+    const float base_z = (v < a.n_top) ? a.grid[gx + gy * a.W] : a.wall_z[v - a.n_top];
+    const float gxf = (float)gx, gyf = (float)gy;
+    const unsigned W = (unsigned)a.W, H = (unsigned)a.H;
+    const float* __restrict__ grid = a.grid;
+    int32_t* out = a.raw + local * kDirs + blockIdx.y * kDirsPerGroup;
+    unsigned long long steps = 0;
+
+    for (int k = 0; k < kDirsPerGroup; ++k) {
+        const DirConst dc = a.dirs[blockIdx.y * kDirsPerGroup + k];   // wave-uniform
+        const bool upper = dc.info & 1;
+        const float ox = gxf + dc.cx * 2.0f;
+        const float oy = gyf + dc.cy * 2.0f;
+        const float oz = base_z + dc.cz * 2.0f;
+        int ix = (int)floorf(ox);
+        int iy = (int)floorf(oy);
+        float tmx = ((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix)) / dc.ax;
+        float tmy = ((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy)) / dc.ay;
+        const int sx = (dc.cx >= 0.0f) ? 1 : -1;
+        const int sy = (dc.cy >= 0.0f) ? 1 : -1;
+        int hit = 0;
+        for (;;) {
+            float t;
+            if (tmx < tmy) { t = tmx; ix += sx; tmx += dc.rx; }
+            else { t = tmy; iy += sy; tmy += dc.ry; }
+            if (COUNT_STEPS) ++steps;
+            if ((unsigned)ix >= W || (unsigned)iy >= H) break;
+            const float zr = oz + dc.cz * t;
+            if (upper ? (zr > a.maxz) : (zr < 0.0f)) break;
+            const int cell = ix + iy * a.W;
+            if (grid[cell] - zr > 0.01f) { hit = cell + 1; break; }
+        }
+        out[k] = hit;
+    }
+    if (COUNT_STEPS) atomicAdd(a.steps, steps);
+}
+
+/* Inclusive wave64 scan. */
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+/* Exclusive scan over the 256 threads of a block; *total gets the block sum. */
+__device__ __forceinline__ int block_excl_scan(int x, int* tmp, int* total) {
+    const int inc = wave_incl_scan(x);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) tmp[wave] = inc;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kListThreads / 64; ++w) {
+        const int s = tmp[w];
+        if (w < wave) before += s;
+        all += s;
+    }
+    *total = all;
+    return before + inc - x;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
+    __shared__ int keys[kHashSize];
+    __shared__ int mins[kHashSize];
+    __shared__ unsigned svf_acc[kAzBins * kElBins / 8 + 1];
+    __shared__ int tmp[kListThreads / 64];
+
+    const int64_t local = blockIdx.x;
+    const int64_t v = a.v0 + local;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kHashSize; i += kListThreads) { keys[i] = 0; mins[i] = INT_MAX; }
+    if (!WRITE)
+        for (int i = tid; i < a.svf_bytes; i += kListThreads) svf_acc[i] = 0u;
+    __syncthreads();
+
+    const int32_t* raw = a.raw + local * kDirs;
+    const int d0 = tid * kDirsPerThread;
+    int hv[kDirsPerThread];
+    int slot[kDirsPerThread];
+#pragma unroll
+    for (int k = 0; k < kDirsPerThread; ++k) {
+        const int d = d0 + k;
+        const int h = (d < kDirs) ? raw[d] : 0;
+        hv[k] = h;
+        slot[k] = 0;
+        if (h) {
+            unsigned s = ((unsigned)h * 2654435761u) >> (32 - 12);
+            for (;;) {
+                const int prev = atomicCAS(&keys[s], 0, h);
+                if (prev == 0 || prev == h) break;
+                s = (s + 1) & (kHashSize - 1);
+            }
+            slot[k] = (int)s;
+            atomicMin(&mins[s], d);
+            if (!WRITE) {
+                const int dense = (a.dirs[d].info >> 8) - 1;
+                if (dense >= 0) atomicOr(&svf_acc[dense >> 3], 1u << (dense & 7));
+            }
+        }
+    }
+    __syncthreads();
+
+    int kept = 0;
+#pragma unroll
+    for (int k = 0; k < kDirsPerThread; ++k)
+        if (hv[k] && mins[slot[k]] == d0 + k) ++kept;
+
+    int total;
+    const int excl = block_excl_scan(kept, tmp, &total);
+    if (!WRITE) {
+        if (tid == 0) a.counts[local] = total;
+        for (int i = tid; i < a.svf_bytes; i += kListThreads)
+            a.svf[(int64_t)i * a.n_total + v] = (uint8_t)svf_acc[i];
+    } else {
+        int32_t* dst = a.ids + a.offsets[local] + excl;
+#pragma unroll
+        for (int k = 0; k < kDirsPerThread; ++k)
+            if (hv[k] && mins[slot[k]] == d0 + k) *dst++ = hv[k];
+    }
+}
+
+/* ---- int32 counts -> int64 exclusive offsets ---- */
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ long long wave_incl_scan64(long long x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ long long block_excl_scan64(long long x, long long* tmp, long long* total) {
+    const long long inc = wave_incl_scan64(x);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) tmp[wave] = inc;
+    __syncthreads();
+    long long before = 0, all = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        if (w < wave) before += tmp[w];
+        all += tmp[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - x;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_tile_sums(const int32_t* c, int64_t n, int64_t* sums) {
+    __shared__ long long tmp[kScanThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    long long s = 0;
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += c[base + k];
+    long long total;
+    block_excl_scan64(s, tmp, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+/* Single block: exclusive scan of the tile sums in place; writes the grand total. */
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(int64_t* sums, int64_t ntiles, int64_t* total_out) {
+    __shared__ long long tmp[1024 / 64];
+    long long carry = 0;
+    for (int64_t b = 0; b < ntiles; b += 1024) {
+        const int64_t i = b + threadIdx.x;
+        const long long x = (i < ntiles) ? sums[i] : 0;
+        long long tot;
+        const long long ex = block_excl_scan64(x, tmp, &tot);
+        if (i < ntiles) sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int32_t* c, int64_t n, const int64_t* sums,
+                                                           int64_t* off) {
+    __shared__ long long tmp[kScanThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    long long s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? c[base + k] : 0;
+        s += v[k];
+    }
+    long long total;
+    long long run = sums[blockIdx.x] + block_excl_scan64(s, tmp, &total);
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) off[base + k] = run;
+        run += v[k];
+    }
+}
+
+}  // namespace
+
+void launch_trace(const TraceArgs& a, hipStream_t s) {
+    if (a.count <= 0) return;
+    dim3 grid((unsigned)((a.count + kTraceBlock - 1) / kTraceBlock), kDirs / kDirsPerGroup);
+    if (a.steps) hipLaunchKernelGGL(k_trace<true>, grid, dim3(kTraceBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_trace<false>, grid, dim3(kTraceBlock), 0, s, a);
+    SGPU_HIP(hipGetLastError());
+}
+
+void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s) {
+    if (a.count <= 0) return;
+    if (a.svf_bytes > kAzBins * kElBins / 8 + 1) fail(SGPU_ERR_ARG, "svf_bytes %d out of range", a.svf_bytes);
+    dim3 grid((unsigned)a.count);
+    if (write_pass) hipLaunchKernelGGL(k_lists<true>, grid, dim3(kListThreads), 0, s, a);
+    else hipLaunchKernelGGL(k_lists<false>, grid, dim3(kListThreads), 0, s, a);
+    SGPU_HIP(hipGetLastError());
+}
+
+size_t scan_scratch_elems(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+void launch_scan_counts(const int32_t* counts, int64_t n, int64_t* off, int64_t* scratch, hipStream_t s) {
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles == 0) {
+        SGPU_HIP(hipMemsetAsync(off, 0, sizeof(int64_t), s));
+        return;
+    }
+    hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, counts, n, scratch);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(1024), 0, s, scratch, tiles, off + n);
+    hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, counts, n, scratch, off);
+    SGPU_HIP(hipGetLastError());
+}
+
+}  // namespace sgpu

@@ -1,0 +1,971 @@
+/* C ABI of libsolargpu (include/sgpu.h): per-context device state, the
+ * out-of-core viewshed driver, the GPU ba-RLE encoder driver and the
+ * sunlit-hour loop.  All work runs on the context's own HIP stream; every
+ * failure becomes a status code and a thread-local message. */
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sgpu_internal.h"
+
+using namespace sgpu;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& fn) {
+    try {
+        fn();
+        return SGPU_OK;
+    } catch (const Failure& f) {
+        g_last_error = f.what();
+        return f.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return SGPU_ERR_OOM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SGPU_ERR_HIP;
+    }
+}
+
+template <typename T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    SGPU_HIP(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        SGPU_HIP(hipEventCreate(&a));
+        SGPU_HIP(hipEventCreate(&b));
+    }
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    float ms() const {
+        float t = 0.0f;
+        SGPU_HIP(hipEventElapsedTime(&t, a, b));
+        return t;
+    }
+};
+
+__global__ void k_add_base(const int64_t* src, int64_t n, int64_t base, int64_t* dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + base;
+}
+
+__global__ void k_fill_zero(float* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+}  // namespace
+
+struct sgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    int W = 0, H = 0;
+    int64_t n_total = 0, n_top = 0;
+    float maxz = 0.0f;
+    float* grid = nullptr;
+    int32_t* pix_parent = nullptr;
+    float* wall_z = nullptr;
+    float* slopes = nullptr;
+    float* aspects = nullptr;
+    bool scene = false;
+
+    bool sky_set = false;
+    SkyTables sky;
+    std::vector<HourConst> hours;
+    DirConst* dirs = nullptr;
+
+    uint8_t* svf = nullptr;
+    bool svf_ok = false;
+    int64_t* csr_off = nullptr;
+    int32_t* csr_ids = nullptr;
+    int64_t nnz = -1;
+    int64_t* br_off = nullptr;
+    uint8_t* br_bytes = nullptr;
+    int64_t br_size = -1;
+
+    float* sum = nullptr;
+    float* comp = nullptr;
+    float* hourly_top = nullptr;
+    size_t hourly_top_elems = 0;
+    HourConst* hours_d = nullptr;
+    size_t hours_d_cap = 0;
+    bool irr_ok = false;
+
+    int64_t chunk_override = 0;
+    bool kernel_timing = false;
+    bool count_steps = false;
+    sgpu_info info{};
+
+    void set_device() const { SGPU_HIP(hipSetDevice(device)); }
+
+    void drop_products() {
+        dfree(svf);
+        dfree(csr_off);
+        dfree(csr_ids);
+        dfree(br_off);
+        dfree(br_bytes);
+        svf_ok = false;
+        nnz = -1;
+        br_size = -1;
+        irr_ok = false;
+    }
+    void drop_scene() {
+        dfree(grid);
+        dfree(pix_parent);
+        dfree(wall_z);
+        dfree(slopes);
+        dfree(aspects);
+        dfree(sum);
+        dfree(comp);
+        dfree(hourly_top);
+        hourly_top_elems = 0;
+        scene = false;
+    }
+    ~sgpu_ctx() {
+        if (device >= 0) (void)hipSetDevice(device);
+        drop_products();
+        drop_scene();
+        dfree(dirs);
+        dfree(hours_d);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void need(bool ok, int code, const char* msg) {
+    if (!ok) fail(code, "%s", msg);
+}
+
+int64_t choose_chunk(const sgpu_ctx* c) {
+    if (c->chunk_override > 0) return std::min<int64_t>(c->chunk_override, c->n_total);
+    /* Raw first-hit table + list counters per voxel; keep 40% of free memory as the
+       ceiling (the reference's budget, gpu_memory_budget.cuh:19-35), capped at 2^31
+       table entries per chunk. */
+    size_t free_b = 0, total_b = 0;
+    SGPU_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double per_voxel = (double)kDirs * 4.0 + 32.0;
+    int64_t chunk = (int64_t)(0.4 * (double)free_b / per_voxel);
+    chunk = std::min<int64_t>(chunk, (int64_t)1 << 20 << 2);
+    chunk = std::max<int64_t>(chunk, 1);
+    return std::min<int64_t>(chunk, c->n_total);
+}
+
+void ensure_hours_d(sgpu_ctx* c, const std::vector<HourConst>& hs) {
+    if (hs.size() > c->hours_d_cap) {
+        dfree(c->hours_d);
+        c->hours_d = dalloc<HourConst>(hs.size());
+        c->hours_d_cap = hs.size();
+    }
+    if (!hs.empty())
+        SGPU_HIP(hipMemcpyAsync(c->hours_d, hs.data(), hs.size() * sizeof(HourConst), hipMemcpyHostToDevice,
+                                c->stream));
+}
+
+void ensure_accumulators(sgpu_ctx* c, int K) {
+    if (!c->sum) c->sum = dalloc<float>(c->n_total);
+    if (!c->comp) c->comp = dalloc<float>(c->n_total);
+    const size_t need_elems = (size_t)c->n_top * K;
+    if (need_elems > c->hourly_top_elems) {
+        dfree(c->hourly_top);
+        c->hourly_top = dalloc<float>(need_elems);
+        c->hourly_top_elems = need_elems;
+    }
+}
+
+HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
+    HourArgs a{};
+    a.grid = c->grid;
+    a.W = c->W;
+    a.H = c->H;
+    a.n_total = c->n_total;
+    a.n_top = c->n_top;
+    a.pix_parent = c->pix_parent;
+    a.wall_z = c->wall_z;
+    a.slopes = c->slopes;
+    a.aspects = c->aspects;
+    a.svf = c->svf;
+    a.hourly_top = c->hourly_top;
+    a.sum = c->sum;
+    a.comp = c->comp;
+    if (fmt == SGPU_FMT_CSR) {
+        a.nb_off = c->csr_off;
+        a.nb_ids = c->csr_ids;
+    } else {
+        a.nb_off = c->br_off;
+        a.nb_bytes = c->br_bytes;
+    }
+    return a;
+}
+
+void check_stage2_ready(sgpu_ctx* c, unsigned fmt) {
+    need(c->scene, SGPU_ERR_STATE, "scene not set");
+    need(c->sky_set, SGPU_ERR_STATE, "sky not set");
+    need(c->svf_ok, SGPU_ERR_STATE, "SVF bitmap missing (run sgpu_viewshed or sgpu_set_svf)");
+    if (fmt == SGPU_FMT_CSR) need(c->nnz >= 0, SGPU_ERR_STATE, "CSR lists missing");
+    else if (fmt == SGPU_FMT_BARLE) need(c->br_size >= 0, SGPU_ERR_STATE, "ba-RLE streams missing");
+    else fail(SGPU_ERR_ARG, "format must be SGPU_FMT_CSR or SGPU_FMT_BARLE");
+}
+
+int autotuned_hours_per_pass(const sgpu_ctx* c, unsigned fmt) {
+    (void)c;
+    (void)fmt;
+    return 16;
+}
+
+/* Selected sunlit hours and the passes over them. */
+std::vector<HourConst> select_hours(const sgpu_ctx* c, int stride) {
+    std::vector<HourConst> hs;
+    for (size_t i = 0; i < c->hours.size(); i += (size_t)std::max(stride, 1)) hs.push_back(c->hours[i]);
+    return hs;
+}
+
+void run_hour_passes(sgpu_ctx* c, unsigned fmt, int K, int nsel, std::vector<EventPair>* ev_refl,
+                     std::vector<EventPair>* ev_solar) {
+    HourArgs a = hour_args(c, fmt);
+    int pass = 0;
+    for (int h0 = 0; h0 < nsel; h0 += K, ++pass) {
+        a.hours = c->hours_d + h0;
+        a.nh = std::min(K, nsel - h0);
+        if (ev_solar) SGPU_HIP(hipEventRecord((*ev_solar)[pass].a, c->stream));
+        launch_solar_top(a, K, c->stream);
+        if (ev_solar) SGPU_HIP(hipEventRecord((*ev_solar)[pass].b, c->stream));
+        if (ev_refl) SGPU_HIP(hipEventRecord((*ev_refl)[pass].a, c->stream));
+        launch_reflect_accum(a, K, fmt, c->stream);
+        if (ev_refl) SGPU_HIP(hipEventRecord((*ev_refl)[pass].b, c->stream));
+    }
+}
+
+void zero_accumulators(sgpu_ctx* c) {
+    const unsigned g = (unsigned)((c->n_total + 255) / 256);
+    hipLaunchKernelGGL(k_fill_zero, dim3(g), dim3(256), 0, c->stream, c->sum, c->n_total);
+    hipLaunchKernelGGL(k_fill_zero, dim3(g), dim3(256), 0, c->stream, c->comp, c->n_total);
+    SGPU_HIP(hipGetLastError());
+}
+
+void do_viewshed(sgpu_ctx* c, unsigned formats) {
+    need(c->scene, SGPU_ERR_STATE, "scene not set");
+    need(c->sky_set, SGPU_ERR_STATE, "sky not set");
+    need((formats & ~(SGPU_FMT_CSR | SGPU_FMT_BARLE)) == 0 && formats != 0, SGPU_ERR_ARG, "bad formats");
+    c->drop_products();
+    const int64_t N = c->n_total;
+    const int64_t chunk = choose_chunk(c);
+    c->info.chunk_voxels = chunk;
+    c->info.viewshed_chunks = (int32_t)((N + chunk - 1) / chunk);
+    c->info.t_trace_ms = c->info.t_lists_ms = 0.0;
+    c->info.dda_steps = 0;
+
+    EventPair all, ev_trace, ev_lists;
+    SGPU_HIP(hipEventRecord(all.a, c->stream));
+    c->svf = dalloc<uint8_t>((size_t)std::max(c->sky.svf_bytes, 1) * N);
+    c->csr_off = dalloc<int64_t>(N + 1);
+    int32_t* raw = dalloc<int32_t>((size_t)chunk * kDirs);
+    int32_t* counts = dalloc<int32_t>(chunk);
+    int64_t* off = dalloc<int64_t>(chunk + 1);
+    int64_t* scratch = dalloc<int64_t>(scan_scratch_elems(chunk));
+    unsigned long long* steps = c->count_steps ? dalloc<unsigned long long>(1) : nullptr;
+    if (steps) SGPU_HIP(hipMemsetAsync(steps, 0, sizeof(unsigned long long), c->stream));
+    std::vector<std::pair<int32_t*, int64_t>> pieces;
+    int64_t base = 0;
+    try {
+        for (int64_t v0 = 0; v0 < N; v0 += chunk) {
+            const int64_t cnt = std::min(chunk, N - v0);
+            TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, cnt, raw,
+                         steps};
+            SGPU_HIP(hipEventRecord(ev_trace.a, c->stream));
+            launch_trace(ta, c->stream);
+            SGPU_HIP(hipEventRecord(ev_trace.b, c->stream));
+            ListArgs la{raw, v0, cnt, N, c->dirs, c->sky.svf_bytes, c->svf, counts, off, nullptr};
+            SGPU_HIP(hipEventRecord(ev_lists.a, c->stream));
+            launch_lists(la, false, c->stream);
+            launch_scan_counts(counts, cnt, off, scratch, c->stream);
+            int64_t piece = 0;
+            SGPU_HIP(hipMemcpyAsync(&piece, off + cnt, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            int32_t* ids = dalloc<int32_t>((size_t)std::max<int64_t>(piece, 1));
+            pieces.push_back({ids, piece});
+            la.ids = ids;
+            launch_lists(la, true, c->stream);
+            const unsigned g = (unsigned)((cnt + 255) / 256);
+            hipLaunchKernelGGL(k_add_base, dim3(g), dim3(256), 0, c->stream, off, cnt, base, c->csr_off + v0);
+            SGPU_HIP(hipGetLastError());
+            SGPU_HIP(hipEventRecord(ev_lists.b, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            c->info.t_trace_ms += ev_trace.ms();
+            c->info.t_lists_ms += ev_lists.ms();
+            base += piece;
+        }
+        SGPU_HIP(hipMemcpyAsync(c->csr_off + N, &base, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+        if (pieces.size() == 1) {
+            c->csr_ids = pieces[0].first;
+            pieces.clear();
+        } else {
+            c->csr_ids = dalloc<int32_t>((size_t)std::max<int64_t>(base, 1));
+            int64_t pos = 0;
+            for (auto& p : pieces) {
+                if (p.second)
+                    SGPU_HIP(hipMemcpyAsync(c->csr_ids + pos, p.first, p.second * sizeof(int32_t),
+                                            hipMemcpyDeviceToDevice, c->stream));
+                pos += p.second;
+            }
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            for (auto& p : pieces) dfree(p.first);
+            pieces.clear();
+        }
+        if (steps) {
+            unsigned long long st = 0;
+            SGPU_HIP(hipMemcpyAsync(&st, steps, sizeof st, hipMemcpyDeviceToHost, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            c->info.dda_steps = (int64_t)st;
+        }
+    } catch (...) {
+        for (auto& p : pieces) dfree(p.first);
+        dfree(raw);
+        dfree(counts);
+        dfree(off);
+        dfree(scratch);
+        dfree(steps);
+        throw;
+    }
+    dfree(raw);
+    dfree(counts);
+    dfree(off);
+    dfree(scratch);
+    dfree(steps);
+    c->nnz = base;
+    c->svf_ok = true;
+    c->info.nnz = base;
+    if (formats & SGPU_FMT_BARLE) {
+        BarleArgs ba{};
+        ba.csr_off = c->csr_off;
+        ba.csr_ids = c->csr_ids;
+        ba.n = N;
+        int32_t* sizes = dalloc<int32_t>(N);
+        int64_t* scr = dalloc<int64_t>(scan_scratch_elems(N));
+        c->br_off = dalloc<int64_t>(N + 1);
+        ba.sizes = sizes;
+        SGPU_HIP(hipEventRecord(ev_lists.a, c->stream));
+        launch_barle(ba, false, c->stream);
+        launch_scan_counts(sizes, N, c->br_off, scr, c->stream);
+        int64_t total = 0;
+        SGPU_HIP(hipMemcpyAsync(&total, c->br_off + N, sizeof total, hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->br_bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
+        ba.out_off = c->br_off;
+        ba.out = c->br_bytes;
+        launch_barle(ba, true, c->stream);
+        SGPU_HIP(hipEventRecord(ev_lists.b, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->info.t_lists_ms += ev_lists.ms();
+        dfree(sizes);
+        dfree(scr);
+        c->br_size = total;
+        c->info.barle_bytes = total;
+    }
+    SGPU_HIP(hipEventRecord(all.b, c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+    c->info.t_viewshed_ms = all.ms();
+}
+
+void do_barle_encode(sgpu_ctx* c) {
+    need(c->nnz >= 0, SGPU_ERR_STATE, "CSR lists missing");
+    const int64_t N = c->n_total;
+    dfree(c->br_off);
+    dfree(c->br_bytes);
+    c->br_size = -1;
+    BarleArgs ba{};
+    ba.csr_off = c->csr_off;
+    ba.csr_ids = c->csr_ids;
+    ba.n = N;
+    int32_t* sizes = dalloc<int32_t>(N);
+    int64_t* scr = dalloc<int64_t>(scan_scratch_elems(N));
+    c->br_off = dalloc<int64_t>(N + 1);
+    ba.sizes = sizes;
+    launch_barle(ba, false, c->stream);
+    launch_scan_counts(sizes, N, c->br_off, scr, c->stream);
+    int64_t total = 0;
+    SGPU_HIP(hipMemcpyAsync(&total, c->br_off + N, sizeof total, hipMemcpyDeviceToHost, c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+    c->br_bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
+    ba.out_off = c->br_off;
+    ba.out = c->br_bytes;
+    launch_barle(ba, true, c->stream);
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+    dfree(sizes);
+    dfree(scr);
+    c->br_size = total;
+    c->info.barle_bytes = total;
+}
+
+void do_hour_loop(sgpu_ctx* c, unsigned fmt, int K, int stride) {
+    check_stage2_ready(c, fmt);
+    if (K == 0) K = autotuned_hours_per_pass(c, fmt);
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16 or 32");
+    need(stride >= 1, SGPU_ERR_ARG, "hour_stride must be >= 1");
+    ensure_accumulators(c, K);
+    std::vector<HourConst> hs = select_hours(c, stride);
+    ensure_hours_d(c, hs);
+    const int nsel = (int)hs.size();
+    const int passes = (nsel + K - 1) / K;
+    std::vector<EventPair> ev_r, ev_s;
+    if (c->kernel_timing) {
+        ev_r.resize(passes);
+        ev_s.resize(passes);
+    }
+    EventPair all;
+    zero_accumulators(c);
+    SGPU_HIP(hipEventRecord(all.a, c->stream));
+    run_hour_passes(c, fmt, K, nsel, c->kernel_timing ? &ev_r : nullptr, c->kernel_timing ? &ev_s : nullptr);
+    SGPU_HIP(hipEventRecord(all.b, c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+    c->info.t_hours_ms = all.ms();
+    c->info.hours_computed = nsel;
+    c->info.hour_passes = passes;
+    c->info.hours_per_pass = K;
+    c->info.timed_launches = c->kernel_timing ? passes : 0;
+    c->info.t_refl_kernel_ms = 0.0;
+    c->info.t_solar_kernel_ms = 0.0;
+    for (auto& e : ev_r) c->info.t_refl_kernel_ms += e.ms();
+    for (auto& e : ev_s) c->info.t_solar_kernel_ms += e.ms();
+    c->irr_ok = true;
+}
+
+void do_hour_loop_graph(sgpu_ctx* c, unsigned fmt, int K, int replays) {
+    check_stage2_ready(c, fmt);
+    if (K == 0) K = autotuned_hours_per_pass(c, fmt);
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16 or 32");
+    need(replays >= 1, SGPU_ERR_ARG, "replays must be >= 1");
+    ensure_accumulators(c, K);
+    ensure_hours_d(c, c->hours);
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+    const int nsel = (int)c->hours.size();
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    SGPU_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        zero_accumulators(c);
+        run_hour_passes(c, fmt, K, nsel, nullptr, nullptr);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    SGPU_HIP(hipStreamEndCapture(c->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        SGPU_HIP(e);
+    }
+    EventPair all;
+    SGPU_HIP(hipEventRecord(all.a, c->stream));
+    for (int r = 0; r < replays; ++r) SGPU_HIP(hipGraphLaunch(exec, c->stream));
+    SGPU_HIP(hipEventRecord(all.b, c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    c->info.t_hours_ms = all.ms() / replays;
+    c->info.hours_computed = nsel;
+    c->info.hour_passes = (nsel + K - 1) / K;
+    c->info.hours_per_pass = K;
+    c->info.timed_launches = 0;
+    c->irr_ok = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sgpu_version(void) { return "solargpu-mi355x 0.1"; }
+const char* sgpu_last_error(void) { return g_last_error.c_str(); }
+
+int sgpu_sun_positions(float lon, float lat, int tz, float* out) {
+    return guarded([&] {
+        need(out != nullptr, SGPU_ERR_ARG, "null output");
+        sun_positions(lon, lat, tz, out);
+    });
+}
+
+int sgpu_sky_bins(const float* az_alt, float* nudged, int32_t* f2d, int32_t* sdi, int32_t* active,
+                  int32_t* svf_bytes) {
+    return guarded([&] {
+        need(az_alt && nudged && f2d && sdi && active && svf_bytes, SGPU_ERR_ARG, "null argument");
+        SkyTables t = sky_bins(az_alt);
+        std::memcpy(nudged, t.nudged.data(), t.nudged.size() * sizeof(float));
+        std::memcpy(f2d, t.full_to_dense.data(), t.full_to_dense.size() * sizeof(int32_t));
+        std::memcpy(sdi, t.sun_dense.data(), t.sun_dense.size() * sizeof(int32_t));
+        *active = t.active;
+        *svf_bytes = t.svf_bytes;
+    });
+}
+
+int sgpu_directions(float* cdir, int32_t* is_upper) {
+    return guarded([&] {
+        need(cdir && is_upper, SGPU_ERR_ARG, "null argument");
+        SkyTables t;
+        t.full_to_dense.assign(kAzBins * kElBins, -1);
+        std::vector<DirConst> d(kDirs);
+        directions(t, d.data());
+        for (int i = 0; i < kDirs; ++i) {
+            cdir[3 * i] = d[i].cx;
+            cdir[3 * i + 1] = d[i].cy;
+            cdir[3 * i + 2] = d[i].cz;
+            is_upper[i] = d[i].info & 1;
+        }
+    });
+}
+
+int sgpu_detect_walls(const float* heights, int w, int h, float voxel, int32_t* parent, float* z, float* aspect,
+                      int64_t cap, int64_t* count) {
+    return guarded([&] {
+        need(heights && count && w > 0 && h > 0, SGPU_ERR_ARG, "bad arguments");
+        Walls wl = detect_walls(heights, w, h, voxel);
+        *count = (int64_t)wl.parent.size();
+        const int64_t n = std::min<int64_t>(cap, *count);
+        if (n > 0) {
+            need(parent && z && aspect, SGPU_ERR_ARG, "null output");
+            std::memcpy(parent, wl.parent.data(), n * sizeof(int32_t));
+            std::memcpy(z, wl.z.data(), n * sizeof(float));
+            std::memcpy(aspect, wl.aspect.data(), n * sizeof(float));
+        }
+    });
+}
+
+int sgpu_top_slope_aspect(const float* heights, int w, int h, double sx, double sy, float* slope, float* aspect) {
+    return guarded([&] {
+        need(heights && slope && aspect && w > 0 && h > 0, SGPU_ERR_ARG, "bad arguments");
+        top_slope_aspect(heights, w, h, sx, sy, slope, aspect);
+    });
+}
+
+int sgpu_assemble_voxels(const float* heights, int w, int h, double sx, double sy, int64_t cap, int64_t* n_total,
+                         int32_t* pp, float* slopes, float* aspects, float* wall_z) {
+    return guarded([&] {
+        need(heights && n_total && w > 0 && h > 0, SGPU_ERR_ARG, "bad arguments");
+        /* main.cpp:112-127: tops in raster order, then walls; the voxel size is the x pixel scale. */
+        Walls wl = detect_walls(heights, w, h, (float)sx);
+        const int64_t ntop = (int64_t)w * h;
+        const int64_t ntot = ntop + (int64_t)wl.parent.size();
+        *n_total = ntot;
+        if (cap == 0) return;
+        need(cap >= ntot, SGPU_ERR_ARG, "capacity smaller than the voxel count");
+        need(pp && slopes && aspects, SGPU_ERR_ARG, "null output");
+        top_slope_aspect(heights, w, h, sx, sy, slopes, aspects);
+        for (int64_t i = 0; i < ntop; ++i) pp[i] = (int32_t)i;
+        for (size_t k = 0; k < wl.parent.size(); ++k) {
+            pp[ntop + k] = wl.parent[k];
+            slopes[ntop + k] = (float)3.14159265358979323846 * 0.5f;
+            aspects[ntop + k] = wl.aspect[k];
+            if (wall_z) wall_z[k] = wl.z[k];
+        }
+    });
+}
+
+int sgpu_synth_dsm(uint64_t seed, int w, int h, float cell_m, float density, int trees, float* out) {
+    return guarded([&] {
+        need(out && w > 0 && h > 0 && cell_m > 0.0f, SGPU_ERR_ARG, "bad arguments");
+        synth_dsm(seed, w, h, cell_m, density, trees, out);
+    });
+}
+
+int sgpu_synth_weather(uint64_t seed, const float* az_alt, float* dni, float* dhi) {
+    return guarded([&] {
+        need(az_alt && dni && dhi, SGPU_ERR_ARG, "null argument");
+        synth_weather(seed, az_alt, dni, dhi);
+    });
+}
+
+int sgpu_create(int device, sgpu_ctx** out) {
+    return guarded([&] {
+        need(out != nullptr, SGPU_ERR_ARG, "null output");
+        *out = nullptr;
+        int n = 0;
+        SGPU_HIP(hipGetDeviceCount(&n));
+        need(device >= 0 && device < n, SGPU_ERR_ARG, "no such device");
+        std::unique_ptr<sgpu_ctx> c(new sgpu_ctx());
+        c->device = device;
+        c->set_device();
+        SGPU_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->dirs = dalloc<DirConst>(kDirs);
+        *out = c.release();
+    });
+}
+
+int sgpu_destroy(sgpu_ctx* ctx) {
+    return guarded([&] { delete ctx; });
+}
+
+int sgpu_get_info(sgpu_ctx* c, sgpu_info* out) {
+    return guarded([&] {
+        need(c && out, SGPU_ERR_ARG, "null argument");
+        c->info.grid_width = c->W;
+        c->info.grid_height = c->H;
+        c->info.num_total = c->n_total;
+        c->info.num_top = c->n_top;
+        c->info.active_bins = c->sky.active;
+        c->info.svf_bytes_per_voxel = c->sky.svf_bytes;
+        c->info.sunlit_hours = (int32_t)c->hours.size();
+        *out = c->info;
+    });
+}
+
+int sgpu_set_scene(sgpu_ctx* c, const float* grid, int w, int h, int64_t n_total, int64_t n_top, const int32_t* pp,
+                   const float* wall_z, float maxz, const float* slopes, const float* aspects) {
+    return guarded([&] {
+        need(c && grid && pp && slopes && aspects, SGPU_ERR_ARG, "null argument");
+        need(w > 0 && h > 0 && (int64_t)w * h < (int64_t)1 << 31, SGPU_ERR_ARG, "grid size out of range");
+        need(n_top == (int64_t)w * h, SGPU_ERR_ARG, "num_top_voxels must equal grid_width * grid_height");
+        need(n_total >= n_top && n_total < (int64_t)1 << 31, SGPU_ERR_ARG, "num_total_voxels out of range");
+        need(n_total == n_top || wall_z != nullptr, SGPU_ERR_ARG, "wall_z required when there are wall voxels");
+        for (int64_t i = 0; i < n_total; ++i)
+            if (pp[i] < 0 || pp[i] >= n_top) fail(SGPU_ERR_ARG, "pixel_parent[%lld] out of range", (long long)i);
+        c->set_device();
+        c->drop_products();
+        c->drop_scene();
+        c->W = w;
+        c->H = h;
+        c->n_total = n_total;
+        c->n_top = n_top;
+        c->maxz = maxz;
+        c->grid = dalloc<float>(n_top);
+        c->pix_parent = dalloc<int32_t>(n_total);
+        c->slopes = dalloc<float>(n_total);
+        c->aspects = dalloc<float>(n_total);
+        const int64_t nw = n_total - n_top;
+        c->wall_z = dalloc<float>(std::max<int64_t>(nw, 1));
+        SGPU_HIP(hipMemcpyAsync(c->grid, grid, n_top * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipMemcpyAsync(c->pix_parent, pp, n_total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipMemcpyAsync(c->slopes, slopes, n_total * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipMemcpyAsync(c->aspects, aspects, n_total * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        if (nw > 0)
+            SGPU_HIP(hipMemcpyAsync(c->wall_z, wall_z, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->scene = true;
+    });
+}
+
+int sgpu_set_sky(sgpu_ctx* c, const float* az_alt, const float* dni, const float* dhi) {
+    return guarded([&] {
+        need(c && az_alt && dni && dhi, SGPU_ERR_ARG, "null argument");
+        c->set_device();
+        c->sky = sky_bins(az_alt);
+        c->hours = hour_constants(c->sky, dni, dhi);
+        std::vector<DirConst> d(kDirs);
+        directions(c->sky, d.data());
+        SGPU_HIP(hipMemcpyAsync(c->dirs, d.data(), kDirs * sizeof(DirConst), hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->drop_products();
+        c->sky_set = true;
+    });
+}
+
+int sgpu_set_chunk_voxels(sgpu_ctx* c, int64_t chunk) {
+    return guarded([&] {
+        need(c && chunk >= 0, SGPU_ERR_ARG, "bad argument");
+        c->chunk_override = chunk;
+    });
+}
+
+int sgpu_viewshed(sgpu_ctx* c, unsigned formats) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        c->set_device();
+        do_viewshed(c, formats);
+    });
+}
+
+int sgpu_get_svf(sgpu_ctx* c, uint8_t* out) {
+    return guarded([&] {
+        need(c && out, SGPU_ERR_ARG, "null argument");
+        need(c->svf_ok, SGPU_ERR_STATE, "no SVF bitmap");
+        c->set_device();
+        const size_t n = (size_t)c->sky.svf_bytes * c->n_total;
+        if (n) SGPU_HIP(hipMemcpyAsync(out, c->svf, n, hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int sgpu_get_csr(sgpu_ctx* c, int64_t* off, int32_t* ids) {
+    return guarded([&] {
+        need(c && off, SGPU_ERR_ARG, "null argument");
+        need(c->nnz >= 0, SGPU_ERR_STATE, "no CSR lists");
+        c->set_device();
+        SGPU_HIP(hipMemcpyAsync(off, c->csr_off, (c->n_total + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                c->stream));
+        if (c->nnz > 0) {
+            need(ids != nullptr, SGPU_ERR_ARG, "null ids");
+            SGPU_HIP(hipMemcpyAsync(ids, c->csr_ids, c->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        }
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int sgpu_get_barle(sgpu_ctx* c, int64_t* off, uint8_t* bytes) {
+    return guarded([&] {
+        need(c && off, SGPU_ERR_ARG, "null argument");
+        need(c->br_size >= 0, SGPU_ERR_STATE, "no ba-RLE streams");
+        c->set_device();
+        SGPU_HIP(hipMemcpyAsync(off, c->br_off, (c->n_total + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                c->stream));
+        if (c->br_size > 0) {
+            need(bytes != nullptr, SGPU_ERR_ARG, "null bytes");
+            SGPU_HIP(hipMemcpyAsync(bytes, c->br_bytes, c->br_size, hipMemcpyDeviceToHost, c->stream));
+        }
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int sgpu_trace_raw(sgpu_ctx* c, int64_t v0, int64_t count, int32_t* out) {
+    return guarded([&] {
+        need(c && out, SGPU_ERR_ARG, "null argument");
+        need(c->scene && c->sky_set, SGPU_ERR_STATE, "scene/sky not set");
+        need(v0 >= 0 && count >= 0 && v0 + count <= c->n_total, SGPU_ERR_ARG, "voxel range out of bounds");
+        c->set_device();
+        int32_t* raw = dalloc<int32_t>((size_t)count * kDirs);
+        try {
+            TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, count,
+                         raw, nullptr};
+            launch_trace(ta, c->stream);
+            SGPU_HIP(hipMemcpyAsync(out, raw, (size_t)count * kDirs * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                    c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            dfree(raw);
+            throw;
+        }
+        dfree(raw);
+    });
+}
+
+int sgpu_set_svf(sgpu_ctx* c, const uint8_t* svf) {
+    return guarded([&] {
+        need(c && svf, SGPU_ERR_ARG, "null argument");
+        need(c->scene && c->sky_set, SGPU_ERR_STATE, "scene/sky not set");
+        c->set_device();
+        dfree(c->svf);
+        const size_t n = (size_t)c->sky.svf_bytes * c->n_total;
+        c->svf = dalloc<uint8_t>(std::max<size_t>(n, 1));
+        if (n) SGPU_HIP(hipMemcpyAsync(c->svf, svf, n, hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->svf_ok = true;
+    });
+}
+
+int sgpu_set_csr(sgpu_ctx* c, const int64_t* off, const int32_t* ids) {
+    return guarded([&] {
+        need(c && off, SGPU_ERR_ARG, "null argument");
+        need(c->scene, SGPU_ERR_STATE, "scene not set");
+        const int64_t N = c->n_total;
+        need(off[0] == 0, SGPU_ERR_ARG, "offsets[0] must be 0");
+        for (int64_t i = 0; i < N; ++i) {
+            const int64_t len = off[i + 1] - off[i];
+            if (len < 0 || len > kDirs) fail(SGPU_ERR_ARG, "list %lld has bad length %lld", (long long)i, (long long)len);
+        }
+        const int64_t nnz = off[N];
+        for (int64_t k = 0; k < nnz; ++k)
+            if (ids[k] <= 0) fail(SGPU_ERR_ARG, "ids[%lld] must be a 1-based cell id", (long long)k);
+        c->set_device();
+        dfree(c->csr_off);
+        dfree(c->csr_ids);
+        c->csr_off = dalloc<int64_t>(N + 1);
+        c->csr_ids = dalloc<int32_t>((size_t)std::max<int64_t>(nnz, 1));
+        SGPU_HIP(hipMemcpyAsync(c->csr_off, off, (N + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+        if (nnz) SGPU_HIP(hipMemcpyAsync(c->csr_ids, ids, nnz * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->nnz = nnz;
+        c->info.nnz = nnz;
+    });
+}
+
+int sgpu_set_barle(sgpu_ctx* c, const int64_t* off, const uint8_t* bytes) {
+    return guarded([&] {
+        need(c && off, SGPU_ERR_ARG, "null argument");
+        need(c->scene, SGPU_ERR_STATE, "scene not set");
+        const int64_t N = c->n_total;
+        need(off[0] == 0, SGPU_ERR_ARG, "offsets[0] must be 0");
+        /* Walk every stream once on the host so the decoder never reads past a range. */
+        for (int64_t v = 0; v < N; ++v) {
+            const int64_t a = off[v], b = off[v + 1];
+            if (b < a || b - a > (int64_t)1 << 20) fail(SGPU_ERR_FORMAT, "stream %lld has a bad range", (long long)v);
+            int64_t i = a;
+            while (i < b) {
+                if (i + 3 > b) fail(SGPU_ERR_FORMAT, "stream %lld: truncated block header", (long long)v);
+                const unsigned bits = ((unsigned)bytes[i] << 16) | ((unsigned)bytes[i + 1] << 8) | bytes[i + 2];
+                i += 3;
+                unsigned w[8];
+                int nc = 0;
+                for (int j = 0; j < 8; ++j) {
+                    const unsigned cd = (bits >> (21 - 3 * j)) & 7u;
+                    if (cd == 7u) break;
+                    if (cd > 4u) fail(SGPU_ERR_FORMAT, "stream %lld: width code %u", (long long)v, cd);
+                    w[nc++] = cd;
+                }
+                if (i + nc > b) fail(SGPU_ERR_FORMAT, "stream %lld: truncated lengths", (long long)v);
+                int64_t payload = 0;
+                for (int j = 0; j < nc; ++j) {
+                    const int64_t len = bytes[i + j];
+                    payload += (w[j] == 0) ? (len + 1) / 2 : len * (int64_t)w[j];
+                }
+                i += nc + payload;
+                if (i > b) fail(SGPU_ERR_FORMAT, "stream %lld: truncated payload", (long long)v);
+            }
+        }
+        const int64_t total = off[N];
+        c->set_device();
+        dfree(c->br_off);
+        dfree(c->br_bytes);
+        c->br_off = dalloc<int64_t>(N + 1);
+        c->br_bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
+        SGPU_HIP(hipMemcpyAsync(c->br_off, off, (N + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+        if (total) SGPU_HIP(hipMemcpyAsync(c->br_bytes, bytes, total, hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->br_size = total;
+        c->info.barle_bytes = total;
+    });
+}
+
+int sgpu_barle_encode(sgpu_ctx* c) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        c->set_device();
+        do_barle_encode(c);
+    });
+}
+
+int sgpu_hour_loop(sgpu_ctx* c, unsigned fmt, int K, int stride) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        c->set_device();
+        do_hour_loop(c, fmt, K, stride);
+    });
+}
+
+int sgpu_hour_loop_graph(sgpu_ctx* c, unsigned fmt, int K, int replays) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        c->set_device();
+        do_hour_loop_graph(c, fmt, K, replays);
+    });
+}
+
+int sgpu_get_irradiance(sgpu_ctx* c, float* out) {
+    return guarded([&] {
+        need(c && out, SGPU_ERR_ARG, "null argument");
+        need(c->irr_ok, SGPU_ERR_STATE, "no irradiance computed");
+        c->set_device();
+        SGPU_HIP(hipMemcpyAsync(out, c->sum, c->n_total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int sgpu_hour_single(sgpu_ctx* c, unsigned fmt, int hour_index, float* hourly_out, float* refl_out) {
+    return guarded([&] {
+        need(c && hourly_out && refl_out, SGPU_ERR_ARG, "null argument");
+        check_stage2_ready(c, fmt);
+        c->set_device();
+        const HourConst* hc = nullptr;
+        for (const HourConst& h : c->hours)
+            if (h.index == hour_index) hc = &h;
+        need(hc != nullptr, SGPU_ERR_ARG, "hour is not a sunlit hour");
+        ensure_accumulators(c, 1);
+        ensure_hours_d(c, std::vector<HourConst>{*hc});
+        float* dh = dalloc<float>(c->n_total);
+        float* dr = dalloc<float>(c->n_total);
+        try {
+            HourArgs a = hour_args(c, fmt);
+            a.hours = c->hours_d;
+            a.nh = 1;
+            a.dbg_hourly = dh;
+            a.dbg_reflected = dr;
+            launch_solar_top(a, 1, c->stream);
+            launch_reflect_accum(a, 1, fmt, c->stream);
+            SGPU_HIP(hipMemcpyAsync(hourly_out, dh, c->n_total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+            SGPU_HIP(hipMemcpyAsync(refl_out, dr, c->n_total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            dfree(dh);
+            dfree(dr);
+            throw;
+        }
+        dfree(dh);
+        dfree(dr);
+    });
+}
+
+int sgpu_set_kernel_timing(sgpu_ctx* c, int enable) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        c->kernel_timing = enable != 0;
+    });
+}
+
+int sgpu_set_step_counting(sgpu_ctx* c, int enable) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        c->count_steps = enable != 0;
+    });
+}
+
+int sgpu_sync(sgpu_ctx* c) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        c->set_device();
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+static int whole_pipeline(unsigned fmt, const float* grid, int w, int h, int n_total, int n_top, const int32_t* pp,
+                          const float* wall_z, float maxz, const float* az_alt, const float* dni, const float* dhi,
+                          const float* slopes, const float* aspects, float* out) {
+    sgpu_ctx* c = nullptr;
+    int st = sgpu_create(0, &c);
+    if (st) return st;
+    st = sgpu_set_scene(c, grid, w, h, n_total, n_top, pp, wall_z, maxz, slopes, aspects);
+    if (!st) st = sgpu_set_sky(c, az_alt, dni, dhi);
+    if (!st) st = sgpu_viewshed(c, fmt);
+    if (!st) st = sgpu_hour_loop(c, fmt, 0, 1);
+    if (!st) st = sgpu_get_irradiance(c, out);
+    std::string keep = g_last_error;
+    sgpu_destroy(c);
+    g_last_error = keep;
+    return st;
+}
+
+int sgpu_uncompressed_irradiance(const float* grid, int w, int h, int n_total, int n_top, const int32_t* pp,
+                                 const float* wall_z, float maxz, const float* az_alt, const float* dni,
+                                 const float* dhi, const float* slopes, const float* aspects, float* out) {
+    return whole_pipeline(SGPU_FMT_CSR, grid, w, h, n_total, n_top, pp, wall_z, maxz, az_alt, dni, dhi, slopes,
+                          aspects, out);
+}
+
+int sgpu_compressed_irradiance(const float* grid, int w, int h, int n_total, int n_top, const int32_t* pp,
+                               const float* wall_z, float maxz, const float* az_alt, const float* dni,
+                               const float* dhi, const float* slopes, const float* aspects, float* out) {
+    return whole_pipeline(SGPU_FMT_BARLE, grid, w, h, n_total, n_top, pp, wall_z, maxz, az_alt, dni, dhi, slopes,
+                          aspects, out);
+}
+
+}  // extern "C"

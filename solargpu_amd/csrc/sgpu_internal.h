@@ -1,0 +1,141 @@
+/* Internal declarations of libsolargpu: host tables, kernel parameter blocks
+ * and launcher entry points shared by the host and HIP translation units. */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sgpu.h"
+
+namespace sgpu {
+
+constexpr int kHours = SGPU_HOURS_PER_YEAR;
+constexpr int kAzBins = 72;            // 360 / 5 deg  (sky_bins.cpp:13)
+constexpr int kElBins = 18;            // 90 / 5 deg   (sky_bins.cpp:14)
+constexpr int kDirs = kAzBins * kElBins * 2;  // 2592 ray directions (viewshed_stage.cuh:46)
+constexpr float kBinDeg = 5.0f;
+constexpr float kDegToRad = 0.0174532925f;    // pipeline_math_constants.h:15
+constexpr int kMaxHoursPerPass = 32;
+
+/* ---- errors: thrown inside the library, turned into status codes at the C ABI ---- */
+struct Failure : std::runtime_error {
+    int code;
+    Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const char* fmt, ...);
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define SGPU_HIP(expr) ::sgpu::hip_check((expr), #expr, __FILE__, __LINE__)
+
+/* ---- host-side contract (host_sky.cpp, host_voxel.cpp, host_synth.cpp) ---- */
+void sun_positions(float lon, float lat, int tz, float* az_alt);
+
+struct SkyTables {
+    std::vector<float> nudged;          // 2*8760, az/alt - 1e-8f (sky_bins.cpp:17-25)
+    std::vector<int32_t> full_to_dense; // 72*18
+    std::vector<int32_t> sun_dense;     // 8760
+    int active = 0;
+    int svf_bytes = 0;                  // ceil(active / 8)
+};
+SkyTables sky_bins(const float* az_alt);
+
+/* One ray direction, hour-invariant constants of the DDA (viewshed_kernels.cu:27-35). */
+struct alignas(16) DirConst {
+    float cx, cy, cz;
+    float ax, ay;   // fmaxf(fabsf(c), 1e-8f)
+    float rx, ry;   // 1.0f / a
+    int32_t info;   // bit 0: upper hemisphere; bits 8..: dense SVF bin + 1 (0 = none)
+};
+void directions(const SkyTables& sky, DirConst* out);   // kDirs entries, reference order
+
+/* Per sunlit-hour constants of the solar kernel (solar_kernel.cu:8-50, 54-77). */
+struct alignas(16) HourConst {
+    float cos_z, sin_z, azk, dni_eff;   // cos zenith, sin(acos(cos_z)), azimuth rad, fmaxf(DNI,1e-3f)
+    float dhi, b, f1, f2;               // DHI, max(cos 85deg, cos_z), Perez F1, F2
+    int32_t dense;                      // sun bin (dense SVF index) or -1
+    int32_t index;                      // day*24 + hour
+    int32_t eps_bin;
+    int32_t pad;
+};
+std::vector<HourConst> hour_constants(const SkyTables& sky, const float* dni, const float* dhi);
+
+struct Walls {
+    std::vector<int32_t> parent;
+    std::vector<float> z, aspect;
+};
+Walls detect_walls(const float* heights, int w, int h, float voxel);
+void top_slope_aspect(const float* heights, int w, int h, double sx, double sy, float* slope, float* aspect);
+
+void synth_dsm(uint64_t seed, int w, int h, float cell_m, float density, int trees, float* out);
+void synth_weather(uint64_t seed, const float* az_alt, float* dni, float* dhi);
+
+/* ---- device kernels (k_viewshed.hip, k_barle.hip, k_hourly.hip) ---- */
+struct TraceArgs {
+    const float* grid;
+    int W, H;
+    int64_t n_top;
+    const int32_t* pix_parent;
+    const float* wall_z;
+    float maxz;
+    const DirConst* dirs;
+    int64_t v0, count;      // voxel range of this chunk
+    int32_t* raw;           // count x kDirs first-hit ids (+1), 0 = none
+    unsigned long long* steps;  // optional DDA step counter
+};
+void launch_trace(const TraceArgs& a, hipStream_t s);
+
+struct ListArgs {
+    const int32_t* raw;
+    int64_t v0, count, n_total;
+    const DirConst* dirs;
+    int svf_bytes;
+    uint8_t* svf;           // word-major SVF bitmap (voxel_types.h:28-38)
+    int32_t* counts;        // pass 1 output
+    const int64_t* offsets; // pass 2 input (chunk-local, count+1)
+    int32_t* ids;           // pass 2 output
+};
+void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s);
+
+/* Exclusive scan of n int32 counts into n+1 int64 offsets (offsets[n] = total). */
+void launch_scan_counts(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* scratch_blocks,
+                        hipStream_t s);
+size_t scan_scratch_elems(int64_t n);
+
+struct BarleArgs {
+    const int64_t* csr_off;
+    const int32_t* csr_ids;
+    int64_t n;
+    int32_t* sizes;         // size pass output
+    const int64_t* out_off; // write pass input
+    uint8_t* out;           // write pass output
+};
+void launch_barle(const BarleArgs& a, bool write_pass, hipStream_t s);
+
+struct HourArgs {
+    const float* grid;
+    int W, H;
+    int64_t n_total, n_top;
+    const int32_t* pix_parent;
+    const float* wall_z;
+    const float* slopes;
+    const float* aspects;
+    const uint8_t* svf;
+    const HourConst* hours;    // this pass's hours (nh entries)
+    int nh;
+    float* hourly_top;         // n_top x K, cell-major
+    const int64_t* nb_off;     // CSR offsets or ba-RLE byte offsets
+    const int32_t* nb_ids;
+    const uint8_t* nb_bytes;
+    float* sum;                // Kahan annual sum
+    float* comp;               // Kahan compensation
+    float* dbg_hourly;         // optional single-hour outputs
+    float* dbg_reflected;
+};
+void launch_solar_top(const HourArgs& a, int K, hipStream_t s);
+void launch_reflect_accum(const HourArgs& a, int K, unsigned format, hipStream_t s);
+bool supported_hours_per_pass(int K);
+
+}  // namespace sgpu
