@@ -1,0 +1,164 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars: neighbour lists, SVF bitmaps and ba-RLE streams bit-exact; per-hour
+hourly/reflected irradiance within 2e-5 relative (device vs host libm);
+annual irradiance within 1e-4 relative of the FP64 oracle (floor 1 Wh/m^2).
+"""
+import numpy as np
+import pytest
+
+import solargpu_amd as sg
+import oracle_lib as orc
+from scenes import oracle_lists, sky, synth_scene, scene_from_heights
+
+pytestmark = pytest.mark.gpu
+
+ANNUAL_RTOL = 1e-4
+HOUR_RTOL = 2e-5
+
+
+def _ctx(sc, formats=sg.FMT_CSR | sg.FMT_BARLE, chunk=0):
+    az_alt, dni, dhi, _ = sky()
+    ctx = sg.Context(0)
+    ctx.set_scene(sc)
+    ctx.set_sky(az_alt, dni, dhi)
+    if chunk:
+        ctx.set_chunk_voxels(chunk)
+    if formats:
+        ctx.viewshed(formats)
+    return ctx
+
+
+def _rel(a, b, floor=1.0):
+    return np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), floor)
+
+
+@pytest.mark.parametrize("seed,w,h", [(11, 48, 40), (12, 64, 64), (13, 37, 53)])
+def test_viewshed_lists_svf_bit_exact(seed, w, h):
+    sc = synth_scene(seed, w, h)
+    ref = oracle_lists(seed, w, h)
+    with _ctx(sc) as ctx:
+        off, ids = ctx.csr()
+        svf = ctx.svf()
+        np.testing.assert_array_equal(off, ref["off"])
+        np.testing.assert_array_equal(ids, ref["ids"])
+        np.testing.assert_array_equal(svf, ref["svf"])
+
+
+def test_trace_raw_matches_oracle():
+    sc = synth_scene(12, 64, 64)
+    bins = sky()[3]
+    picks = [0, 1, 63, 64 * 32 + 17, sc.num_top - 1, sc.num_top, sc.num_total - 1]
+    with _ctx(sc, formats=0) as ctx:
+        for v in picks:
+            np.testing.assert_array_equal(ctx.trace_raw(v, 1), orc.trace_raw(sc, bins, v, 1), err_msg=f"voxel {v}")
+
+
+def test_barle_gpu_encoder_bit_exact():
+    sc = synth_scene(12, 64, 64)
+    ref = oracle_lists(12, 64, 64)
+    eoff, edata = orc.barle_encode_all(ref["off"], ref["ids"])
+    with _ctx(sc) as ctx:
+        boff, data = ctx.barle()
+        np.testing.assert_array_equal(boff, eoff)
+        np.testing.assert_array_equal(data, edata)
+
+
+def test_barle_encoder_edge_lists():
+    """Crafted lists through the GPU encoder (set_csr -> barle_encode) vs the oracle encoder."""
+    sc = synth_scene(11, 48, 40)
+    rng = np.random.default_rng(5)
+    lists = [[], [1], [15], [16], [255, 256], [70000, 3], list(range(600, 0, -1)),
+             [1 << 25, (1 << 25) - 1, 5, 4], list(range(1, 2592 * 3, 3))[:2592]]
+    for _ in range(sc.num_total - len(lists)):
+        n = int(rng.integers(0, 60))
+        lists.append(list(rng.choice(sc.num_top, size=n, replace=False) + 1))
+    lists = lists[: sc.num_total]
+    off = np.zeros(sc.num_total + 1, np.int64)
+    off[1:] = np.cumsum([len(x) for x in lists])
+    ids = np.concatenate([np.asarray(x, np.int32) for x in lists]) if off[-1] else np.zeros(0, np.int32)
+    eoff, edata = orc.barle_encode_all(off, ids)
+    with _ctx(sc, formats=0) as ctx:
+        ctx.set_csr(off, ids)
+        ctx.barle_encode()
+        boff, data = ctx.barle()
+    np.testing.assert_array_equal(boff, eoff)
+    np.testing.assert_array_equal(data, edata)
+
+
+def test_chunking_is_results_neutral():
+    sc = synth_scene(13, 37, 53)
+    ref = oracle_lists(13, 37, 53)
+    with _ctx(sc, chunk=777) as ctx:
+        assert ctx.info()["viewshed_chunks"] > 1
+        off, ids = ctx.csr()
+        np.testing.assert_array_equal(off, ref["off"])
+        np.testing.assert_array_equal(ids, ref["ids"])
+        np.testing.assert_array_equal(ctx.svf(), ref["svf"])
+        boff, data = ctx.barle()
+        eoff, edata = orc.barle_encode_all(ref["off"], ref["ids"])
+        np.testing.assert_array_equal(data, edata)
+
+
+@pytest.mark.parametrize("fmt", [sg.FMT_CSR, sg.FMT_BARLE])
+def test_single_hour_matches_oracle(fmt):
+    sc = synth_scene(11, 48, 40)
+    ref = oracle_lists(11, 48, 40)
+    az_alt, dni, dhi, bins = sky()
+    hours = orc.sunlit_hours(bins["nudged"])
+    boff, data = orc.barle_encode_all(ref["off"], ref["ids"])
+    with _ctx(sc) as ctx:
+        for hi in [hours[0], hours[len(hours) // 3], hours[len(hours) // 2], hours[-1]]:
+            hv, rv = ctx.hour_single(fmt, int(hi))
+            eh = orc.solar_hour(sc, bins, dni, dhi, ref["svf"], int(hi))
+            er = (orc.reflect_hour_csr(sc, ref["off"], ref["ids"], eh) if fmt == sg.FMT_CSR
+                  else orc.reflect_hour_barle(sc, boff, data, eh))
+            assert _rel(hv, eh, 1e-3).max() <= HOUR_RTOL, f"hourly, hour {hi}"
+            assert np.abs(rv - er).max() <= HOUR_RTOL * max(float(np.abs(er).max()), 1.0), f"reflected, hour {hi}"
+
+
+@pytest.mark.parametrize("fmt,K", [(sg.FMT_CSR, 1), (sg.FMT_CSR, 16), (sg.FMT_BARLE, 1), (sg.FMT_BARLE, 8),
+                                   (sg.FMT_BARLE, 32)])
+def test_annual_within_1e4_of_fp64(fmt, K):
+    sc = synth_scene(12, 64, 64)
+    ref = oracle_lists(12, 64, 64)
+    az_alt, dni, dhi, bins = sky()
+    exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
+    with _ctx(sc) as ctx:
+        ctx.hour_loop(fmt, K)
+        inf = ctx.info()
+        assert inf["hours_computed"] == len(orc.sunlit_hours(bins["nudged"]))
+        irr = ctx.irradiance()
+    assert _rel(irr, exact).max() <= ANNUAL_RTOL
+
+
+def test_graph_replays_match_eager_loop():
+    sc = synth_scene(11, 48, 40)
+    with _ctx(sc) as ctx:
+        ctx.hour_loop(sg.FMT_BARLE, 16)
+        eager = ctx.irradiance()
+        for _ in range(3):
+            ctx.hour_loop_graph(sg.FMT_BARLE, 16, replays=2)
+            np.testing.assert_array_equal(ctx.irradiance(), eager)
+            ctx.sync()
+
+
+def test_drop_in_entry_points():
+    sc = synth_scene(11, 48, 40)
+    ref = oracle_lists(11, 48, 40)
+    az_alt, dni, dhi, bins = sky()
+    exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
+    u = sg.uncompressed_gpu_irradiance(sc, az_alt, dni, dhi)
+    c = sg.compressed_gpu_irradiance(sc, az_alt, dni, dhi)
+    assert _rel(u, exact).max() <= ANNUAL_RTOL
+    assert _rel(c, exact).max() <= ANNUAL_RTOL
+
+
+def test_flat_plane_has_no_upper_hits():
+    """Flat ground: no upward ray can hit; the SVF is all clear; lists hold only ground cells."""
+    sc = scene_from_heights(np.full((32, 32), 205.0, np.float32))
+    with _ctx(sc) as ctx:
+        assert not ctx.svf().any()
+        off, ids = ctx.csr()
+        ref = orc.viewshed(sc, sky()[3])
+        np.testing.assert_array_equal(ids, ref["ids"])
