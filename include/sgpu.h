@@ -116,6 +116,8 @@ int sgpu_viewshed(sgpu_ctx* ctx, unsigned formats);
 int sgpu_get_svf(sgpu_ctx* ctx, uint8_t* out /* svf_bytes_per_voxel * num_total */);
 int sgpu_get_csr(sgpu_ctx* ctx, int64_t* offsets /* num_total+1 */, int32_t* ids /* nnz */);
 int sgpu_get_barle(sgpu_ctx* ctx, int64_t* offsets /* num_total+1 */, uint8_t* bytes);
+/* CSR lists of voxels [v0, v0+count): offsets rebased to 0 (count+1), ids up to cap (new). */
+int sgpu_get_csr_range(sgpu_ctx* ctx, int64_t v0, int64_t count, int64_t* offsets, int32_t* ids, int64_t cap);
 /* Raw first-hit ids (id+1, 0 = none) of voxels [v0, v0+count) x 2592 directions (new; tests). */
 int sgpu_trace_raw(sgpu_ctx* ctx, int64_t v0, int64_t count, int32_t* raw_out);
 /* Supply lists/SVF from elsewhere (tests, a persisted cache) instead of tracing. (new) */
@@ -142,6 +144,57 @@ int sgpu_set_kernel_timing(sgpu_ctx* ctx, int enable);
 int sgpu_set_step_counting(sgpu_ctx* ctx, int enable);
 /* Synchronise the context stream. */
 int sgpu_sync(sgpu_ctx* ctx);
+/* Plain device copy of `bytes` (float4 loads/stores), `reps` launches; *gbps = read+write
+   bytes / mean launch time.  Calibrates HBM counters and the measured copy bandwidth (new). */
+int sgpu_copy_bandwidth(sgpu_ctx* ctx, int64_t bytes, int reps, double* gbps);
+
+/* ---- per-voxel surrogate MLP (new; no reference counterpart) ----
+   64 inputs (48 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and
+   activations, FP32 accumulation on MFMA; Adam.  Parameters as one fp32 vector: per layer
+   W[out][in] then b[out]. */
+typedef struct sgpu_mlp sgpu_mlp;
+const char* sgpu_mlp_last_error(void);
+int sgpu_mlp_create(int device, uint64_t seed, sgpu_mlp** out);
+int sgpu_mlp_destroy(sgpu_mlp* mlp);
+int sgpu_mlp_param_count(void);
+int sgpu_mlp_get_params(sgpu_mlp* mlp, float* params);
+int sgpu_mlp_set_params(sgpu_mlp* mlp, const float* params);
+int sgpu_mlp_set_loss_weights(sgpu_mlp* mlp, float w0, float w1);
+/* X: n x 64 bf16 (host), Y: n x 2 (host). */
+int sgpu_mlp_forward(sgpu_mlp* mlp, const uint16_t* X, int64_t n, float* Y);
+/* X, Y device pointers; *ms = kernel time (HIP events). */
+int sgpu_mlp_forward_device(sgpu_mlp* mlp, const uint16_t* X, int64_t n, float* Y, double* ms);
+int sgpu_mlp_train(sgpu_mlp* mlp, const uint16_t* X, const float* T, int64_t n, int epochs, int batch, float lr,
+                   uint64_t seed, double* epoch_losses);
+/* Gradient of the mean weighted squared error over rows [0, B) (B multiple of 64). */
+int sgpu_mlp_gradient(sgpu_mlp* mlp, const uint16_t* X, const float* T, int B, float* grad, double* loss);
+
+/* ---- surrogate path (new; no reference counterpart) ----
+   The reflection pass replaced by the MLP where its predicted error of the annual total is
+   under gate->tau; elsewhere the exact plain result (2592-direction trace, ba-RLE lists,
+   per-hour reflection loop).  Direct and diffuse are always exact. */
+typedef struct sgpu_gate {
+    float tau;               /* admissible predicted relative error of the annual total */
+    float refl_rel_err;      /* calibrated relative error of the predicted reflected share */
+    float diff_miss_weight;  /* weight of the diffuse-share miss (out-of-distribution signal) */
+    float share_abs_err;     /* calibrated absolute error of the reflected share */
+} sgpu_gate;
+
+typedef struct sgpu_surrogate_stats {
+    int64_t fallback;        /* voxels sent to the exact path */
+    double t_svf_ms, t_dd_ms, t_features_ms, t_mlp_ms, t_fallback_ms, t_total_ms;
+} sgpu_surrogate_stats;
+
+/* Sun-bin SVF (identical bits to sgpu_viewshed's), exact annual direct + diffuse and the
+   feature matrix, on the device.  lat is a feature (site latitude, degrees). */
+int sgpu_surrogate_prepare(sgpu_ctx* ctx, float lat, sgpu_surrogate_stats* stats);
+/* Download features (N x 64 bf16), annual direct+diffuse and diffuse (any may be NULL). */
+int sgpu_surrogate_get(sgpu_ctx* ctx, uint16_t* X, float* dd, float* diff);
+/* Annual reflected irradiance of the plain method (needs the neighbour lists). */
+int sgpu_surrogate_labels(sgpu_ctx* ctx, unsigned format, float* reflected);
+/* The whole surrogate path; out: annual totals (N), fallback: per-voxel flag (N, optional). */
+int sgpu_surrogate_run(sgpu_ctx* ctx, sgpu_mlp* mlp, const sgpu_gate* gate, int hours_per_pass, float* out,
+                       int32_t* fallback, sgpu_surrogate_stats* stats);
 
 /* ---- drop-in whole pipelines (solar_gpu_api.h:37-54) ----
    sun_az_alt: 2*8760 floats; dni/dhi: 8760 floats each. */

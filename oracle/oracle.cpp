@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -823,6 +824,49 @@ int orc_annual_fp32(const float* grid, int w, int h, int n_total, int n_top, con
         }
         ++done;
     }
+    return done;
+}
+
+int orc_annual_fp32_range(const float* grid, int w, int h, int n_total, int n_top, const int* pp,
+                          const float* wall_z, const float* slopes, const float* aspects, const float* naa,
+                          const int* sdi, const float* dni, const float* dhi, const uint8_t* svf, int64_t v0,
+                          int64_t count, const int64_t* off, const int32_t* ids, int stride, int nthreads,
+                          float* out, double* t_top_s, double* t_range_s) {
+    std::vector<int> hs = sunlit(naa);
+    std::vector<float> hourly(n_top), comp(count, 0.0f);
+    double tt = 0.0, tr = 0.0;
+    std::fill(out, out + count, 0.0f);
+    if (stride < 1) stride = 1;
+    int done = 0;
+    for (size_t q = 0; q < hs.size(); q += stride) {
+        const int idx = hs[q];
+        auto t0 = std::chrono::steady_clock::now();
+        orc_solar_hour(naa, sdi, dni, dhi, idx, n_top, slopes, aspects, svf, n_total, nthreads, hourly.data());
+        auto t1 = std::chrono::steady_clock::now();
+        const float az = naa[2 * idx], alt = naa[2 * idx + 1];
+        const int di = sdi[idx], day = idx / 24;
+        parallel_blocks(count, 1024, nthreads, [&](int64_t a, int64_t b, int) {
+            for (int64_t k = a; k < b; ++k) {
+                const int64_t t = v0 + k;
+                bool blk = (di >= 0) ? svf_bit(svf, t, di, n_total) : false;
+                float hv = solar_voxel(az, alt, di, dni[idx], dhi[idx], day, slopes[t], aspects[t], blk);
+                Receiver r = receiver(t, w, n_top, pp, grid, wall_z, slopes, aspects);
+                float R = 0.0f;
+                for (int64_t e = off[k]; e < off[k + 1]; ++e) add_reflection(ids[e] - 1, r, grid, w, h, hourly.data(), R);
+                float add = R + hv;
+                float y = add - comp[k];
+                float s = out[k] + y;
+                comp[k] = (s - out[k]) - y;
+                out[k] = s;
+            }
+        });
+        auto t2 = std::chrono::steady_clock::now();
+        tt += std::chrono::duration<double>(t1 - t0).count();
+        tr += std::chrono::duration<double>(t2 - t1).count();
+        ++done;
+    }
+    if (t_top_s) *t_top_s = tt;
+    if (t_range_s) *t_range_s = tr;
     return done;
 }
 

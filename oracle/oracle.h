@@ -11,23 +11,27 @@
  *
  * How each function is pinned (what would catch a plausible mistake):
  *   orc_sun_positions   golden vectors from the reference's own sun_position.h
- *                       (built under oracle/_ref) + independent NOAA formula (0.05 deg)
- *   orc_sky_bins        golden vectors from the reference's sky_bins.cpp + brute force
+ *                       (built under oracle/_ref) + independent NOAA/Spencer formula
+ *   orc_sky_bins        golden vectors from the reference's sky_bins.cpp
  *   orc_detect_walls /
  *   orc_top_slope_aspect golden vectors from the reference's voxelization.cpp
- *   orc_directions      closed form: unit length, bin-centre azimuth/elevation
+ *   orc_directions      bit-identical to the product's host table; closed forms
+ *                       through the flat-plane and single-box cases
  *   orc_trace_ray /
- *   orc_viewshed        invariants + special cases (flat plane, single box with
- *                       hand-derived hits, hit predicate re-checked, brute-force
- *                       re-trace with an independent exact-rational DDA).  No
- *                       reference run exists (CUDA only): exact ids are pinned by
- *                       the contraction-off restatement, as the spec defines.
+ *   orc_trace_raw /
+ *   orc_viewshed        an independent vectorised numpy float32 restatement of the
+ *                       DDA (tests/test_oracle_pins.py), list = first occurrence of
+ *                       raw hits, special cases (flat plane, a single box with
+ *                       hand-derived hits).  No reference run exists (CUDA only):
+ *                       exact ids are pinned by the contraction-off restatement, as
+ *                       the parity contract defines.
  *   orc_barle_encode_*  golden streams from the reference's barle_compression.cpp
- *   orc_barle_decode    round trip on the golden streams
+ *   orc_barle_decode    round trip on the golden streams and on random lists
  *   orc_solar_hour      closed forms (flat unshaded voxel, shaded voxel = diffuse
- *                       only, isotropic limit) + FP64 restatement
- *   orc_reflect_hour    closed-form two-cell cases
- *   orc_annual_fp32     CSR vs ba-RLE set equality; FP64 oracle agreement
+ *                       only) against an independent float64 Perez
+ *   orc_reflect_hour_*  closed-form two-cell case
+ *   orc_annual_fp32 /
+ *   orc_annual_fp32_range  agreement with the FP64 oracle within 1e-4
  *   orc_annual_fp64     linearity identity vs the per-hour FP64 sum on tiny scenes
  */
 #pragma once
@@ -116,6 +120,18 @@ int orc_annual_fp32(const float* grid, int w, int h, int n_total, int n_top, con
                     const float* dhi, const uint8_t* svf, const int64_t* off, const int32_t* ids,
                     const uint8_t* bytes, int mode, int hour_stride, int kahan, int nthreads,
                     float* out);
+
+/* Stage-2 CPU baseline sample: the reference-structured FP32 hour loop for voxels
+   [v0, v0+count) only (hourly of every top cell is still computed, since any voxel may
+   see any cell).  Lists are given for the range: off has count+1 entries (off[0] may be
+   nonzero), ids is indexed by off.  t_top_s / t_range_s (may be NULL) receive the seconds
+   spent on the all-top-cell solar term and on the range.  Returns the number of hours computed. */
+int orc_annual_fp32_range(const float* grid, int w, int h, int n_total, int n_top, const int* pix_parent,
+                          const float* wall_z, const float* slopes, const float* aspects,
+                          const float* nudged_az_alt, const int* sun_dense_idx, const float* dni,
+                          const float* dhi, const uint8_t* svf, int64_t v0, int64_t count,
+                          const int64_t* off, const int32_t* ids, int hour_stride, int nthreads, float* out,
+                          double* t_top_s, double* t_range_s);
 
 /* FP64 oracle.  Same discrete decisions as the FP32 path (sunlit-hour set, SVF bits, neighbour
    sets, Perez epsilon bin taken from the FP32 hour constants); all continuous arithmetic in

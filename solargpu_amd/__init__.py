@@ -269,6 +269,13 @@ class Context:
         _check(_lib.sgpu_get_csr(self._h, _ptr(off), _ptr(ids)))
         return off, ids[: inf["nnz"]]
 
+    def csr_range(self, v0: int, count: int) -> tuple[np.ndarray, np.ndarray]:
+        off = np.zeros(count + 1, np.int64)
+        cap = count * NDIRS
+        ids = np.zeros(max(cap, 1), np.int32)
+        _check(_lib.sgpu_get_csr_range(self._h, _I64(v0), _I64(count), _ptr(off), _ptr(ids), _I64(cap)))
+        return off, ids[: off[-1]].copy()
+
     def barle(self) -> tuple[np.ndarray, np.ndarray]:
         inf = self.info()
         off = np.zeros(inf["num_total"] + 1, np.int64)
@@ -319,6 +326,12 @@ class Context:
 
     def sync(self) -> None:
         _check(_lib.sgpu_sync(self._h))
+
+    def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:
+        """GB/s (read + write) of a plain float4 device copy of nbytes."""
+        g = ctypes.c_double()
+        _check(_lib.sgpu_copy_bandwidth(self._h, _I64(nbytes), reps, ctypes.byref(g)))
+        return g.value
 
 
 def _whole(fn, sc: Scene, az_alt, dni, dhi) -> np.ndarray:

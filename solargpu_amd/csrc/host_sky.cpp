@@ -151,12 +151,14 @@ std::vector<HourConst> hour_constants(const SkyTables& sky, const float* dni, co
             c.cos_z = cosf((90.0f - alt) * kDegToRad);
             const float za = acosf(c.cos_z);
             c.sin_z = sinf(za);
-            c.azk = az * kDegToRad;
+            const float azk = az * kDegToRad;
+            c.cos_az = cosf(azk);
+            c.sin_az = sinf(azk);
             c.dni_eff = fmaxf(dir, 1e-3f);
             c.dhi = dif;
             const float z_deg = za * (180.0f / pi_f);
             const float am = 1.0f / (c.cos_z + 0.50572f * powf(96.07995f - z_deg, -1.6364f));
-            c.b = fmaxf(0.08715574f, c.cos_z);
+            c.inv_b = 1.0f / fmaxf(0.08715574f, c.cos_z);
             float eps = ((dif + dir / cosf(za)) / dif + 1.041f * (za * za * za)) / (1.0f + 1.041f * za * za * za);
             const float B = 2 * pi_f * (day / 365.0f);
             const float e0 = 1367 * (1.00011f + 0.034221f * cosf(B) + 0.00128f * sinf(B) + 0.000719f * cosf(2 * B) +

@@ -65,6 +65,7 @@ void synth_dsm(uint64_t seed, int w, int h, float cell_m, float density, int tre
     const double span = (hi > lo) ? hi - lo : 1.0;
     std::vector<double> z((size_t)w * h);
     for (size_t i = 0; i < z.size(); ++i) z[i] = 200.0 + 20.0 * (g[i] - lo) / span;
+    const std::vector<double> ground = z;
     std::vector<char> bldg((size_t)w * h, 0);
     const int64_t target = (int64_t)((double)density * w * h);
     int64_t covered = 0;
@@ -75,7 +76,7 @@ void synth_dsm(uint64_t seed, int w, int h, float cell_m, float density, int tre
         const int x0 = (int)(rng.uni() * w) - sx / 2, y0 = (int)(rng.uni() * h) - sy / 2;
         const double height = rng.uni(5.0, 40.0);
         const int cx = std::min(std::max(x0 + sx / 2, 0), w - 1), cy = std::min(std::max(y0 + sy / 2, 0), h - 1);
-        const double roof = z[(size_t)cy * w + cx] + height;
+        const double roof = ground[(size_t)cy * w + cx] + height;
         for (int y = std::max(y0, 0); y < std::min(y0 + sy, h); ++y)
             for (int x = std::max(x0, 0); x < std::min(x0 + sx, w); ++x) {
                 size_t i = (size_t)y * w + x;

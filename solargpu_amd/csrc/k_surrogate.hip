@@ -1,0 +1,225 @@
+/* Surrogate path kernels on gfx950 (no reference counterpart):
+ *   k_sun_svf    the SVF bitmap from the ~265 sun-bin directions only (same DDA
+ *                as the plain tracer, so the bits are identical), lane = voxel;
+ *   k_annual_dd  exact annual direct + diffuse per voxel (FP32 Kahan over all
+ *                sunlit hours) and the diffuse part alone -- the solar kernel
+ *                of solar_kernel.cu:54-77 without the reflection term;
+ *   k_features   32-sector horizon march + the 48 features of the MLP, bf16;
+ *   k_gate       surrogate estimate, predicted error, fallback flags;
+ *   k_combine    scatter of the exact fallback totals. */
+#include <cmath>
+
+#include "dda.h"
+#include "sgpu_internal.h"
+#include "sgpu_surrogate.h"
+
+namespace sgpu {
+namespace {
+
+__device__ __forceinline__ uint16_t f2bf(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ void voxel_origin(const SurArgs& a, int64_t v, int& top, int& gx, int& gy, float& z) {
+    top = a.pix_parent[v];
+    gx = top % a.W;
+    gy = top / a.W;
+    z = (v < a.n_top) ? a.grid[top] : a.wall_z[v - a.n_top];
+}
+
+__global__ __launch_bounds__(256) void k_sun_svf(SurArgs a) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= a.n_total) return;
+    int top, gx, gy;
+    float z;
+    voxel_origin(a, v, top, gx, gy, z);
+    unsigned long long steps = 0;
+    int hits = 0;
+    for (int byte = 0; byte < a.svf_bytes; ++byte) {
+        unsigned bits = 0;
+        for (int bit = 0; bit < 8; ++bit) {
+            const int dense = byte * 8 + bit;
+            if (dense >= a.n_sun_dirs) break;
+            const DirConst dc = a.dirs[a.sun_dirs[dense]];   // wave-uniform
+            if (dda_first_hit<false>(a.grid, a.W, (unsigned)a.W, (unsigned)a.H, a.maxz, (float)gx, (float)gy, z, dc,
+                                     steps)) {
+                bits |= 1u << bit;
+                ++hits;
+            }
+        }
+        a.svf[(int64_t)byte * a.n_total + v] = (uint8_t)bits;
+    }
+    a.sun_hits[v] = (uint16_t)hits;
+}
+
+__global__ __launch_bounds__(256) void k_annual_dd(SurArgs a) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= a.n_total) return;
+    float ss, cs, sa, ca;
+    sincosf(a.slopes[v], &ss, &cs);
+    sincosf(a.aspects[v], &sa, &ca);
+    float s = 0.0f, c = 0.0f, sd = 0.0f, cd = 0.0f;
+    for (int h = 0; h < a.n_hours; ++h) {
+        const HourConst& hc = a.hours[h];   // wave-uniform
+        const float cdiff = hc.cos_az * ca + hc.sin_az * sa;
+        const float cos_aoi = hc.cos_z * cs + hc.sin_z * ss * cdiff;
+        bool shaded = false;
+        if (hc.dense >= 0) shaded = (a.svf[(int64_t)(hc.dense >> 3) * a.n_total + v] >> (hc.dense & 7)) & 1;
+        float dir = 0.0f;
+        if (cos_aoi > 0.0f && hc.dense >= 0 && !shaded) dir = fmaxf(0.0f, hc.dni_eff * cos_aoi);
+        const float aa = fmaxf(0.0f, cos_aoi);
+        const float dif = fmaxf(0.0f, hc.dhi * ((1 - hc.f1) * ((1 + cs) / 2) + hc.f1 * (aa * hc.inv_b) + hc.f2 * ss));
+        const float x = dir + dif;
+        float y = x - c, t = s + y;
+        c = (t - s) - y;
+        s = t;
+        y = dif - cd;
+        t = sd + y;
+        cd = (t - sd) - y;
+        sd = t;
+    }
+    a.dd[v] = s;
+    a.diff[v] = sd;
+}
+
+/* Horizon of one azimuth sector: march cell by cell from the cell centre along
+ * (sin az, -cos az) (azimuth clockwise from north = -y, as the ray directions),
+ * tracking the steepest tan(elevation) = (h - z0) / distance (distance in cells,
+ * heights in metres, the geometry of the viewshed rays).  Stops at the grid edge,
+ * at 1024 cells, or when even the highest cell could no longer raise it. */
+__device__ __forceinline__ void horizon(const SurArgs& a, int gx, int gy, float z0, float sx, float sy,
+                                        float& best, int& best_cell) {
+    const float ox = gx + 0.5f, oy = gy + 0.5f;
+    int ix = gx, iy = gy;
+    const float ax = fmaxf(fabsf(sx), 1e-8f), ay = fmaxf(fabsf(sy), 1e-8f);
+    float tmx = ((sx >= 0.0f) ? (ix + 1.0f - ox) : (ox - ix)) / ax;
+    float tmy = ((sy >= 0.0f) ? (iy + 1.0f - oy) : (oy - iy)) / ay;
+    const float rx = 1.0f / ax, ry = 1.0f / ay;
+    const int stx = (sx >= 0.0f) ? 1 : -1, sty = (sy >= 0.0f) ? 1 : -1;
+    best = -1e30f;
+    best_cell = -1;
+    const float top = a.maxz - z0;
+    for (int k = 0; k < 2048; ++k) {
+        float t;
+        if (tmx < tmy) { t = tmx; ix += stx; tmx += rx; }
+        else { t = tmy; iy += sty; tmy += ry; }
+        if ((unsigned)ix >= (unsigned)a.W || (unsigned)iy >= (unsigned)a.H || t > 1024.0f) break;
+        const int cell = ix + iy * a.W;
+        const float tn = (a.grid[cell] - z0) / fmaxf(t, 0.5f);
+        if (tn > best) { best = tn; best_cell = cell; }
+        if (top <= best * t) break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_features(SurArgs a) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= a.n_total) return;
+    int top, gx, gy;
+    float z;
+    voxel_origin(a, v, top, gx, gy, z);
+    float ss, cs, sa, ca;
+    sincosf(a.slopes[v], &ss, &cs);
+    sincosf(a.aspects[v], &sa, &ca);
+    uint16_t* row = a.X + v * kSurInPad;
+    float svf_h = 0.0f;
+    float quad[4] = {0.f, 0.f, 0.f, 0.f};
+    const float aspect = a.aspects[v];
+    for (int s = 0; s < kSectors; ++s) {
+        const float az = (s + 0.5f) * (6.28318531f / kSectors);
+        float sx, cy;
+        sincosf(az, &sx, &cy);
+        float best;
+        int cell;
+        horizon(a, gx, gy, z, sx, -cy, best, cell);
+        const float ang = (cell >= 0) ? atanf(best) : 0.0f;
+        row[s] = f2bf(ang * 0.63661977f);
+        const float se = sinf(fmaxf(ang, 0.0f));
+        svf_h += se * se;
+        if (cell >= 0 && ang > 0.0f) {
+            /* quadrant of the sector relative to the facing direction: 0 front, 1 right, 2 back, 3 left */
+            float rel = az - aspect;
+            rel -= 6.28318531f * floorf(rel * 0.15915494f);
+            const int q = (int)((rel + 0.78539816f) * 0.63661977f) & 3;
+            quad[q] += a.dd[cell] * se;
+        }
+    }
+    const bool wall = v >= a.n_top;
+    const float dd = a.dd[v];
+    row[32] = f2bf(ss * sa);
+    row[33] = f2bf(-ss * ca);
+    row[34] = f2bf(cs);
+    row[35] = f2bf((a.maxz - z) * 0.02f);
+    row[36] = f2bf(log1pf((float)a.sun_hits[v]) * 0.179f);
+    row[37] = f2bf(1.0f - svf_h / kSectors);
+    row[38] = f2bf(a.lat / 90.0f);
+    row[39] = f2bf(a.albedo);
+    row[40] = f2bf(wall ? 1.0f : 0.0f);
+    row[41] = f2bf(a.slopes[v] * 0.63661977f);
+    row[42] = f2bf(dd * 1e-6f);
+    row[43] = f2bf(dd > 0.0f ? a.diff[v] / dd : 0.0f);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) row[44 + q] = f2bf(quad[q] * (1e-6f / 8.0f));
+#pragma unroll
+    for (int f = kSurFeatures; f < kSurInPad; ++f) row[f] = 0;
+}
+
+/* Surrogate total = exact direct + diffuse + predicted reflected share x (direct + diffuse).
+ * Predicted error of the total: the calibrated relative error of the reflected share
+ * times the predicted reflected fraction of the total, plus the miss on the diffuse share
+ * (known exactly here; a large miss marks an input the network does not represent). */
+__global__ __launch_bounds__(256) void k_gate(SurArgs a) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= a.n_total) return;
+    const float dd = a.dd[v];
+    const float share = fmaxf(a.Y[2 * v], 0.0f);
+    const float refl = share * dd;
+    const float total = dd + refl;
+    const float exact_dshare = dd > 0.0f ? a.diff[v] / dd : 0.0f;
+    const float dmiss = fabsf(a.Y[2 * v + 1] - exact_dshare);
+    const float frac = total > 0.0f ? refl / total : 0.0f;
+    const float err = a.refl_rel_err * frac + a.diff_miss_weight * dmiss + a.share_abs_err * dd / fmaxf(total, 1e-3f);
+    a.out[v] = total;
+    a.fallback[v] = (err > a.tau || !(total > 0.0f)) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const int32_t* ids, const float* vals, int64_t n, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[ids[i]] = vals[i];
+}
+
+__global__ __launch_bounds__(256) void k_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v < n && flags[v]) ids[off[v]] = (int32_t)v;
+}
+
+}  // namespace
+
+void launch_sun_svf(const SurArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_sun_svf, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);
+    SGPU_HIP(hipGetLastError());
+}
+void launch_annual_dd(const SurArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_annual_dd, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);
+    SGPU_HIP(hipGetLastError());
+}
+void launch_features(const SurArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);
+    SGPU_HIP(hipGetLastError());
+}
+void launch_gate(const SurArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_gate, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);
+    SGPU_HIP(hipGetLastError());
+}
+void launch_scatter(const int32_t* ids, const float* vals, int64_t n, float* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ids, vals, n, out);
+    SGPU_HIP(hipGetLastError());
+}
+void launch_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, off, n, ids);
+    SGPU_HIP(hipGetLastError());
+}
+
+}  // namespace sgpu

@@ -17,6 +17,7 @@
  * the library is compiled with -ffp-contract=off and IEEE division. */
 #include <climits>
 
+#include "dda.h"
 #include "sgpu_internal.h"
 
 namespace sgpu {
@@ -32,7 +33,7 @@ template <bool COUNT_STEPS>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     const int64_t local = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
     if (local >= a.count) return;
-    const int64_t v = a.v0 + local;
+    const int64_t v = a.subset ? (int64_t)a.subset[a.v0 + local] : a.v0 + local;
     const int top = a.pix_parent[v];
     const int gx = top % a.W, gy = top / a.W;
     const float base_z = (v < a.n_top) ? a.grid[gx + gy * a.W] : a.wall_z[v - a.n_top];
@@ -44,29 +45,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 
     for (int k = 0; k < kDirsPerGroup; ++k) {
         const DirConst dc = a.dirs[blockIdx.y * kDirsPerGroup + k];   // wave-uniform
-        const bool upper = dc.info & 1;
-        const float ox = gxf + dc.cx * 2.0f;
-        const float oy = gyf + dc.cy * 2.0f;
-        const float oz = base_z + dc.cz * 2.0f;
-        int ix = (int)floorf(ox);
-        int iy = (int)floorf(oy);
-        float tmx = ((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix)) / dc.ax;
-        float tmy = ((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy)) / dc.ay;
-        const int sx = (dc.cx >= 0.0f) ? 1 : -1;
-        const int sy = (dc.cy >= 0.0f) ? 1 : -1;
-        int hit = 0;
-        for (;;) {
-            float t;
-            if (tmx < tmy) { t = tmx; ix += sx; tmx += dc.rx; }
-            else { t = tmy; iy += sy; tmy += dc.ry; }
-            if (COUNT_STEPS) ++steps;
-            if ((unsigned)ix >= W || (unsigned)iy >= H) break;
-            const float zr = oz + dc.cz * t;
-            if (upper ? (zr > a.maxz) : (zr < 0.0f)) break;
-            const int cell = ix + iy * a.W;
-            if (grid[cell] - zr > 0.01f) { hit = cell + 1; break; }
-        }
-        out[k] = hit;
+        out[k] = dda_first_hit<COUNT_STEPS>(grid, a.W, W, H, a.maxz, gxf, gyf, base_z, dc, steps);
     }
     if (COUNT_STEPS) atomicAdd(a.steps, steps);
 }
@@ -133,7 +112,7 @@ __global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
             }
             slot[k] = (int)s;
             atomicMin(&mins[s], d);
-            if (!WRITE) {
+            if (!WRITE && a.svf) {
                 const int dense = (a.dirs[d].info >> 8) - 1;
                 if (dense >= 0) atomicOr(&svf_acc[dense >> 3], 1u << (dense & 7));
             }
@@ -150,8 +129,9 @@ __global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
     const int excl = block_excl_scan(kept, tmp, &total);
     if (!WRITE) {
         if (tid == 0) a.counts[local] = total;
-        for (int i = tid; i < a.svf_bytes; i += kListThreads)
-            a.svf[(int64_t)i * a.n_total + v] = (uint8_t)svf_acc[i];
+        if (a.svf)
+            for (int i = tid; i < a.svf_bytes; i += kListThreads)
+                a.svf[(int64_t)i * a.n_total + v] = (uint8_t)svf_acc[i];
     } else {
         int32_t* dst = a.ids + a.offsets[local] + excl;
 #pragma unroll

@@ -10,61 +10,19 @@
 #include <string>
 #include <vector>
 
+#include "sgpu_ctx.h"
 #include "sgpu_internal.h"
 
 using namespace sgpu;
 
+namespace sgpu {
+std::string& last_error() {
+    thread_local std::string msg;
+    return msg;
+}
+}  // namespace sgpu
+
 namespace {
-
-thread_local std::string g_last_error;
-
-template <typename F>
-int guarded(F&& fn) {
-    try {
-        fn();
-        return SGPU_OK;
-    } catch (const Failure& f) {
-        g_last_error = f.what();
-        return f.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return SGPU_ERR_OOM;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return SGPU_ERR_HIP;
-    }
-}
-
-template <typename T>
-T* dalloc(size_t n) {
-    void* p = nullptr;
-    if (n == 0) n = 1;
-    SGPU_HIP(hipMalloc(&p, n * sizeof(T)));
-    return static_cast<T*>(p);
-}
-
-template <typename T>
-void dfree(T*& p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-struct EventPair {
-    hipEvent_t a = nullptr, b = nullptr;
-    EventPair() {
-        SGPU_HIP(hipEventCreate(&a));
-        SGPU_HIP(hipEventCreate(&b));
-    }
-    ~EventPair() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-    }
-    float ms() const {
-        float t = 0.0f;
-        SGPU_HIP(hipEventElapsedTime(&t, a, b));
-        return t;
-    }
-};
 
 __global__ void k_add_base(const int64_t* src, int64_t n, int64_t base, int64_t* dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,92 +34,16 @@ __global__ void k_fill_zero(float* p, int64_t n) {
     if (i < n) p[i] = 0.0f;
 }
 
-}  // namespace
-
-struct sgpu_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-
-    int W = 0, H = 0;
-    int64_t n_total = 0, n_top = 0;
-    float maxz = 0.0f;
-    float* grid = nullptr;
-    int32_t* pix_parent = nullptr;
-    float* wall_z = nullptr;
-    float* slopes = nullptr;
-    float* aspects = nullptr;
-    bool scene = false;
-
-    bool sky_set = false;
-    SkyTables sky;
-    std::vector<HourConst> hours;
-    DirConst* dirs = nullptr;
-
-    uint8_t* svf = nullptr;
-    bool svf_ok = false;
-    int64_t* csr_off = nullptr;
-    int32_t* csr_ids = nullptr;
-    int64_t nnz = -1;
-    int64_t* br_off = nullptr;
-    uint8_t* br_bytes = nullptr;
-    int64_t br_size = -1;
-
-    float* sum = nullptr;
-    float* comp = nullptr;
-    float* hourly_top = nullptr;
-    size_t hourly_top_elems = 0;
-    HourConst* hours_d = nullptr;
-    size_t hours_d_cap = 0;
-    bool irr_ok = false;
-
-    int64_t chunk_override = 0;
-    bool kernel_timing = false;
-    bool count_steps = false;
-    sgpu_info info{};
-
-    void set_device() const { SGPU_HIP(hipSetDevice(device)); }
-
-    void drop_products() {
-        dfree(svf);
-        dfree(csr_off);
-        dfree(csr_ids);
-        dfree(br_off);
-        dfree(br_bytes);
-        svf_ok = false;
-        nnz = -1;
-        br_size = -1;
-        irr_ok = false;
-    }
-    void drop_scene() {
-        dfree(grid);
-        dfree(pix_parent);
-        dfree(wall_z);
-        dfree(slopes);
-        dfree(aspects);
-        dfree(sum);
-        dfree(comp);
-        dfree(hourly_top);
-        hourly_top_elems = 0;
-        scene = false;
-    }
-    ~sgpu_ctx() {
-        if (device >= 0) (void)hipSetDevice(device);
-        drop_products();
-        drop_scene();
-        dfree(dirs);
-        dfree(hours_d);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-namespace {
-
-void need(bool ok, int code, const char* msg) {
-    if (!ok) fail(code, "%s", msg);
+/* Grid-stride float4 copy: the calibration kernel for HBM counters. */
+__global__ __launch_bounds__(256) void k_copy_f4(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                  int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
-int64_t choose_chunk(const sgpu_ctx* c) {
-    if (c->chunk_override > 0) return std::min<int64_t>(c->chunk_override, c->n_total);
+int64_t choose_chunk(const sgpu_ctx* c, int64_t n) {
+    if (n <= 0) return 1;
+    if (c->chunk_override > 0) return std::min<int64_t>(c->chunk_override, n);
     /* Raw first-hit table + list counters per voxel; keep 40% of free memory as the
        ceiling (the reference's budget, gpu_memory_budget.cuh:19-35), capped at 2^31
        table entries per chunk. */
@@ -171,7 +53,17 @@ int64_t choose_chunk(const sgpu_ctx* c) {
     int64_t chunk = (int64_t)(0.4 * (double)free_b / per_voxel);
     chunk = std::min<int64_t>(chunk, (int64_t)1 << 20 << 2);
     chunk = std::max<int64_t>(chunk, 1);
-    return std::min<int64_t>(chunk, c->n_total);
+    return std::min<int64_t>(chunk, n);
+}
+
+}  // namespace
+
+namespace sgpu {
+
+void zero_floats(float* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    SGPU_HIP(hipGetLastError());
 }
 
 void ensure_hours_d(sgpu_ctx* c, const std::vector<HourConst>& hs) {
@@ -195,6 +87,10 @@ void ensure_accumulators(sgpu_ctx* c, int K) {
         c->hourly_top_elems = need_elems;
     }
 }
+
+}  // namespace sgpu
+
+namespace {
 
 HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
     HourArgs a{};
@@ -260,45 +156,46 @@ void run_hour_passes(sgpu_ctx* c, unsigned fmt, int K, int nsel, std::vector<Eve
 }
 
 void zero_accumulators(sgpu_ctx* c) {
-    const unsigned g = (unsigned)((c->n_total + 255) / 256);
-    hipLaunchKernelGGL(k_fill_zero, dim3(g), dim3(256), 0, c->stream, c->sum, c->n_total);
-    hipLaunchKernelGGL(k_fill_zero, dim3(g), dim3(256), 0, c->stream, c->comp, c->n_total);
-    SGPU_HIP(hipGetLastError());
+    zero_floats(c->sum, c->n_total, c->stream);
+    zero_floats(c->comp, c->n_total, c->stream);
 }
 
-void do_viewshed(sgpu_ctx* c, unsigned formats) {
-    need(c->scene, SGPU_ERR_STATE, "scene not set");
-    need(c->sky_set, SGPU_ERR_STATE, "sky not set");
-    need((formats & ~(SGPU_FMT_CSR | SGPU_FMT_BARLE)) == 0 && formats != 0, SGPU_ERR_ARG, "bad formats");
-    c->drop_products();
-    const int64_t N = c->n_total;
-    const int64_t chunk = choose_chunk(c);
+/* Lists of N voxels (voxel ids 0..N-1, or subset[0..N) when given) in out-of-core
+ * chunks: trace -> count pass (+ SVF bytes when svf) -> int64 scan -> write pass.
+ * Each chunk's compact ids land in their own buffer; one chunk keeps it, several are
+ * concatenated.  Offsets are int64, indexed by position. */
+struct ListsOut {
+    int64_t* off = nullptr;
+    int32_t* ids = nullptr;
+    int64_t nnz = 0;
+};
+
+ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf, unsigned long long* steps) {
+    const int64_t chunk = choose_chunk(c, N);
     c->info.chunk_voxels = chunk;
     c->info.viewshed_chunks = (int32_t)((N + chunk - 1) / chunk);
-    c->info.t_trace_ms = c->info.t_lists_ms = 0.0;
-    c->info.dda_steps = 0;
-
-    EventPair all, ev_trace, ev_lists;
-    SGPU_HIP(hipEventRecord(all.a, c->stream));
-    c->svf = dalloc<uint8_t>((size_t)std::max(c->sky.svf_bytes, 1) * N);
-    c->csr_off = dalloc<int64_t>(N + 1);
-    int32_t* raw = dalloc<int32_t>((size_t)chunk * kDirs);
-    int32_t* counts = dalloc<int32_t>(chunk);
-    int64_t* off = dalloc<int64_t>(chunk + 1);
-    int64_t* scratch = dalloc<int64_t>(scan_scratch_elems(chunk));
-    unsigned long long* steps = c->count_steps ? dalloc<unsigned long long>(1) : nullptr;
-    if (steps) SGPU_HIP(hipMemsetAsync(steps, 0, sizeof(unsigned long long), c->stream));
+    EventPair ev_trace, ev_lists;
+    ListsOut out;
+    out.off = dalloc<int64_t>(N + 1);
+    int32_t* raw = nullptr;
+    int32_t* counts = nullptr;
+    int64_t* off = nullptr;
+    int64_t* scratch = nullptr;
     std::vector<std::pair<int32_t*, int64_t>> pieces;
     int64_t base = 0;
     try {
+        raw = dalloc<int32_t>((size_t)chunk * kDirs);
+        counts = dalloc<int32_t>(chunk);
+        off = dalloc<int64_t>(chunk + 1);
+        scratch = dalloc<int64_t>(scan_scratch_elems(chunk));
         for (int64_t v0 = 0; v0 < N; v0 += chunk) {
             const int64_t cnt = std::min(chunk, N - v0);
             TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, cnt, raw,
-                         steps};
+                         steps, subset};
             SGPU_HIP(hipEventRecord(ev_trace.a, c->stream));
             launch_trace(ta, c->stream);
             SGPU_HIP(hipEventRecord(ev_trace.b, c->stream));
-            ListArgs la{raw, v0, cnt, N, c->dirs, c->sky.svf_bytes, c->svf, counts, off, nullptr};
+            ListArgs la{raw, v0, cnt, c->n_total, c->dirs, c->sky.svf_bytes, svf, counts, off, nullptr};
             SGPU_HIP(hipEventRecord(ev_lists.a, c->stream));
             launch_lists(la, false, c->stream);
             launch_scan_counts(counts, cnt, off, scratch, c->stream);
@@ -310,7 +207,7 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
             la.ids = ids;
             launch_lists(la, true, c->stream);
             const unsigned g = (unsigned)((cnt + 255) / 256);
-            hipLaunchKernelGGL(k_add_base, dim3(g), dim3(256), 0, c->stream, off, cnt, base, c->csr_off + v0);
+            hipLaunchKernelGGL(k_add_base, dim3(g), dim3(256), 0, c->stream, off, cnt, base, out.off + v0);
             SGPU_HIP(hipGetLastError());
             SGPU_HIP(hipEventRecord(ev_lists.b, c->stream));
             SGPU_HIP(hipStreamSynchronize(c->stream));
@@ -318,23 +215,97 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
             c->info.t_lists_ms += ev_lists.ms();
             base += piece;
         }
-        SGPU_HIP(hipMemcpyAsync(c->csr_off + N, &base, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipMemcpyAsync(out.off + N, &base, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
         if (pieces.size() == 1) {
-            c->csr_ids = pieces[0].first;
+            out.ids = pieces[0].first;
             pieces.clear();
         } else {
-            c->csr_ids = dalloc<int32_t>((size_t)std::max<int64_t>(base, 1));
+            out.ids = dalloc<int32_t>((size_t)std::max<int64_t>(base, 1));
             int64_t pos = 0;
             for (auto& p : pieces) {
                 if (p.second)
-                    SGPU_HIP(hipMemcpyAsync(c->csr_ids + pos, p.first, p.second * sizeof(int32_t),
-                                            hipMemcpyDeviceToDevice, c->stream));
+                    SGPU_HIP(hipMemcpyAsync(out.ids + pos, p.first, p.second * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                            c->stream));
                 pos += p.second;
             }
             SGPU_HIP(hipStreamSynchronize(c->stream));
             for (auto& p : pieces) dfree(p.first);
             pieces.clear();
         }
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        for (auto& p : pieces) dfree(p.first);
+        dfree(raw);
+        dfree(counts);
+        dfree(off);
+        dfree(scratch);
+        dfree(out.off);
+        dfree(out.ids);
+        throw;
+    }
+    dfree(raw);
+    dfree(counts);
+    dfree(off);
+    dfree(scratch);
+    out.nnz = base;
+    return out;
+}
+
+/* ba-RLE streams of N CSR lists: size pass -> int64 scan -> write pass. */
+void encode_barle(sgpu_ctx* c, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,
+                  uint8_t** br_bytes, int64_t* size) {
+    BarleArgs ba{};
+    ba.csr_off = off;
+    ba.csr_ids = ids;
+    ba.n = N;
+    int32_t* sizes = dalloc<int32_t>(N);
+    int64_t* scr = nullptr;
+    int64_t* boff = nullptr;
+    uint8_t* bytes = nullptr;
+    try {
+        scr = dalloc<int64_t>(scan_scratch_elems(N));
+        boff = dalloc<int64_t>(N + 1);
+        ba.sizes = sizes;
+        launch_barle(ba, false, c->stream);
+        launch_scan_counts(sizes, N, boff, scr, c->stream);
+        int64_t total = 0;
+        SGPU_HIP(hipMemcpyAsync(&total, boff + N, sizeof total, hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
+        ba.out_off = boff;
+        ba.out = bytes;
+        launch_barle(ba, true, c->stream);
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        *size = total;
+    } catch (...) {
+        dfree(sizes);
+        dfree(scr);
+        dfree(boff);
+        dfree(bytes);
+        throw;
+    }
+    dfree(sizes);
+    dfree(scr);
+    *br_off = boff;
+    *br_bytes = bytes;
+}
+
+void do_viewshed(sgpu_ctx* c, unsigned formats) {
+    need(c->scene, SGPU_ERR_STATE, "scene not set");
+    need(c->sky_set, SGPU_ERR_STATE, "sky not set");
+    need((formats & ~(SGPU_FMT_CSR | SGPU_FMT_BARLE)) == 0 && formats != 0, SGPU_ERR_ARG, "bad formats");
+    c->drop_products();
+    const int64_t N = c->n_total;
+    c->info.t_trace_ms = c->info.t_lists_ms = 0.0;
+    c->info.dda_steps = 0;
+    EventPair all, ev_enc;
+    SGPU_HIP(hipEventRecord(all.a, c->stream));
+    c->svf = dalloc<uint8_t>((size_t)std::max(c->sky.svf_bytes, 1) * N);
+    unsigned long long* steps = c->count_steps ? dalloc<unsigned long long>(1) : nullptr;
+    if (steps) SGPU_HIP(hipMemsetAsync(steps, 0, sizeof(unsigned long long), c->stream));
+    ListsOut L;
+    try {
+        L = build_lists(c, nullptr, N, c->svf, steps);
         if (steps) {
             unsigned long long st = 0;
             SGPU_HIP(hipMemcpyAsync(&st, steps, sizeof st, hipMemcpyDeviceToHost, c->stream));
@@ -342,48 +313,22 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
             c->info.dda_steps = (int64_t)st;
         }
     } catch (...) {
-        for (auto& p : pieces) dfree(p.first);
-        dfree(raw);
-        dfree(counts);
-        dfree(off);
-        dfree(scratch);
         dfree(steps);
         throw;
     }
-    dfree(raw);
-    dfree(counts);
-    dfree(off);
-    dfree(scratch);
     dfree(steps);
-    c->nnz = base;
+    c->csr_off = L.off;
+    c->csr_ids = L.ids;
+    c->nnz = L.nnz;
     c->svf_ok = true;
-    c->info.nnz = base;
+    c->info.nnz = L.nnz;
     if (formats & SGPU_FMT_BARLE) {
-        BarleArgs ba{};
-        ba.csr_off = c->csr_off;
-        ba.csr_ids = c->csr_ids;
-        ba.n = N;
-        int32_t* sizes = dalloc<int32_t>(N);
-        int64_t* scr = dalloc<int64_t>(scan_scratch_elems(N));
-        c->br_off = dalloc<int64_t>(N + 1);
-        ba.sizes = sizes;
-        SGPU_HIP(hipEventRecord(ev_lists.a, c->stream));
-        launch_barle(ba, false, c->stream);
-        launch_scan_counts(sizes, N, c->br_off, scr, c->stream);
-        int64_t total = 0;
-        SGPU_HIP(hipMemcpyAsync(&total, c->br_off + N, sizeof total, hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipEventRecord(ev_enc.a, c->stream));
+        encode_barle(c, c->csr_off, c->csr_ids, N, &c->br_off, &c->br_bytes, &c->br_size);
+        SGPU_HIP(hipEventRecord(ev_enc.b, c->stream));
         SGPU_HIP(hipStreamSynchronize(c->stream));
-        c->br_bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
-        ba.out_off = c->br_off;
-        ba.out = c->br_bytes;
-        launch_barle(ba, true, c->stream);
-        SGPU_HIP(hipEventRecord(ev_lists.b, c->stream));
-        SGPU_HIP(hipStreamSynchronize(c->stream));
-        c->info.t_lists_ms += ev_lists.ms();
-        dfree(sizes);
-        dfree(scr);
-        c->br_size = total;
-        c->info.barle_bytes = total;
+        c->info.t_lists_ms += ev_enc.ms();
+        c->info.barle_bytes = c->br_size;
     }
     SGPU_HIP(hipEventRecord(all.b, c->stream));
     SGPU_HIP(hipStreamSynchronize(c->stream));
@@ -392,32 +337,13 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
 
 void do_barle_encode(sgpu_ctx* c) {
     need(c->nnz >= 0, SGPU_ERR_STATE, "CSR lists missing");
-    const int64_t N = c->n_total;
     dfree(c->br_off);
     dfree(c->br_bytes);
     c->br_size = -1;
-    BarleArgs ba{};
-    ba.csr_off = c->csr_off;
-    ba.csr_ids = c->csr_ids;
-    ba.n = N;
-    int32_t* sizes = dalloc<int32_t>(N);
-    int64_t* scr = dalloc<int64_t>(scan_scratch_elems(N));
-    c->br_off = dalloc<int64_t>(N + 1);
-    ba.sizes = sizes;
-    launch_barle(ba, false, c->stream);
-    launch_scan_counts(sizes, N, c->br_off, scr, c->stream);
-    int64_t total = 0;
-    SGPU_HIP(hipMemcpyAsync(&total, c->br_off + N, sizeof total, hipMemcpyDeviceToHost, c->stream));
-    SGPU_HIP(hipStreamSynchronize(c->stream));
-    c->br_bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
-    ba.out_off = c->br_off;
-    ba.out = c->br_bytes;
-    launch_barle(ba, true, c->stream);
-    SGPU_HIP(hipStreamSynchronize(c->stream));
-    dfree(sizes);
-    dfree(scr);
-    c->br_size = total;
-    c->info.barle_bytes = total;
+    int64_t size = 0;
+    encode_barle(c, c->csr_off, c->csr_ids, c->n_total, &c->br_off, &c->br_bytes, &size);
+    c->br_size = size;
+    c->info.barle_bytes = size;
 }
 
 void do_hour_loop(sgpu_ctx* c, unsigned fmt, int K, int stride) {
@@ -497,10 +423,91 @@ void do_hour_loop_graph(sgpu_ctx* c, unsigned fmt, int K, int replays) {
 
 }  // namespace
 
+namespace sgpu {
+
+SubsetLists subset_viewshed(sgpu_ctx* c, const int32_t* subset, int64_t n, bool barle) {
+    SubsetLists s;
+    if (n <= 0) return s;
+    ListsOut L = build_lists(c, subset, n, nullptr, nullptr);
+    s.csr_off = L.off;
+    s.csr_ids = L.ids;
+    s.nnz = L.nnz;
+    if (barle) {
+        try {
+            encode_barle(c, s.csr_off, s.csr_ids, n, &s.br_off, &s.br_bytes, &s.br_size);
+        } catch (...) {
+            s.release();
+            throw;
+        }
+    }
+    return s;
+}
+
+void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const SubsetLists& L, unsigned fmt, int K,
+                      float* out) {
+    if (n <= 0) return;
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "bad hours per pass");
+    ensure_accumulators(c, K);
+    ensure_hours_d(c, c->hours);
+    float* comp = dalloc<float>(n);
+    try {
+        zero_floats(out, n, c->stream);
+        zero_floats(comp, n, c->stream);
+        HourArgs a = hour_args(c, fmt);
+        a.subset = subset;
+        a.n_sub = n;
+        a.sum = out;
+        a.comp = comp;
+        if (fmt == SGPU_FMT_CSR) {
+            a.nb_off = L.csr_off;
+            a.nb_ids = L.csr_ids;
+        } else {
+            a.nb_off = L.br_off;
+            a.nb_bytes = L.br_bytes;
+        }
+        const int nsel = (int)c->hours.size();
+        for (int h0 = 0; h0 < nsel; h0 += K) {
+            a.hours = c->hours_d + h0;
+            a.nh = std::min(K, nsel - h0);
+            launch_solar_top(a, K, c->stream);
+            launch_reflect_accum(a, K, fmt, c->stream);
+        }
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        dfree(comp);
+        throw;
+    }
+    dfree(comp);
+}
+
+/* Annual reflected irradiance via single-bounce linearity: one reflection walk with the
+   annual top-cell totals in place of one hour's values (labels and checks only). */
+void annual_reflected_linear(sgpu_ctx* c, unsigned fmt, const float* dd, float* out_dev) {
+    check_stage2_ready(c, fmt);
+    ensure_hours_d(c, c->hours);
+    HourArgs a = hour_args(c, fmt);
+    a.hourly_top = const_cast<float*>(dd);   // cell-major with K = 1: the top voxels' annual totals
+    a.hours = c->hours_d;
+    a.nh = 1;
+    float* own = dalloc<float>(c->n_total);
+    try {
+        a.dbg_hourly = own;
+        a.dbg_reflected = out_dev;
+        launch_reflect_accum(a, 1, fmt, c->stream);
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        dfree(own);
+        throw;
+    }
+    dfree(own);
+}
+
+}  // namespace sgpu
+
 extern "C" {
 
 const char* sgpu_version(void) { return "solargpu-mi355x 0.1"; }
-const char* sgpu_last_error(void) { return g_last_error.c_str(); }
+const char* sgpu_last_error(void) { return last_error().c_str(); }
 
 int sgpu_sun_positions(float lon, float lat, int tz, float* out) {
     return guarded([&] {
@@ -738,6 +745,26 @@ int sgpu_get_barle(sgpu_ctx* c, int64_t* off, uint8_t* bytes) {
     });
 }
 
+int sgpu_get_csr_range(sgpu_ctx* c, int64_t v0, int64_t count, int64_t* off, int32_t* ids, int64_t cap) {
+    return guarded([&] {
+        need(c && off, SGPU_ERR_ARG, "null argument");
+        need(c->nnz >= 0, SGPU_ERR_STATE, "no CSR lists");
+        need(v0 >= 0 && count >= 0 && v0 + count <= c->n_total, SGPU_ERR_ARG, "voxel range out of bounds");
+        c->set_device();
+        SGPU_HIP(hipMemcpyAsync(off, c->csr_off + v0, (count + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        const int64_t base = off[0], n = off[count] - base;
+        need(n <= cap, SGPU_ERR_ARG, "ids capacity too small");
+        for (int64_t i = 0; i <= count; ++i) off[i] -= base;
+        if (n > 0) {
+            need(ids != nullptr, SGPU_ERR_ARG, "null ids");
+            SGPU_HIP(hipMemcpyAsync(ids, c->csr_ids + base, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
 int sgpu_trace_raw(sgpu_ctx* c, int64_t v0, int64_t count, int32_t* out) {
     return guarded([&] {
         need(c && out, SGPU_ERR_ARG, "null argument");
@@ -937,6 +964,34 @@ int sgpu_sync(sgpu_ctx* c) {
     });
 }
 
+int sgpu_copy_bandwidth(sgpu_ctx* c, int64_t bytes, int reps, double* gbps) {
+    return guarded([&] {
+        need(c && gbps && bytes >= 16 && reps >= 1, SGPU_ERR_ARG, "bad arguments");
+        c->set_device();
+        const int64_t n = bytes / 16;
+        float4* src = dalloc<float4>(n);
+        float4* dst = nullptr;
+        try {
+            dst = dalloc<float4>(n);
+            SGPU_HIP(hipMemsetAsync(src, 0, n * 16, c->stream));
+            const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 256 * 1024);
+            hipLaunchKernelGGL(k_copy_f4, dim3(g), dim3(256), 0, c->stream, src, dst, n);  // warm-up
+            EventPair ev;
+            SGPU_HIP(hipEventRecord(ev.a, c->stream));
+            for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_f4, dim3(g), dim3(256), 0, c->stream, src, dst, n);
+            SGPU_HIP(hipEventRecord(ev.b, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            *gbps = 2.0 * (double)n * 16.0 * reps / (ev.ms() * 1e-3) / 1e9;
+        } catch (...) {
+            dfree(src);
+            dfree(dst);
+            throw;
+        }
+        dfree(src);
+        dfree(dst);
+    });
+}
+
 static int whole_pipeline(unsigned fmt, const float* grid, int w, int h, int n_total, int n_top, const int32_t* pp,
                           const float* wall_z, float maxz, const float* az_alt, const float* dni, const float* dhi,
                           const float* slopes, const float* aspects, float* out) {
@@ -948,9 +1003,9 @@ static int whole_pipeline(unsigned fmt, const float* grid, int w, int h, int n_t
     if (!st) st = sgpu_viewshed(c, fmt);
     if (!st) st = sgpu_hour_loop(c, fmt, 0, 1);
     if (!st) st = sgpu_get_irradiance(c, out);
-    std::string keep = g_last_error;
+    std::string keep = last_error();
     sgpu_destroy(c);
-    g_last_error = keep;
+    last_error() = keep;
     return st;
 }
 

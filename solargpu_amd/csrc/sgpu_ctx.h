@@ -1,0 +1,200 @@
+/* The per-context device state behind the C ABI and the helpers shared by
+ * the ABI translation units (sgpu_capi.hip, sgpu_surrogate.hip). */
+#pragma once
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "sgpu_internal.h"
+
+namespace sgpu {
+
+/* Message of the last failing C-ABI call on this thread. */
+std::string& last_error();
+
+template <typename F>
+int guarded(F&& fn) {
+    try {
+        fn();
+        return SGPU_OK;
+    } catch (const Failure& f) {
+        last_error() = f.what();
+        return f.code;
+    } catch (const std::bad_alloc&) {
+        last_error() = "host allocation failed";
+        return SGPU_ERR_OOM;
+    } catch (const std::exception& e) {
+        last_error() = e.what();
+        return SGPU_ERR_HIP;
+    }
+}
+
+inline void need(bool ok, int code, const char* msg) {
+    if (!ok) fail(code, "%s", msg);
+}
+
+template <typename T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    SGPU_HIP(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        SGPU_HIP(hipEventCreate(&a));
+        SGPU_HIP(hipEventCreate(&b));
+    }
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    EventPair(EventPair&& o) noexcept : a(o.a), b(o.b) { o.a = o.b = nullptr; }
+    float ms() const {
+        float t = 0.0f;
+        SGPU_HIP(hipEventElapsedTime(&t, a, b));
+        return t;
+    }
+};
+
+/* Surrogate-path buffers of a context (sgpu_surrogate.hip). */
+struct SurrogateState {
+    uint16_t* X = nullptr;       // features [N][64] bf16
+    float* dd = nullptr;         // annual direct + diffuse
+    float* diff = nullptr;       // annual diffuse
+    uint16_t* sun_hits = nullptr;
+    int16_t* sun_dirs = nullptr;
+    bool ready = false;
+    void release() {
+        dfree(X);
+        dfree(dd);
+        dfree(diff);
+        dfree(sun_hits);
+        dfree(sun_dirs);
+        ready = false;
+    }
+};
+
+}  // namespace sgpu
+
+struct sgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    int W = 0, H = 0;
+    int64_t n_total = 0, n_top = 0;
+    float maxz = 0.0f;
+    float* grid = nullptr;
+    int32_t* pix_parent = nullptr;
+    float* wall_z = nullptr;
+    float* slopes = nullptr;
+    float* aspects = nullptr;
+    bool scene = false;
+
+    bool sky_set = false;
+    sgpu::SkyTables sky;
+    std::vector<sgpu::HourConst> hours;
+    sgpu::DirConst* dirs = nullptr;
+    float lat = 0.0f;
+
+    uint8_t* svf = nullptr;
+    bool svf_ok = false;
+    int64_t* csr_off = nullptr;
+    int32_t* csr_ids = nullptr;
+    int64_t nnz = -1;
+    int64_t* br_off = nullptr;
+    uint8_t* br_bytes = nullptr;
+    int64_t br_size = -1;
+
+    float* sum = nullptr;
+    float* comp = nullptr;
+    float* hourly_top = nullptr;
+    size_t hourly_top_elems = 0;
+    sgpu::HourConst* hours_d = nullptr;
+    size_t hours_d_cap = 0;
+    bool irr_ok = false;
+
+    int64_t chunk_override = 0;
+    bool kernel_timing = false;
+    bool count_steps = false;
+    sgpu_info info{};
+    sgpu::SurrogateState sur;
+
+    void set_device() const { SGPU_HIP(hipSetDevice(device)); }
+
+    void drop_products() {
+        sgpu::dfree(svf);
+        sgpu::dfree(csr_off);
+        sgpu::dfree(csr_ids);
+        sgpu::dfree(br_off);
+        sgpu::dfree(br_bytes);
+        svf_ok = false;
+        nnz = -1;
+        br_size = -1;
+        irr_ok = false;
+        sur.release();
+    }
+    void drop_scene() {
+        sgpu::dfree(grid);
+        sgpu::dfree(pix_parent);
+        sgpu::dfree(wall_z);
+        sgpu::dfree(slopes);
+        sgpu::dfree(aspects);
+        sgpu::dfree(sum);
+        sgpu::dfree(comp);
+        sgpu::dfree(hourly_top);
+        hourly_top_elems = 0;
+        scene = false;
+    }
+    ~sgpu_ctx() {
+        if (device >= 0) (void)hipSetDevice(device);
+        drop_products();
+        drop_scene();
+        sgpu::dfree(dirs);
+        sgpu::dfree(hours_d);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace sgpu {
+
+/* Shared pieces of the pipeline driver (sgpu_capi.hip) used by the surrogate path. */
+void ensure_accumulators(sgpu_ctx* c, int K);
+void ensure_hours_d(sgpu_ctx* c, const std::vector<HourConst>& hs);
+void zero_floats(float* p, int64_t n, hipStream_t s);
+/* Viewshed of a voxel subset (ids on the device) into compact CSR lists and, if
+   barle, ba-RLE streams, indexed by subset position. */
+struct SubsetLists {
+    int64_t* csr_off = nullptr;
+    int32_t* csr_ids = nullptr;
+    int64_t nnz = 0;
+    int64_t* br_off = nullptr;
+    uint8_t* br_bytes = nullptr;
+    int64_t br_size = 0;
+    void release() {
+        dfree(csr_off);
+        dfree(csr_ids);
+        dfree(br_off);
+        dfree(br_bytes);
+    }
+};
+SubsetLists subset_viewshed(sgpu_ctx* c, const int32_t* subset, int64_t n, bool barle);
+/* The plain per-hour annual loop for a voxel subset; annual totals per position (device). */
+void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const SubsetLists& L, unsigned fmt, int K,
+                      float* out);
+/* Annual reflected irradiance per voxel from the annual direct + diffuse of the top cells
+   (device arrays), by single-bounce linearity. */
+void annual_reflected_linear(sgpu_ctx* c, unsigned fmt, const float* dd, float* out_dev);
+
+}  // namespace sgpu

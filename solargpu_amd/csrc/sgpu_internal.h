@@ -53,12 +53,13 @@ void directions(const SkyTables& sky, DirConst* out);   // kDirs entries, refere
 
 /* Per sunlit-hour constants of the solar kernel (solar_kernel.cu:8-50, 54-77). */
 struct alignas(16) HourConst {
-    float cos_z, sin_z, azk, dni_eff;   // cos zenith, sin(acos(cos_z)), azimuth rad, fmaxf(DNI,1e-3f)
-    float dhi, b, f1, f2;               // DHI, max(cos 85deg, cos_z), Perez F1, F2
+    float cos_z, sin_z;                 // cos zenith, sin(acos(cos_z))
+    float cos_az, sin_az;               // of the sun azimuth (radians)
+    float dni_eff, dhi;                 // fmaxf(DNI, 1e-3f), DHI
+    float inv_b, f1, f2;                // 1 / max(cos 85deg, cos_z), Perez F1, F2
     int32_t dense;                      // sun bin (dense SVF index) or -1
     int32_t index;                      // day*24 + hour
     int32_t eps_bin;
-    int32_t pad;
 };
 std::vector<HourConst> hour_constants(const SkyTables& sky, const float* dni, const float* dhi);
 
@@ -81,9 +82,10 @@ struct TraceArgs {
     const float* wall_z;
     float maxz;
     const DirConst* dirs;
-    int64_t v0, count;      // voxel range of this chunk
+    int64_t v0, count;      // voxel range of this chunk (positions in `subset` when given)
     int32_t* raw;           // count x kDirs first-hit ids (+1), 0 = none
     unsigned long long* steps;  // optional DDA step counter
+    const int32_t* subset;  // optional voxel id list
 };
 void launch_trace(const TraceArgs& a, hipStream_t s);
 
@@ -92,7 +94,7 @@ struct ListArgs {
     int64_t v0, count, n_total;
     const DirConst* dirs;
     int svf_bytes;
-    uint8_t* svf;           // word-major SVF bitmap (voxel_types.h:28-38)
+    uint8_t* svf;           // word-major SVF bitmap (voxel_types.h:28-38); NULL = not written
     int32_t* counts;        // pass 1 output
     const int64_t* offsets; // pass 2 input (chunk-local, count+1)
     int32_t* ids;           // pass 2 output
@@ -133,6 +135,8 @@ struct HourArgs {
     float* comp;               // Kahan compensation
     float* dbg_hourly;         // optional single-hour outputs
     float* dbg_reflected;
+    const int32_t* subset;     // optional voxel ids; then nb_off, sum, comp, dbg_* are per position
+    int64_t n_sub;
 };
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s);
 void launch_reflect_accum(const HourArgs& a, int K, unsigned format, hipStream_t s);

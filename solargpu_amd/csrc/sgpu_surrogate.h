@@ -1,0 +1,49 @@
+/* Surrogate path: kernel arguments and launchers (k_surrogate.hip). */
+#pragma once
+
+#include <cstdint>
+
+#include "sgpu_internal.h"
+
+namespace sgpu {
+
+constexpr int kSectors = 32;
+constexpr int kSurFeatures = 48;
+constexpr int kSurInPad = 64;
+
+struct SurArgs {
+    const float* grid;
+    int W, H;
+    int64_t n_total, n_top;
+    const int32_t* pix_parent;
+    const float* wall_z;
+    const float* slopes;
+    const float* aspects;
+    float maxz;
+    const DirConst* dirs;
+    const int16_t* sun_dirs;   // direction index of every dense sun bin
+    int n_sun_dirs;
+    int svf_bytes;
+    uint8_t* svf;
+    uint16_t* sun_hits;
+    const HourConst* hours;
+    int n_hours;
+    float* dd;                 // annual direct + diffuse
+    float* diff;               // annual diffuse
+    uint16_t* X;               // features [n_total][64] bf16
+    float lat, albedo;
+    const float* Y;            // MLP output [n_total][2]
+    float* out;                // surrogate annual total
+    int32_t* fallback;         // 1 = take the exact plain result
+    float tau, refl_rel_err, diff_miss_weight, share_abs_err;
+};
+
+void launch_sun_svf(const SurArgs& a, hipStream_t s);
+void launch_annual_dd(const SurArgs& a, hipStream_t s);
+void launch_features(const SurArgs& a, hipStream_t s);
+void launch_gate(const SurArgs& a, hipStream_t s);
+void launch_scatter(const int32_t* ids, const float* vals, int64_t n, float* out, hipStream_t s);
+/* ids[off[v]] = v for every flagged v (off = exclusive scan of the flags). */
+void launch_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids, hipStream_t s);
+
+}  // namespace sgpu

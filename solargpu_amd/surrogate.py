@@ -1,0 +1,113 @@
+"""Per-voxel surrogate for the reflection pass: the MLP binding (sgpu_mlp_* of
+include/sgpu.h) and the surrogate pipeline helpers.
+
+The network (64 -> 4 x 256 GELU -> 2, BF16 weights/activations, FP32
+accumulation on MFMA) is trained by the library's own forward, backward and
+Adam kernels on labels produced by the plain pipeline.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _check, _lib, _ptr, _c, _I64
+
+IN_PAD = 64
+FEATURES = 48
+HIDDEN = 256
+OUTPUTS = 2
+
+_lib.sgpu_mlp_last_error.restype = ctypes.c_char_p
+
+
+def _mcheck(code: int) -> None:
+    if code != 0:
+        from . import SolarGPUError
+        raise SolarGPUError(code, _lib.sgpu_mlp_last_error().decode(errors="replace"))
+
+
+def to_bf16(x: np.ndarray) -> np.ndarray:
+    """float32 -> bf16 bit patterns (uint16), round to nearest even."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bf16(h: np.ndarray) -> np.ndarray:
+    return (np.asarray(h, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def layer_shapes():
+    ins = [IN_PAD, HIDDEN, HIDDEN, HIDDEN, HIDDEN]
+    outs = [HIDDEN, HIDDEN, HIDDEN, HIDDEN, OUTPUTS]
+    return list(zip(outs, ins))
+
+
+def unpack_params(p: np.ndarray):
+    """Flat parameter vector -> [(W[out][in], b[out]), ...]."""
+    out, pos = [], 0
+    for o, i in layer_shapes():
+        W = p[pos:pos + o * i].reshape(o, i)
+        pos += o * i
+        b = p[pos:pos + o]
+        pos += o
+        out.append((W, b))
+    return out
+
+
+def pack_params(layers) -> np.ndarray:
+    return np.concatenate([np.concatenate([W.ravel(), b.ravel()]) for W, b in layers]).astype(np.float32)
+
+
+class MLP:
+    def __init__(self, seed: int = 1, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _mcheck(_lib.sgpu_mlp_create(device, ctypes.c_uint64(seed), ctypes.byref(self._h)))
+        self.nparams = _lib.sgpu_mlp_param_count()
+
+    def close(self):
+        if self._h:
+            _mcheck(_lib.sgpu_mlp_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparams, np.float32)
+        _mcheck(_lib.sgpu_mlp_get_params(self._h, _ptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = _c(p, np.float32)
+        assert p.size == self.nparams
+        _mcheck(_lib.sgpu_mlp_set_params(self._h, _ptr(p)))
+
+    def set_loss_weights(self, w0: float, w1: float) -> None:
+        _mcheck(_lib.sgpu_mlp_set_loss_weights(self._h, ctypes.c_float(w0), ctypes.c_float(w1)))
+
+    def forward(self, X_bf16: np.ndarray) -> np.ndarray:
+        X = _c(X_bf16, np.uint16)
+        n = X.shape[0]
+        Y = np.zeros((n, OUTPUTS), np.float32)
+        _mcheck(_lib.sgpu_mlp_forward(self._h, _ptr(X), _I64(n), _ptr(Y)))
+        return Y
+
+    def train(self, X_bf16, T, epochs: int, batch: int, lr: float, seed: int = 1) -> np.ndarray:
+        X = _c(X_bf16, np.uint16)
+        T = _c(T, np.float32)
+        losses = np.zeros(epochs, np.float64)
+        _mcheck(_lib.sgpu_mlp_train(self._h, _ptr(X), _ptr(T), _I64(X.shape[0]), epochs, batch,
+                                    ctypes.c_float(lr), ctypes.c_uint64(seed), _ptr(losses)))
+        return losses
+
+    def gradient(self, X_bf16, T) -> tuple[np.ndarray, float]:
+        X = _c(X_bf16, np.uint16)
+        T = _c(T, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_double()
+        _mcheck(_lib.sgpu_mlp_gradient(self._h, _ptr(X), _ptr(T), X.shape[0], _ptr(g), ctypes.byref(loss)))
+        return g, loss.value

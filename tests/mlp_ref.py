@@ -1,0 +1,53 @@
+"""Float64 numpy restatement of the surrogate MLP (test infrastructure):
+forward and analytic backward of 64 -> 4 x 256 GELU -> 2 with the mean
+weighted squared-error loss used by the library's trainer."""
+from __future__ import annotations
+
+import numpy as np
+from scipy.special import erf
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+
+
+def gelu_grad(z):
+    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+
+
+def forward(layers, X):
+    """layers: [(W[out][in], b[out])]; returns (Y, cache)."""
+    a = X.astype(np.float64)
+    cache = [a]
+    zs = []
+    for W, b in layers[:-1]:
+        z = a @ W.astype(np.float64).T + b
+        zs.append(z)
+        a = gelu(z)
+        cache.append(a)
+    W, b = layers[-1]
+    y = a @ W.astype(np.float64).T + b
+    return y, (cache, zs)
+
+
+def loss_and_grad(layers, X, T, wt=(1.0, 1.0)):
+    B = X.shape[0]
+    y, (acts, zs) = forward(layers, X)
+    w = np.asarray(wt, np.float64)
+    e = y - T
+    loss = float((w * e * e).sum() / B)
+    dy = 2.0 * e * w / B
+    grads = [None] * len(layers)
+    W, b = layers[-1]
+    grads[-1] = (dy.T @ acts[-1], dy.sum(0))
+    dh = dy @ W.astype(np.float64)
+    for l in range(len(layers) - 2, -1, -1):
+        dz = dh * gelu_grad(zs[l])
+        Wl, _ = layers[l]
+        grads[l] = (dz.T @ acts[l], dz.sum(0))
+        dh = dz @ Wl.astype(np.float64)
+    return loss, grads
+
+
+def flat(grads):
+    return np.concatenate([np.concatenate([gW.ravel(), gb.ravel()]) for gW, gb in grads])

@@ -216,6 +216,19 @@ def annual_fp32(sc, bins, dni, dhi, svf, off, ids, boff=None, data=None, mode=0,
     return out, n
 
 
+def annual_fp32_range(sc, bins, dni, dhi, svf, v0, off_slice, ids, stride=1, nthreads=0):
+    """Reference-structured FP32 hour loop for voxels [v0, v0 + len(off_slice) - 1)."""
+    count = off_slice.size - 1
+    out = np.zeros(count, np.float32)
+    ids = _c(ids, np.int32)
+    tt, tr = ctypes.c_double(), ctypes.c_double()
+    n = O.orc_annual_fp32_range(*_scene_args(sc), _ptr(bins["nudged"]), _ptr(bins["sun_dense_idx"]),
+                                _ptr(_c(dni, np.float32)), _ptr(_c(dhi, np.float32)), _ptr(_c(svf, np.uint8)),
+                                _I64(v0), _I64(count), _ptr(_c(off_slice, np.int64)), _ptr(ids), stride, nthreads,
+                                _ptr(out), ctypes.byref(tt), ctypes.byref(tr))
+    return out, n, tt.value, tr.value
+
+
 def annual_fp64(sc, bins, dni, dhi, svf, off, ids, voxels=None, nthreads=0):
     ids = _c(ids, np.int32)
     sel = None if voxels is None else _c(voxels, np.int32)

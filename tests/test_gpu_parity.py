@@ -1,7 +1,8 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
 Bars: neighbour lists, SVF bitmaps and ba-RLE streams bit-exact; per-hour
-hourly/reflected irradiance within 2e-5 relative (device vs host libm);
+hourly/reflected irradiance within 1e-4 relative + 1e-3 W/m^2 (device fast
+reciprocals and angle-addition cosine vs host libm);
 annual irradiance within 1e-4 relative of the FP64 oracle (floor 1 Wh/m^2).
 """
 import numpy as np
@@ -14,7 +15,7 @@ from scenes import oracle_lists, sky, synth_scene, scene_from_heights
 pytestmark = pytest.mark.gpu
 
 ANNUAL_RTOL = 1e-4
-HOUR_RTOL = 2e-5
+HOUR_RTOL = 1e-4
 
 
 def _ctx(sc, formats=sg.FMT_CSR | sg.FMT_BARLE, chunk=0):
@@ -113,8 +114,8 @@ def test_single_hour_matches_oracle(fmt):
             eh = orc.solar_hour(sc, bins, dni, dhi, ref["svf"], int(hi))
             er = (orc.reflect_hour_csr(sc, ref["off"], ref["ids"], eh) if fmt == sg.FMT_CSR
                   else orc.reflect_hour_barle(sc, boff, data, eh))
-            assert _rel(hv, eh, 1e-3).max() <= HOUR_RTOL, f"hourly, hour {hi}"
-            assert np.abs(rv - er).max() <= HOUR_RTOL * max(float(np.abs(er).max()), 1.0), f"reflected, hour {hi}"
+            assert (np.abs(hv - eh) <= HOUR_RTOL * np.abs(eh) + 1e-3).all(), f"hourly, hour {hi}"
+            assert (np.abs(rv - er) <= HOUR_RTOL * np.abs(er) + 1e-4).all(), f"reflected, hour {hi}"
 
 
 @pytest.mark.parametrize("fmt,K", [(sg.FMT_CSR, 1), (sg.FMT_CSR, 16), (sg.FMT_BARLE, 1), (sg.FMT_BARLE, 8),

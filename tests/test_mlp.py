@@ -1,0 +1,97 @@
+"""Surrogate MLP: the float64 reference pinned by finite differences (CPU),
+and the HIP forward / backward / Adam kernels against it (GPU)."""
+import numpy as np
+import pytest
+
+import solargpu_amd.surrogate as sm
+import mlp_ref
+
+
+def _small_layers(rng, widths):
+    return [(rng.normal(0, 1.0 / np.sqrt(i), (o, i)), rng.normal(0, 0.1, o)) for i, o in zip(widths[:-1], widths[1:])]
+
+
+def test_reference_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    layers = _small_layers(rng, [6, 5, 5, 2])
+    X = rng.normal(size=(7, 6))
+    T = rng.normal(size=(7, 2))
+    loss, grads = mlp_ref.loss_and_grad(layers, X, T, (1.0, 0.5))
+    g = mlp_ref.flat(grads)
+    params = [p for W, b in layers for p in (W, b)]
+    num = []
+    h = 1e-6
+    for p in params:
+        for idx in np.ndindex(p.shape):
+            old = p[idx]
+            p[idx] = old + h
+            lp, _ = mlp_ref.loss_and_grad(layers, X, T, (1.0, 0.5))
+            p[idx] = old - h
+            lm, _ = mlp_ref.loss_and_grad(layers, X, T, (1.0, 0.5))
+            p[idx] = old
+            num.append((lp - lm) / (2 * h))
+    np.testing.assert_allclose(g, np.array(num), rtol=1e-6, atol=1e-9)
+
+
+def test_bf16_round_trip():
+    x = np.array([1.0, -2.5, 3.14159265, 1e-3, 65504.0, 0.0], np.float32)
+    y = sm.from_bf16(sm.to_bf16(x))
+    assert np.all(np.abs(y - x) <= np.abs(x) * 2.0 ** -8)
+
+
+def _inputs(rng, n):
+    X = np.zeros((n, sm.IN_PAD), np.float32)
+    X[:, :sm.FEATURES] = rng.normal(0, 0.7, (n, sm.FEATURES))
+    return sm.to_bf16(X)
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_fp64_reference():
+    rng = np.random.default_rng(1)
+    mlp = sm.MLP(seed=3)
+    p = mlp.params()
+    layers = [(sm.from_bf16(sm.to_bf16(W)), b) for W, b in sm.unpack_params(p)]
+    Xb = _inputs(rng, 1000)
+    y_gpu = mlp.forward(Xb)
+    y_ref, _ = mlp_ref.forward(layers, sm.from_bf16(Xb))
+    scale = np.abs(y_ref).max()
+    assert np.abs(y_gpu - y_ref).max() <= 2e-2 * scale + 1e-4
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_fp64_reference():
+    rng = np.random.default_rng(2)
+    mlp = sm.MLP(seed=4)
+    mlp.set_loss_weights(1.0, 0.5)
+    p = mlp.params()
+    layers = [(sm.from_bf16(sm.to_bf16(W)), b) for W, b in sm.unpack_params(p)]
+    Xb = _inputs(rng, 512)
+    T = rng.normal(0, 0.5, (512, 2)).astype(np.float32)
+    g_gpu, loss_gpu = mlp.gradient(Xb, T)
+    loss_ref, grads = mlp_ref.loss_and_grad(layers, sm.from_bf16(Xb), T, (1.0, 0.5))
+    assert abs(loss_gpu - loss_ref) <= 2e-2 * loss_ref
+    pos = 0
+    for li, ((gW, gb), (o, i)) in enumerate(zip(grads, sm.layer_shapes())):
+        got_W = g_gpu[pos:pos + o * i].reshape(o, i)
+        pos += o * i
+        got_b = g_gpu[pos:pos + o]
+        pos += o
+        for name, got, ref in (("W", got_W, gW), ("b", got_b, gb)):
+            cos = float((got.ravel() @ ref.ravel()) / (np.linalg.norm(got) * np.linalg.norm(ref) + 1e-30))
+            rel = np.linalg.norm(got - ref) / (np.linalg.norm(ref) + 1e-30)
+            assert cos > 0.999 and rel < 5e-2, (li, name, cos, rel)
+    assert pos == g_gpu.size
+
+
+@pytest.mark.gpu
+def test_gpu_training_fits_a_smooth_function():
+    rng = np.random.default_rng(5)
+    n = 1 << 15
+    Xf = np.zeros((n, sm.IN_PAD), np.float32)
+    Xf[:, :sm.FEATURES] = rng.uniform(-1, 1, (n, sm.FEATURES))
+    T = np.stack([np.sin(2 * Xf[:, 0]) * Xf[:, 1] + 0.3 * Xf[:, 2] ** 2, np.cos(Xf[:, 3] + Xf[:, 4])], 1)
+    mlp = sm.MLP(seed=6)
+    losses = mlp.train(sm.to_bf16(Xf), T.astype(np.float32), epochs=12, batch=1024, lr=2e-3)
+    assert losses[-1] < 0.05 * losses[0], losses
+    Y = mlp.forward(sm.to_bf16(Xf[:2048]))
+    assert np.sqrt(np.mean((Y - T[:2048]) ** 2)) < 0.1
