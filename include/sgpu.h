@@ -93,6 +93,23 @@ int sgpu_top_slope_aspect(const float* heights, int w, int h, double scale_x, do
 int sgpu_assemble_voxels(const float* heights, int w, int h, double scale_x, double scale_y,
                          int64_t cap, int64_t* n_total, int32_t* pixel_parent, float* slopes,
                          float* aspects, float* wall_z);
+/* GeoTIFF heightmap I/O without libtiff (geotiff_io.cpp:50-178, heightmap.cpp:7-24).
+   Open parses the file; dims/scale come from it (the pixel-scale tag is required, as in
+   heightmap.cpp:11-14); read copies the raster; write emits a float32 GeoTIFF carrying the
+   opened file's geo tags and compression (one strip, no predictor). */
+typedef struct sgpu_geotiff sgpu_geotiff;
+int sgpu_geotiff_open(const char* path, sgpu_geotiff** out);
+int sgpu_geotiff_info(const sgpu_geotiff* g, int* width, int* height, double* scale_x, double* scale_y,
+                      int* compression);
+int sgpu_geotiff_read(const sgpu_geotiff* g, float* out /* width*height */);
+int sgpu_geotiff_write(const sgpu_geotiff* meta, const char* path, const float* data, int width, int height);
+int sgpu_geotiff_close(sgpu_geotiff* g);
+/* Whitespace-separated TMY series (main.cpp:30-40); fails on fewer than n values. */
+int sgpu_read_tmy(const char* path, float* out, int n);
+/* (gx, gy, z, irradiance) float records after (n_total, n_top) int32 (main.cpp:44-67). */
+int sgpu_write_voxel_dump(const char* path, int n_total, int n_top, int grid_width, const int32_t* pixel_parent,
+                          const float* grid, const float* wall_z, const float* irradiance);
+
 /* Seeded synthetic inputs (new). */
 int sgpu_synth_dsm(uint64_t seed, int w, int h, float cell_m, float building_density, int trees,
                    float* heights_out);

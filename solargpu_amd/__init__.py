@@ -189,6 +189,51 @@ def assemble_scene(heights: np.ndarray, cell: float, cell_y: float | None = None
     return Scene(h, pp, sl, asp, wz, float(h.max()), float(cell))
 
 
+class GeoTiff:
+    """A float32 GeoTIFF read by the library's native reader (no libtiff)."""
+
+    def __init__(self, path: str):
+        self._h = ctypes.c_void_p()
+        _check(_lib.sgpu_geotiff_open(path.encode(), ctypes.byref(self._h)))
+        w, h, comp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        sx, sy = ctypes.c_double(), ctypes.c_double()
+        _check(_lib.sgpu_geotiff_info(self._h, ctypes.byref(w), ctypes.byref(h), ctypes.byref(sx),
+                                      ctypes.byref(sy), ctypes.byref(comp)))
+        self.width, self.height = w.value, h.value
+        self.scale = (sx.value, sy.value)
+        self.compression = comp.value
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width), np.float32)
+        _check(_lib.sgpu_geotiff_read(self._h, _ptr(out)))
+        return out
+
+    def write(self, path: str, data: np.ndarray) -> None:
+        """Write `data` (H x W float32) with this file's geo tags and compression."""
+        d = _c(data, np.float32)
+        _check(_lib.sgpu_geotiff_write(self._h, path.encode(), _ptr(d), d.shape[1], d.shape[0]))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.sgpu_geotiff_close(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+def read_tmy(path: str, n: int = HOURS) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    _check(_lib.sgpu_read_tmy(path.encode(), _ptr(out), n))
+    return out
+
+
+def write_voxel_dump(path: str, sc: "Scene", irradiance: np.ndarray) -> None:
+    irr = _c(irradiance, np.float32)
+    _check(_lib.sgpu_write_voxel_dump(path.encode(), sc.num_total, sc.num_top, sc.width, _ptr(sc.pixel_parent),
+                                      _ptr(sc.grid), _ptr(sc.wall_z) if sc.wall_z.size else None, _ptr(irr)))
+
+
 def synth_dsm(seed: int, width: int, height: int, cell: float, density: float, trees: int) -> np.ndarray:
     out = np.zeros((height, width), np.float32)
     _check(_lib.sgpu_synth_dsm(ctypes.c_uint64(seed), width, height, _F(cell), _F(density), trees, _ptr(out)))

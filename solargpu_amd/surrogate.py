@@ -111,3 +111,72 @@ class MLP:
         loss = ctypes.c_double()
         _mcheck(_lib.sgpu_mlp_gradient(self._h, _ptr(X), _ptr(T), X.shape[0], _ptr(g), ctypes.byref(loss)))
         return g, loss.value
+
+
+# ---------------- surrogate pipeline ----------------
+
+class Gate(ctypes.Structure):
+    """Fallback rule: exact path where the predicted relative error of the annual total,
+    refl_rel_err * predicted reflected fraction + diff_miss_weight * |diffuse-share miss|
+    + share_abs_err * direct_diffuse / total, exceeds tau."""
+    _fields_ = [("tau", ctypes.c_float), ("refl_rel_err", ctypes.c_float),
+                ("diff_miss_weight", ctypes.c_float), ("share_abs_err", ctypes.c_float)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("fallback", ctypes.c_int64), ("t_svf_ms", ctypes.c_double), ("t_dd_ms", ctypes.c_double),
+                ("t_features_ms", ctypes.c_double), ("t_mlp_ms", ctypes.c_double),
+                ("t_fallback_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def prepare(ctx, lat: float) -> dict:
+    st = Stats()
+    _check(_lib.sgpu_surrogate_prepare(ctx._h, ctypes.c_float(lat), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def features(ctx) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    n = ctx.info()["num_total"]
+    X = np.zeros((n, IN_PAD), np.uint16)
+    dd = np.zeros(n, np.float32)
+    diff = np.zeros(n, np.float32)
+    _check(_lib.sgpu_surrogate_get(ctx._h, _ptr(X), _ptr(dd), _ptr(diff)))
+    return X, dd, diff
+
+
+def labels(ctx, fmt: int = 1) -> np.ndarray:
+    """Annual reflected irradiance of the plain method (needs ctx.viewshed())."""
+    out = np.zeros(ctx.info()["num_total"], np.float32)
+    _check(_lib.sgpu_surrogate_labels(ctx._h, ctypes.c_uint(fmt), _ptr(out)))
+    return out
+
+
+def targets(refl, dd, diff) -> np.ndarray:
+    """Network targets: reflected share and diffuse share of the exact direct + diffuse."""
+    scale = np.maximum(dd, 1e-3)
+    return np.stack([refl / scale, diff / scale], 1).astype(np.float32)
+
+
+def run(ctx, mlp: MLP, gate: Gate, hours_per_pass: int = 16):
+    n = ctx.info()["num_total"]
+    out = np.zeros(n, np.float32)
+    fb = np.zeros(n, np.int32)
+    st = Stats()
+    _check(_lib.sgpu_surrogate_run(ctx._h, mlp._h, ctypes.byref(gate), hours_per_pass, _ptr(out), _ptr(fb),
+                                   ctypes.byref(st)))
+    return out, fb, st.as_dict()
+
+
+def save(path: str, params: np.ndarray, gate: Gate, meta: dict) -> None:
+    np.savez(path, params=params.astype(np.float32),
+             gate=np.array([gate.tau, gate.refl_rel_err, gate.diff_miss_weight, gate.share_abs_err], np.float32),
+             meta=np.frombuffer(repr(meta).encode(), np.uint8))
+
+
+def load(path: str) -> tuple[np.ndarray, Gate, str]:
+    z = np.load(path, allow_pickle=False)
+    g = z["gate"]
+    return z["params"], Gate(*[float(x) for x in g]), z["meta"].tobytes().decode()
