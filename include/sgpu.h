@@ -161,6 +161,24 @@ int sgpu_set_kernel_timing(sgpu_ctx* ctx, int enable);
 int sgpu_set_step_counting(sgpu_ctx* ctx, int enable);
 /* Synchronise the context stream. */
 int sgpu_sync(sgpu_ctx* ctx);
+
+/* Autotuner (new).  Launch shapes of the hot kernels, measured on the context's own scene:
+   the DDA tile (directions per workgroup of 256 voxels), the hours per stage-2 pass and
+   the waves per reflection workgroup (LDS footprint / occupancy).  Results persist in a
+   text cache ($SGPU_AUTOTUNE_CACHE, else ~/.cache/solargpu/autotune.txt) keyed by device,
+   format and scene size class; a cached key is applied without measuring.  Stage-2 shapes
+   are tuned only when the lists of `format` exist.  Every shape gives identical results. */
+typedef struct sgpu_tuning {
+    int32_t dirs_per_group;      /* 16, 32 or 48 */
+    int32_t hours_per_pass;      /* 1..32 */
+    int32_t waves_per_group;     /* 2 or 4 */
+    int32_t from_cache;
+    double rays_per_s;           /* best measured trace rate (0 if from cache) */
+    double voxel_hours_per_s;    /* best measured stage-2 rate (0 if from cache / not tuned) */
+} sgpu_tuning;
+int sgpu_autotune(sgpu_ctx* ctx, unsigned format, sgpu_tuning* out);
+int sgpu_set_tuning(sgpu_ctx* ctx, const sgpu_tuning* t);
+int sgpu_get_tuning(sgpu_ctx* ctx, sgpu_tuning* out);
 /* Plain device copy of `bytes` (float4 loads/stores), `reps` launches; *gbps = read+write
    bytes / mean launch time.  Calibrates HBM counters and the measured copy bandwidth (new). */
 int sgpu_copy_bandwidth(sgpu_ctx* ctx, int64_t bytes, int reps, double* gbps);

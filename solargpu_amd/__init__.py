@@ -68,6 +68,15 @@ class Info(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("dirs_per_group", ctypes.c_int32), ("hours_per_pass", ctypes.c_int32),
+                ("waves_per_group", ctypes.c_int32), ("from_cache", ctypes.c_int32),
+                ("rays_per_s", ctypes.c_double), ("voxel_hours_per_s", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _check(code: int) -> None:
     if code != 0:
         raise SolarGPUError(code, _lib.sgpu_last_error().decode(errors="replace"))
@@ -371,6 +380,20 @@ class Context:
 
     def sync(self) -> None:
         _check(_lib.sgpu_sync(self._h))
+
+    def autotune(self, fmt: int = FMT_BARLE) -> dict:
+        t = Tuning()
+        _check(_lib.sgpu_autotune(self._h, ctypes.c_uint(fmt), ctypes.byref(t)))
+        return t.as_dict()
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(_lib.sgpu_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, dirs_per_group: int = 32, hours_per_pass: int = 16, waves_per_group: int = 4) -> None:
+        t = Tuning(dirs_per_group, hours_per_pass, waves_per_group, 0, 0.0, 0.0)
+        _check(_lib.sgpu_set_tuning(self._h, ctypes.byref(t)))
 
     def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:
         """GB/s (read + write) of a plain float4 device copy of nbytes."""

@@ -33,8 +33,6 @@ namespace sgpu {
 namespace {
 
 constexpr int kSolarBlock = 256;
-constexpr int kWaves = 4;                 // waves per workgroup in k_reflect
-constexpr int kReflBlock = 64 * kWaves;
 constexpr int kXcds = 8;
 
 __device__ __forceinline__ float hour_irradiance(const HourConst& hc, float cs, float ss, float ca, float sa,
@@ -233,11 +231,12 @@ __device__ __forceinline__ void wave_decode(const uint8_t* __restrict__ p, int s
     }
 }
 
-template <int K, int FMT, bool DBG>
-__global__ __launch_bounds__(kReflBlock) void k_reflect(HourArgs a) {
-    __shared__ float rbuf[kWaves][64][K + 1];
-    __shared__ Recv rec[kWaves][64];
-    __shared__ int64_t beg[kWaves][64 + 1];
+/* WAVES waves per workgroup (2 or 4): the LDS footprint / occupancy knob the autotuner picks. */
+template <int K, int FMT, bool DBG, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
+    __shared__ float rbuf[WAVES][64][K + 1];
+    __shared__ Recv rec[WAVES][64];
+    __shared__ int64_t beg[WAVES][64 + 1];
 
     /* XCD-aware block order: block b runs on XCD b % 8; give each XCD a contiguous range. */
     const unsigned nb = gridDim.x;
@@ -248,7 +247,7 @@ __global__ __launch_bounds__(kReflBlock) void k_reflect(HourArgs a) {
     /* positions p index the neighbour offsets and the accumulators; p is the voxel id
        itself unless a subset of voxel ids is given */
     const int64_t n = a.subset ? a.n_sub : a.n_total;
-    const int64_t p0 = ((int64_t)lb * kWaves + wave) * 64;
+    const int64_t p0 = ((int64_t)lb * WAVES + wave) * 64;
     const int nv = (p0 < n) ? (int)min<int64_t>(64, n - p0) : 0;
     const int64_t p = p0 + lane;
     const bool live = lane < nv;
@@ -335,20 +334,27 @@ void solar_top_k(const HourArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_solar_top<K>, dim3(g), dim3(kSolarBlock), 0, s, a);
 }
 
-template <int K>
-void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
+template <int K, int WAVES>
+void reflect_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
     const int64_t n = a.subset ? a.n_sub : a.n_total;
-    const int64_t blocks = (n + kReflBlock - 1) / kReflBlock;
+    const int64_t blocks = (n + 64 * WAVES - 1) / (64 * WAVES);
     /* round up to a multiple of the XCD count so the remap covers every block */
     const unsigned g = (unsigned)(((blocks + kXcds - 1) / kXcds) * kXcds);
     const bool dbg = a.dbg_hourly != nullptr;
+    const dim3 blk(64 * WAVES);
     if (fmt == SGPU_FMT_CSR) {
-        if (dbg) hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_CSR, true>), dim3(g), dim3(kReflBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_CSR, false>), dim3(g), dim3(kReflBlock), 0, s, a);
+        if (dbg) hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_CSR, true, WAVES>), dim3(g), blk, 0, s, a);
+        else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_CSR, false, WAVES>), dim3(g), blk, 0, s, a);
     } else {
-        if (dbg) hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, true>), dim3(g), dim3(kReflBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, false>), dim3(g), dim3(kReflBlock), 0, s, a);
+        if (dbg) hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, true, WAVES>), dim3(g), blk, 0, s, a);
+        else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, false, WAVES>), dim3(g), blk, 0, s, a);
     }
+}
+
+template <int K>
+void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
+    if (a.waves == 2) reflect_kw<K, 2>(a, fmt, s);
+    else reflect_kw<K, 4>(a, fmt, s);
 }
 
 }  // namespace

@@ -24,12 +24,12 @@ namespace sgpu {
 namespace {
 
 constexpr int kTraceBlock = 256;
-constexpr int kDirsPerGroup = 32;            // 2592 = 81 x 32; 128 B of raw per voxel per group
 constexpr int kListThreads = 256;
 constexpr int kDirsPerThread = (kDirs + kListThreads - 1) / kListThreads;  // 11
 constexpr int kHashSize = 4096;              // > 1.5 x the 2592 possible distinct hits
 
-template <bool COUNT_STEPS>
+/* G directions per workgroup (2592 = 162 x 16 = 81 x 32 = 54 x 48): the tile shape the autotuner picks. */
+template <bool COUNT_STEPS, int G>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     const int64_t local = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
     if (local >= a.count) return;
@@ -40,11 +40,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     const float gxf = (float)gx, gyf = (float)gy;
     const unsigned W = (unsigned)a.W, H = (unsigned)a.H;
     const float* __restrict__ grid = a.grid;
-    int32_t* out = a.raw + local * kDirs + blockIdx.y * kDirsPerGroup;
+    int32_t* out = a.raw + local * kDirs + blockIdx.y * G;
     unsigned long long steps = 0;
 
-    for (int k = 0; k < kDirsPerGroup; ++k) {
-        const DirConst dc = a.dirs[blockIdx.y * kDirsPerGroup + k];   // wave-uniform
+    for (int k = 0; k < G; ++k) {
+        const DirConst dc = a.dirs[blockIdx.y * G + k];   // wave-uniform
         out[k] = dda_first_hit<COUNT_STEPS>(grid, a.W, W, H, a.maxz, gxf, gyf, base_z, dc, steps);
     }
     if (COUNT_STEPS) atomicAdd(a.steps, steps);
@@ -218,9 +218,24 @@ __global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int32_t* c, in
 
 void launch_trace(const TraceArgs& a, hipStream_t s) {
     if (a.count <= 0) return;
-    dim3 grid((unsigned)((a.count + kTraceBlock - 1) / kTraceBlock), kDirs / kDirsPerGroup);
-    if (a.steps) hipLaunchKernelGGL(k_trace<true>, grid, dim3(kTraceBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_trace<false>, grid, dim3(kTraceBlock), 0, s, a);
+    const int G = a.group ? a.group : 32;
+    const unsigned bx = (unsigned)((a.count + kTraceBlock - 1) / kTraceBlock);
+    dim3 grid(bx, (unsigned)(kDirs / G));
+    switch (G) {
+        case 16:
+            if (a.steps) hipLaunchKernelGGL((k_trace<true, 16>), grid, dim3(kTraceBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_trace<false, 16>), grid, dim3(kTraceBlock), 0, s, a);
+            break;
+        case 32:
+            if (a.steps) hipLaunchKernelGGL((k_trace<true, 32>), grid, dim3(kTraceBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_trace<false, 32>), grid, dim3(kTraceBlock), 0, s, a);
+            break;
+        case 48:
+            if (a.steps) hipLaunchKernelGGL((k_trace<true, 48>), grid, dim3(kTraceBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_trace<false, 48>), grid, dim3(kTraceBlock), 0, s, a);
+            break;
+        default: fail(SGPU_ERR_ARG, "directions per group must be 16, 32 or 48");
+    }
     SGPU_HIP(hipGetLastError());
 }
 

@@ -107,6 +107,7 @@ HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
     a.hourly_top = c->hourly_top;
     a.sum = c->sum;
     a.comp = c->comp;
+    a.waves = c->tune.waves_per_group;
     if (fmt == SGPU_FMT_CSR) {
         a.nb_off = c->csr_off;
         a.nb_ids = c->csr_ids;
@@ -127,9 +128,8 @@ void check_stage2_ready(sgpu_ctx* c, unsigned fmt) {
 }
 
 int autotuned_hours_per_pass(const sgpu_ctx* c, unsigned fmt) {
-    (void)c;
     (void)fmt;
-    return 16;
+    return c->tune.hours_per_pass;
 }
 
 /* Selected sunlit hours and the passes over them. */
@@ -191,7 +191,7 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
         for (int64_t v0 = 0; v0 < N; v0 += chunk) {
             const int64_t cnt = std::min(chunk, N - v0);
             TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, cnt, raw,
-                         steps, subset};
+                         steps, subset, c->tune.dirs_per_group};
             SGPU_HIP(hipEventRecord(ev_trace.a, c->stream));
             launch_trace(ta, c->stream);
             SGPU_HIP(hipEventRecord(ev_trace.b, c->stream));
@@ -424,6 +424,8 @@ void do_hour_loop_graph(sgpu_ctx* c, unsigned fmt, int K, int replays) {
 }  // namespace
 
 namespace sgpu {
+
+HourArgs stage2_args(sgpu_ctx* c, unsigned fmt) { return hour_args(c, fmt); }
 
 SubsetLists subset_viewshed(sgpu_ctx* c, const int32_t* subset, int64_t n, bool barle) {
     SubsetLists s;

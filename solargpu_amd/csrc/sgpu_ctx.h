@@ -130,6 +130,7 @@ struct sgpu_ctx {
     bool count_steps = false;
     sgpu_info info{};
     sgpu::SurrogateState sur;
+    sgpu_tuning tune{32, 16, 4, 0, 0.0, 0.0};   // trace group, hours per pass, waves; from_cache
 
     void set_device() const { SGPU_HIP(hipSetDevice(device)); }
 
@@ -169,7 +170,9 @@ struct sgpu_ctx {
 
 namespace sgpu {
 
-/* Shared pieces of the pipeline driver (sgpu_capi.hip) used by the surrogate path. */
+/* Shared pieces of the pipeline driver (sgpu_capi.hip) used by the surrogate path and
+   the autotuner. */
+HourArgs stage2_args(sgpu_ctx* c, unsigned fmt);
 void ensure_accumulators(sgpu_ctx* c, int K);
 void ensure_hours_d(sgpu_ctx* c, const std::vector<HourConst>& hs);
 void zero_floats(float* p, int64_t n, hipStream_t s);

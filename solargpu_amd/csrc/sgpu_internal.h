@@ -86,6 +86,7 @@ struct TraceArgs {
     int32_t* raw;           // count x kDirs first-hit ids (+1), 0 = none
     unsigned long long* steps;  // optional DDA step counter
     const int32_t* subset;  // optional voxel id list
+    int group;              // directions per workgroup: 16, 32 or 48 (0 = 32)
 };
 void launch_trace(const TraceArgs& a, hipStream_t s);
 
@@ -137,6 +138,7 @@ struct HourArgs {
     float* dbg_reflected;
     const int32_t* subset;     // optional voxel ids; then nb_off, sum, comp, dbg_* are per position
     int64_t n_sub;
+    int waves;                 // waves per k_reflect workgroup: 2 or 4 (0 = 4)
 };
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s);
 void launch_reflect_accum(const HourArgs& a, int K, unsigned format, hipStream_t s);

@@ -1,0 +1,51 @@
+"""Autotuner: every launch shape it can pick gives identical results, and a
+tuned key is served from the cache the second time."""
+import numpy as np
+import pytest
+
+import solargpu_amd as sg
+import oracle_lib as orc
+from scenes import oracle_lists, sky, synth_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(sc):
+    az_alt, dni, dhi, _ = sky()
+    ctx = sg.Context(0)
+    ctx.set_scene(sc)
+    ctx.set_sky(az_alt, dni, dhi)
+    return ctx
+
+
+def test_every_shape_is_results_neutral():
+    sc = synth_scene(12, 64, 64)
+    ref = oracle_lists(12, 64, 64)
+    az_alt, dni, dhi, bins = sky()
+    exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
+    with _ctx(sc) as ctx:
+        for G, K, waves in ((16, 8, 2), (48, 32, 4), (32, 16, 2)):
+            ctx.set_tuning(G, K, waves)
+            ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
+            off, ids = ctx.csr()
+            np.testing.assert_array_equal(ids, ref["ids"])
+            np.testing.assert_array_equal(ctx.svf(), ref["svf"])
+            ctx.hour_loop(sg.FMT_BARLE, 0)
+            irr = ctx.irradiance()
+            assert (np.abs(irr - exact) / np.maximum(exact, 1.0)).max() <= 1e-4
+
+
+def test_autotune_measures_then_hits_the_cache(tmp_path, monkeypatch):
+    monkeypatch.setenv("SGPU_AUTOTUNE_CACHE", str(tmp_path / "tune.txt"))
+    sc = synth_scene(11, 48, 40)
+    with _ctx(sc) as ctx:
+        ctx.viewshed(sg.FMT_BARLE)
+        t = ctx.autotune(sg.FMT_BARLE)
+        assert t["from_cache"] == 0 and t["rays_per_s"] > 0 and t["voxel_hours_per_s"] > 0
+        assert t["dirs_per_group"] in (16, 32, 48) and t["hours_per_pass"] in (8, 16, 32)
+    with _ctx(sc) as ctx:
+        ctx.viewshed(sg.FMT_BARLE)
+        t2 = ctx.autotune(sg.FMT_BARLE)
+        assert t2["from_cache"] == 1
+        assert (t2["dirs_per_group"], t2["hours_per_pass"], t2["waves_per_group"]) == \
+            (t["dirs_per_group"], t["hours_per_pass"], t["waves_per_group"])
