@@ -128,6 +128,13 @@ int sgpu_set_scene(sgpu_ctx* ctx, const float* grid, int grid_width, int grid_he
 int sgpu_set_sky(sgpu_ctx* ctx, const float* az_alt, const float* dni, const float* dhi);
 /* Override the viewshed chunk (voxels per out-of-core chunk); 0 = from free memory. (new) */
 int sgpu_set_chunk_voxels(sgpu_ctx* ctx, int64_t chunk_voxels);
+/* Where sgpu_viewshed leaves the neighbour lists (new).  HOST: every chunk's lists go to
+   pinned host memory as soon as they are built (the device holds one chunk at a time), and
+   sgpu_hour_loop streams them back tile by tile, double-buffered with hipMemcpyAsync on a
+   second stream overlapping the reflection kernel.  Results are identical to DEVICE. */
+#define SGPU_LISTS_DEVICE 0
+#define SGPU_LISTS_HOST 1
+int sgpu_set_list_residency(sgpu_ctx* ctx, int residency);
 /* Stage 1: trace every voxel x 2592 directions; builds the SVF bitmap and the requested formats. */
 int sgpu_viewshed(sgpu_ctx* ctx, unsigned formats);
 int sgpu_get_svf(sgpu_ctx* ctx, uint8_t* out /* svf_bytes_per_voxel * num_total */);

@@ -301,6 +301,10 @@ class Context:
     def set_chunk_voxels(self, n: int) -> None:
         _check(_lib.sgpu_set_chunk_voxels(self._h, _I64(n)))
 
+    def set_list_residency(self, host: bool) -> None:
+        """host=True: lists in pinned host memory, streamed to the GPU per pass."""
+        _check(_lib.sgpu_set_list_residency(self._h, 1 if host else 0))
+
     def set_kernel_timing(self, on: bool) -> None:
         _check(_lib.sgpu_set_kernel_timing(self._h, int(on)))
 

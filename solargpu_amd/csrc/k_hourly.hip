@@ -246,12 +246,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     /* positions p index the neighbour offsets and the accumulators; p is the voxel id
        itself unless a subset of voxel ids is given */
-    const int64_t n = a.subset ? a.n_sub : a.n_total;
+    const int64_t n = a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total);
     const int64_t p0 = ((int64_t)lb * WAVES + wave) * 64;
     const int nv = (p0 < n) ? (int)min<int64_t>(64, n - p0) : 0;
     const int64_t p = p0 + lane;
     const bool live = lane < nv;
-    const int64_t v = live ? (a.subset ? (int64_t)a.subset[p] : p) : 0;
+    const int64_t v = live ? (a.subset ? (int64_t)a.subset[p] : a.v_base + p) : 0;
     const float inv_w = 1.0f / (float)a.W;
 
     /* phase 1: lane = voxel */
@@ -336,7 +336,7 @@ void solar_top_k(const HourArgs& a, hipStream_t s) {
 
 template <int K, int WAVES>
 void reflect_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
-    const int64_t n = a.subset ? a.n_sub : a.n_total;
+    const int64_t n = a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total);
     const int64_t blocks = (n + 64 * WAVES - 1) / (64 * WAVES);
     /* round up to a multiple of the XCD count so the remap covers every block */
     const unsigned g = (unsigned)(((blocks + kXcds - 1) / kXcds) * kXcds);
@@ -376,7 +376,7 @@ void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
 }
 
 void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s) {
-    if ((a.subset ? a.n_sub : a.n_total) <= 0) return;
+    if ((a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total)) <= 0) return;
     if (fmt != SGPU_FMT_CSR && fmt != SGPU_FMT_BARLE) fail(SGPU_ERR_ARG, "format must be CSR or BARLE");
     switch (K) {
         case 1: reflect_k<1>(a, fmt, s); break;

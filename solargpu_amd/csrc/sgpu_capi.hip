@@ -139,8 +139,86 @@ std::vector<HourConst> select_hours(const sgpu_ctx* c, int stride) {
     return hs;
 }
 
+bool lists_on_host(const sgpu_ctx* c, unsigned fmt) {
+    return (fmt == SGPU_FMT_CSR) ? (c->nnz >= 0 && !c->csr_off) : (c->br_size >= 0 && !c->br_off);
+}
+
+/* Host-resident lists: per pass, the solar term for every top cell, then the voxel tiles
+   (one per viewshed chunk) copied by hipMemcpyAsync on the copy stream into one of two
+   device buffers while the reflection kernel works on the other. */
+void run_hour_passes_streamed(sgpu_ctx* c, unsigned fmt, int K, int nsel, std::vector<EventPair>* ev_refl,
+                              std::vector<EventPair>* ev_solar) {
+    const std::vector<HostPiece>& pieces = (fmt == SGPU_FMT_CSR) ? c->host.csr : c->host.barle;
+    const int64_t max_bytes = (fmt == SGPU_FMT_CSR) ? c->host.max_bytes_csr : c->host.max_bytes_barle;
+    if (!c->copy_stream) SGPU_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    int64_t* doff[2] = {nullptr, nullptr};
+    uint8_t* ddata[2] = {nullptr, nullptr};
+    hipEvent_t ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        for (int b = 0; b < 2; ++b) {
+            dfree(doff[b]);
+            dfree(ddata[b]);
+            if (ready[b]) (void)hipEventDestroy(ready[b]);
+            if (freed[b]) (void)hipEventDestroy(freed[b]);
+        }
+    };
+    try {
+        for (int b = 0; b < 2; ++b) {
+            doff[b] = dalloc<int64_t>(c->host.max_count + 1);
+            ddata[b] = dalloc<uint8_t>((size_t)std::max<int64_t>(max_bytes, 1));
+            SGPU_HIP(hipEventCreateWithFlags(&ready[b], hipEventDisableTiming));
+            SGPU_HIP(hipEventCreateWithFlags(&freed[b], hipEventDisableTiming));
+            SGPU_HIP(hipEventRecord(freed[b], c->stream));
+        }
+        HourArgs a = hour_args(c, fmt);
+        int pass = 0, slot = 0;
+        for (int h0 = 0; h0 < nsel; h0 += K, ++pass) {
+            a.hours = c->hours_d + h0;
+            a.nh = std::min(K, nsel - h0);
+            if (ev_solar) SGPU_HIP(hipEventRecord((*ev_solar)[pass].a, c->stream));
+            launch_solar_top(a, K, c->stream);
+            if (ev_solar) SGPU_HIP(hipEventRecord((*ev_solar)[pass].b, c->stream));
+            if (ev_refl) SGPU_HIP(hipEventRecord((*ev_refl)[pass].a, c->stream));
+            for (const HostPiece& p : pieces) {
+                const int b = slot;
+                slot ^= 1;
+                SGPU_HIP(hipStreamWaitEvent(c->copy_stream, freed[b], 0));
+                SGPU_HIP(hipMemcpyAsync(doff[b], p.off, (size_t)(p.count + 1) * 8, hipMemcpyHostToDevice,
+                                        c->copy_stream));
+                if (p.bytes)
+                    SGPU_HIP(hipMemcpyAsync(ddata[b], p.data, (size_t)p.bytes, hipMemcpyHostToDevice, c->copy_stream));
+                SGPU_HIP(hipEventRecord(ready[b], c->copy_stream));
+                SGPU_HIP(hipStreamWaitEvent(c->stream, ready[b], 0));
+                HourArgs t = a;
+                t.v_base = p.v0;
+                t.n_sub = p.count;
+                t.nb_off = doff[b];
+                t.nb_ids = reinterpret_cast<const int32_t*>(ddata[b]);
+                t.nb_bytes = ddata[b];
+                t.sum = c->sum + p.v0;
+                t.comp = c->comp + p.v0;
+                launch_reflect_accum(t, K, fmt, c->stream);
+                SGPU_HIP(hipEventRecord(freed[b], c->stream));
+            }
+            if (ev_refl) SGPU_HIP(hipEventRecord((*ev_refl)[pass].b, c->stream));
+        }
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->copy_stream));
+    } catch (...) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->copy_stream);
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
 void run_hour_passes(sgpu_ctx* c, unsigned fmt, int K, int nsel, std::vector<EventPair>* ev_refl,
                      std::vector<EventPair>* ev_solar) {
+    if (lists_on_host(c, fmt)) {
+        run_hour_passes_streamed(c, fmt, K, nsel, ev_refl, ev_solar);
+        return;
+    }
     HourArgs a = hour_args(c, fmt);
     int pass = 0;
     for (int h0 = 0; h0 < nsel; h0 += K, ++pass) {
@@ -170,13 +248,67 @@ struct ListsOut {
     int64_t nnz = 0;
 };
 
-ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf, unsigned long long* steps) {
+void encode_barle(sgpu_ctx* c, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,
+                  uint8_t** br_bytes, int64_t* size);
+
+void* pinned(size_t bytes) {
+    void* p = nullptr;
+    SGPU_HIP(hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+    return p;
+}
+
+/* Copy one chunk's lists (chunk-local device offsets and ids) into pinned host pieces. */
+void host_take(sgpu_ctx* c, bool csr, bool barle, int64_t v0, int64_t cnt, const int64_t* off_d,
+               const int32_t* ids_d, int64_t nnz) {
+    HostLists& H = c->host;
+    H.max_count = std::max(H.max_count, cnt);
+    if (csr) {
+        H.csr.push_back(HostPiece{v0, cnt, nullptr, nullptr, nnz * 4});
+        HostPiece& p = H.csr.back();
+        p.off = static_cast<int64_t*>(pinned((size_t)(cnt + 1) * 8));
+        p.data = static_cast<uint8_t*>(pinned((size_t)p.bytes));
+        SGPU_HIP(hipMemcpyAsync(p.off, off_d, (size_t)(cnt + 1) * 8, hipMemcpyDeviceToHost, c->stream));
+        if (p.bytes) SGPU_HIP(hipMemcpyAsync(p.data, ids_d, (size_t)p.bytes, hipMemcpyDeviceToHost, c->stream));
+        H.max_bytes_csr = std::max(H.max_bytes_csr, p.bytes);
+        H.total_csr += p.bytes;
+    }
+    if (barle) {
+        int64_t* boff = nullptr;
+        uint8_t* bytes = nullptr;
+        int64_t size = 0;
+        encode_barle(c, off_d, ids_d, cnt, &boff, &bytes, &size);
+        try {
+            H.barle.push_back(HostPiece{v0, cnt, nullptr, nullptr, size});
+            HostPiece& p = H.barle.back();
+            p.off = static_cast<int64_t*>(pinned((size_t)(cnt + 1) * 8));
+            p.data = static_cast<uint8_t*>(pinned((size_t)size));
+            SGPU_HIP(hipMemcpyAsync(p.off, boff, (size_t)(cnt + 1) * 8, hipMemcpyDeviceToHost, c->stream));
+            if (size) SGPU_HIP(hipMemcpyAsync(p.data, bytes, (size_t)size, hipMemcpyDeviceToHost, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            H.max_bytes_barle = std::max(H.max_bytes_barle, size);
+            H.total_barle += size;
+        } catch (...) {
+            dfree(boff);
+            dfree(bytes);
+            throw;
+        }
+        dfree(boff);
+        dfree(bytes);
+    }
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+}
+
+/* host_csr / host_barle: send every chunk's lists to pinned host memory instead of keeping
+   them on the device (then out.off / out.ids stay null). */
+ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf, unsigned long long* steps,
+                     bool host_csr = false, bool host_barle = false) {
+    const bool to_host = host_csr || host_barle;
     const int64_t chunk = choose_chunk(c, N);
     c->info.chunk_voxels = chunk;
     c->info.viewshed_chunks = (int32_t)((N + chunk - 1) / chunk);
     EventPair ev_trace, ev_lists;
     ListsOut out;
-    out.off = dalloc<int64_t>(N + 1);
+    if (!to_host) out.off = dalloc<int64_t>(N + 1);
     int32_t* raw = nullptr;
     int32_t* counts = nullptr;
     int64_t* off = nullptr;
@@ -206,14 +338,29 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
             pieces.push_back({ids, piece});
             la.ids = ids;
             launch_lists(la, true, c->stream);
-            const unsigned g = (unsigned)((cnt + 255) / 256);
-            hipLaunchKernelGGL(k_add_base, dim3(g), dim3(256), 0, c->stream, off, cnt, base, out.off + v0);
-            SGPU_HIP(hipGetLastError());
+            if (!to_host) {
+                const unsigned g = (unsigned)((cnt + 255) / 256);
+                hipLaunchKernelGGL(k_add_base, dim3(g), dim3(256), 0, c->stream, off, cnt, base, out.off + v0);
+                SGPU_HIP(hipGetLastError());
+            }
             SGPU_HIP(hipEventRecord(ev_lists.b, c->stream));
             SGPU_HIP(hipStreamSynchronize(c->stream));
             c->info.t_trace_ms += ev_trace.ms();
             c->info.t_lists_ms += ev_lists.ms();
+            if (to_host) {
+                host_take(c, host_csr, host_barle, v0, cnt, off, ids, piece);
+                dfree(pieces.back().first);
+                pieces.pop_back();
+            }
             base += piece;
+        }
+        if (to_host) {
+            out.nnz = base;
+            dfree(raw);
+            dfree(counts);
+            dfree(off);
+            dfree(scratch);
+            return out;
         }
         SGPU_HIP(hipMemcpyAsync(out.off + N, &base, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
         if (pieces.size() == 1) {
@@ -304,8 +451,10 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
     unsigned long long* steps = c->count_steps ? dalloc<unsigned long long>(1) : nullptr;
     if (steps) SGPU_HIP(hipMemsetAsync(steps, 0, sizeof(unsigned long long), c->stream));
     ListsOut L;
+    const bool to_host = c->residency == SGPU_LISTS_HOST;
     try {
-        L = build_lists(c, nullptr, N, c->svf, steps);
+        L = build_lists(c, nullptr, N, c->svf, steps, to_host && (formats & SGPU_FMT_CSR),
+                        to_host && (formats & SGPU_FMT_BARLE));
         if (steps) {
             unsigned long long st = 0;
             SGPU_HIP(hipMemcpyAsync(&st, steps, sizeof st, hipMemcpyDeviceToHost, c->stream));
@@ -317,11 +466,21 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
         throw;
     }
     dfree(steps);
+    c->svf_ok = true;
+    c->info.nnz = L.nnz;
+    if (to_host) {
+        /* lists live in pinned host pieces; null device pointers mark them host-resident */
+        c->nnz = (formats & SGPU_FMT_CSR) ? L.nnz : -1;
+        c->br_size = (formats & SGPU_FMT_BARLE) ? c->host.total_barle : -1;
+        c->info.barle_bytes = c->host.total_barle;
+        SGPU_HIP(hipEventRecord(all.b, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->info.t_viewshed_ms = all.ms();
+        return;
+    }
     c->csr_off = L.off;
     c->csr_ids = L.ids;
     c->nnz = L.nnz;
-    c->svf_ok = true;
-    c->info.nnz = L.nnz;
     if (formats & SGPU_FMT_BARLE) {
         SGPU_HIP(hipEventRecord(ev_enc.a, c->stream));
         encode_barle(c, c->csr_off, c->csr_ids, N, &c->br_off, &c->br_bytes, &c->br_size);
@@ -336,7 +495,7 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
 }
 
 void do_barle_encode(sgpu_ctx* c) {
-    need(c->nnz >= 0, SGPU_ERR_STATE, "CSR lists missing");
+    need(c->nnz >= 0 && c->csr_off, SGPU_ERR_STATE, "device-resident CSR lists missing");
     dfree(c->br_off);
     dfree(c->br_bytes);
     c->br_size = -1;
@@ -381,6 +540,7 @@ void do_hour_loop(sgpu_ctx* c, unsigned fmt, int K, int stride) {
 
 void do_hour_loop_graph(sgpu_ctx* c, unsigned fmt, int K, int replays) {
     check_stage2_ready(c, fmt);
+    need(!lists_on_host(c, fmt), SGPU_ERR_STATE, "graph capture needs device-resident lists");
     if (K == 0) K = autotuned_hours_per_pass(c, fmt);
     need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16 or 32");
     need(replays >= 1, SGPU_ERR_ARG, "replays must be >= 1");
@@ -486,6 +646,7 @@ void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const Subse
    annual top-cell totals in place of one hour's values (labels and checks only). */
 void annual_reflected_linear(sgpu_ctx* c, unsigned fmt, const float* dd, float* out_dev) {
     check_stage2_ready(c, fmt);
+    need(!lists_on_host(c, fmt), SGPU_ERR_STATE, "needs device-resident lists");
     ensure_hours_d(c, c->hours);
     HourArgs a = hour_args(c, fmt);
     a.hourly_top = const_cast<float*>(dd);   // cell-major with K = 1: the top voxels' annual totals
@@ -717,11 +878,35 @@ int sgpu_get_svf(sgpu_ctx* c, uint8_t* out) {
     });
 }
 
+/* Host-resident pieces -> one offsets array (rebased) and one concatenated data array. */
+static void gather_host(const std::vector<HostPiece>& pieces, int64_t n_total, int64_t* off, uint8_t* data,
+                        size_t elem) {
+    int64_t pos = 0;
+    for (const HostPiece& p : pieces) {
+        for (int64_t i = 0; i < p.count; ++i) off[p.v0 + i] = pos / (int64_t)elem + p.off[i];
+        if (p.bytes) std::memcpy(data + pos, p.data, (size_t)p.bytes);
+        pos += p.bytes;
+    }
+    off[n_total] = pos / (int64_t)elem;
+}
+
+int sgpu_set_list_residency(sgpu_ctx* c, int residency) {
+    return guarded([&] {
+        need(c && (residency == SGPU_LISTS_DEVICE || residency == SGPU_LISTS_HOST), SGPU_ERR_ARG, "bad residency");
+        c->residency = residency;
+    });
+}
+
 int sgpu_get_csr(sgpu_ctx* c, int64_t* off, int32_t* ids) {
     return guarded([&] {
         need(c && off, SGPU_ERR_ARG, "null argument");
         need(c->nnz >= 0, SGPU_ERR_STATE, "no CSR lists");
         c->set_device();
+        if (!c->csr_off) {
+            need(ids != nullptr || c->nnz == 0, SGPU_ERR_ARG, "null ids");
+            gather_host(c->host.csr, c->n_total, off, reinterpret_cast<uint8_t*>(ids), sizeof(int32_t));
+            return;
+        }
         SGPU_HIP(hipMemcpyAsync(off, c->csr_off, (c->n_total + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
                                 c->stream));
         if (c->nnz > 0) {
@@ -737,6 +922,11 @@ int sgpu_get_barle(sgpu_ctx* c, int64_t* off, uint8_t* bytes) {
         need(c && off, SGPU_ERR_ARG, "null argument");
         need(c->br_size >= 0, SGPU_ERR_STATE, "no ba-RLE streams");
         c->set_device();
+        if (!c->br_off) {
+            need(bytes != nullptr || c->br_size == 0, SGPU_ERR_ARG, "null bytes");
+            gather_host(c->host.barle, c->n_total, off, bytes, 1);
+            return;
+        }
         SGPU_HIP(hipMemcpyAsync(off, c->br_off, (c->n_total + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
                                 c->stream));
         if (c->br_size > 0) {
@@ -750,7 +940,7 @@ int sgpu_get_barle(sgpu_ctx* c, int64_t* off, uint8_t* bytes) {
 int sgpu_get_csr_range(sgpu_ctx* c, int64_t v0, int64_t count, int64_t* off, int32_t* ids, int64_t cap) {
     return guarded([&] {
         need(c && off, SGPU_ERR_ARG, "null argument");
-        need(c->nnz >= 0, SGPU_ERR_STATE, "no CSR lists");
+        need(c->nnz >= 0 && c->csr_off, SGPU_ERR_STATE, "no device-resident CSR lists");
         need(v0 >= 0 && count >= 0 && v0 + count <= c->n_total, SGPU_ERR_ARG, "voxel range out of bounds");
         c->set_device();
         SGPU_HIP(hipMemcpyAsync(off, c->csr_off + v0, (count + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
@@ -914,6 +1104,7 @@ int sgpu_hour_single(sgpu_ctx* c, unsigned fmt, int hour_index, float* hourly_ou
     return guarded([&] {
         need(c && hourly_out && refl_out, SGPU_ERR_ARG, "null argument");
         check_stage2_ready(c, fmt);
+        need(!lists_on_host(c, fmt), SGPU_ERR_STATE, "needs device-resident lists");
         c->set_device();
         const HourConst* hc = nullptr;
         for (const HourConst& h : c->hours)

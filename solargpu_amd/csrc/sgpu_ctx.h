@@ -68,6 +68,32 @@ struct EventPair {
     }
 };
 
+/* Neighbour lists kept in pinned host memory, one piece per viewshed chunk (voxels
+   [v0, v0+count), chunk-local int64 offsets), streamed to the GPU tile by tile in stage 2. */
+struct HostPiece {
+    int64_t v0 = 0, count = 0;
+    int64_t* off = nullptr;   // count + 1, pinned
+    uint8_t* data = nullptr;  // int32 ids (CSR) or stream bytes (ba-RLE), pinned
+    int64_t bytes = 0;
+};
+struct HostLists {
+    std::vector<HostPiece> csr, barle;
+    int64_t max_count = 0, max_bytes_csr = 0, max_bytes_barle = 0;
+    int64_t total_csr = 0, total_barle = 0;
+    static void drop(std::vector<HostPiece>& v) {
+        for (HostPiece& p : v) {
+            if (p.off) (void)hipHostFree(p.off);
+            if (p.data) (void)hipHostFree(p.data);
+        }
+        v.clear();
+    }
+    void release() {
+        drop(csr);
+        drop(barle);
+        max_count = max_bytes_csr = max_bytes_barle = total_csr = total_barle = 0;
+    }
+};
+
 /* Surrogate-path buffers of a context (sgpu_surrogate.hip). */
 struct SurrogateState {
     uint16_t* X = nullptr;       // features [N][64] bf16
@@ -131,6 +157,9 @@ struct sgpu_ctx {
     sgpu_info info{};
     sgpu::SurrogateState sur;
     sgpu_tuning tune{32, 16, 4, 0, 0.0, 0.0};   // trace group, hours per pass, waves; from_cache
+    int residency = SGPU_LISTS_DEVICE;          // where sgpu_viewshed leaves the lists
+    sgpu::HostLists host;                       // the lists when host-resident
+    hipStream_t copy_stream = nullptr;          // H2D tile copies of the streamed hour loop
 
     void set_device() const { SGPU_HIP(hipSetDevice(device)); }
 
@@ -145,6 +174,7 @@ struct sgpu_ctx {
         br_size = -1;
         irr_ok = false;
         sur.release();
+        host.release();
     }
     void drop_scene() {
         sgpu::dfree(grid);
@@ -165,6 +195,7 @@ struct sgpu_ctx {
         sgpu::dfree(dirs);
         sgpu::dfree(hours_d);
         if (stream) (void)hipStreamDestroy(stream);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
     }
 };
 

@@ -136,8 +136,11 @@ struct HourArgs {
     float* comp;               // Kahan compensation
     float* dbg_hourly;         // optional single-hour outputs
     float* dbg_reflected;
-    const int32_t* subset;     // optional voxel ids; then nb_off, sum, comp, dbg_* are per position
+    /* Positions p in [0, n_sub) (all n_total voxels when n_sub == 0) index nb_off, sum, comp
+       and dbg_*; the voxel of position p is subset[p] if given, else v_base + p. */
+    const int32_t* subset;
     int64_t n_sub;
+    int64_t v_base;
     int waves;                 // waves per k_reflect workgroup: 2 or 4 (0 = 4)
 };
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s);

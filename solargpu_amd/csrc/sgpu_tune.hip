@@ -104,7 +104,7 @@ double time_passes(sgpu_ctx* c, unsigned fmt, int K, int waves, int passes) {
 void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
     need(c->scene && c->sky_set, SGPU_ERR_STATE, "scene/sky not set");
     need(fmt == SGPU_FMT_CSR || fmt == SGPU_FMT_BARLE, SGPU_ERR_ARG, "bad format");
-    const bool lists = (fmt == SGPU_FMT_CSR) ? c->nnz >= 0 : c->br_size >= 0;
+    const bool lists = (fmt == SGPU_FMT_CSR) ? (c->nnz >= 0 && c->csr_off) : (c->br_size >= 0 && c->br_off);
     const std::string key = tune_key(c, fmt);
     sgpu_tuning t = c->tune;
     if (lookup(key, t)) {

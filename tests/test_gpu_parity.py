@@ -163,3 +163,27 @@ def test_flat_plane_has_no_upper_hits():
         off, ids = ctx.csr()
         ref = orc.viewshed(sc, sky()[3])
         np.testing.assert_array_equal(ids, ref["ids"])
+
+
+@pytest.mark.parametrize("fmt", [sg.FMT_CSR, sg.FMT_BARLE])
+def test_host_streamed_lists_match_device_resident(fmt):
+    """Out-of-core mode: lists built chunk by chunk into pinned host memory and streamed
+    back per pass (double-buffered copies) give the same lists and bit-identical sums."""
+    sc = synth_scene(13, 37, 53)
+    ref = oracle_lists(13, 37, 53)
+    with _ctx(sc) as ctx:
+        ctx.hour_loop(fmt, 8)
+        resident = ctx.irradiance()
+    with _ctx(sc, formats=0, chunk=500) as ctx:
+        ctx.set_list_residency(True)
+        ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
+        assert ctx.info()["viewshed_chunks"] > 1
+        off, ids = ctx.csr()
+        np.testing.assert_array_equal(off, ref["off"])
+        np.testing.assert_array_equal(ids, ref["ids"])
+        boff, data = ctx.barle()
+        eoff, edata = orc.barle_encode_all(ref["off"], ref["ids"])
+        np.testing.assert_array_equal(boff, eoff)
+        np.testing.assert_array_equal(data, edata)
+        ctx.hour_loop(fmt, 8)
+        np.testing.assert_array_equal(ctx.irradiance(), resident)
