@@ -105,8 +105,11 @@ def main():
     ap.add_argument("--cell", type=float, default=0.5)
     ap.add_argument("--density", type=float, default=0.45)
     ap.add_argument("--hours-per-pass", type=int, default=0, help="0 = autotuned")
+    ap.add_argument("--hour-stride", type=int, default=1,
+                    help="every n-th sunlit hour only (profiling runs; the reported rate stays per computed hour)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-csr", action="store_true", help="skip the CSR hour-loop report")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
 
@@ -137,10 +140,15 @@ def main():
     def step():
         ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
         a = ctx.info()
-        ctx.hour_loop(fmt, K)
+        ctx.hour_loop(fmt, K, args.hour_stride)
         b = ctx.info()
         return a, b
 
+    tuning = None
+    if not args.no_autotune:
+        ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
+        tuning = ctx.autotune(fmt)
+        log(f"autotune: {tuning}")
     for i in range(args.warmup):
         a, b = step()
         log(f"warmup {i}: viewshed {a['t_viewshed_ms']:.0f} ms, hours {b['t_hours_ms']:.0f} ms")
@@ -180,9 +188,12 @@ def main():
     extra = {}
     copy_gbps = ctx.copy_bandwidth(1 << 30, 20)
     if not args.no_csr:
-        ctx.hour_loop(sg.FMT_CSR, K)
+        ctx.hour_loop(sg.FMT_CSR, 16, args.hour_stride)
         c = ctx.info()
         extra["csr_voxel_hours_per_s"] = N * c["hours_computed"] / (c["t_hours_ms"] / 1e3)
+        extra["csr_hours_per_pass"] = 16
+    if tuning is not None:
+        extra["autotune"] = tuning
     res = {
         "metric": "voxel-hours/s (plain ba-RLE pipeline, stage 2); DDA rays/s (stage 1)",
         "value": vh,

@@ -78,22 +78,28 @@ __device__ __forceinline__ int block_excl_scan(int x, int* tmp, int* total) {
     return before + inc - x;
 }
 
-template <bool WRITE>
-__global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
-    __shared__ int keys[kHashSize];
-    __shared__ int mins[kHashSize];
-    __shared__ unsigned svf_acc[kAzBins * kElBins / 8 + 1];
-    __shared__ int tmp[kListThreads / 64];
+struct ListsLds {
+    int keys[kHashSize];
+    int mins[kHashSize];
+    unsigned svf_acc[kAzBins * kElBins / 8 + 1];
+    int tmp[kListThreads / 64];
+};
 
-    const int64_t local = blockIdx.x;
-    const int64_t v = a.v0 + local;
+/* One voxel's first-occurrence list from its raw row (workgroup-wide).  Pass 1 stores the
+ * count and, when svf is given, the voxel's SVF bytes (svf[byte * n + v]); pass 2 writes
+ * the ids at dst. */
+template <bool WRITE>
+__device__ __forceinline__ void voxel_list(ListsLds& s, const int32_t* raw, const DirConst* dirs, uint8_t* svf,
+                                           int svf_bytes, int64_t n, int64_t v, int32_t* count_out,
+                                           int32_t* dst_base) {
+    int* keys = s.keys;
+    int* mins = s.mins;
+    unsigned* svf_acc = s.svf_acc;
     const int tid = threadIdx.x;
     for (int i = tid; i < kHashSize; i += kListThreads) { keys[i] = 0; mins[i] = INT_MAX; }
     if (!WRITE)
-        for (int i = tid; i < a.svf_bytes; i += kListThreads) svf_acc[i] = 0u;
+        for (int i = tid; i < svf_bytes; i += kListThreads) svf_acc[i] = 0u;
     __syncthreads();
-
-    const int32_t* raw = a.raw + local * kDirs;
     const int d0 = tid * kDirsPerThread;
     int hv[kDirsPerThread];
     int slot[kDirsPerThread];
@@ -104,39 +110,103 @@ __global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
         hv[k] = h;
         slot[k] = 0;
         if (h) {
-            unsigned s = ((unsigned)h * 2654435761u) >> (32 - 12);
+            unsigned sl = ((unsigned)h * 2654435761u) >> (32 - 12);
             for (;;) {
-                const int prev = atomicCAS(&keys[s], 0, h);
+                const int prev = atomicCAS(&keys[sl], 0, h);
                 if (prev == 0 || prev == h) break;
-                s = (s + 1) & (kHashSize - 1);
+                sl = (sl + 1) & (kHashSize - 1);
             }
-            slot[k] = (int)s;
-            atomicMin(&mins[s], d);
-            if (!WRITE && a.svf) {
-                const int dense = (a.dirs[d].info >> 8) - 1;
+            slot[k] = (int)sl;
+            atomicMin(&mins[sl], d);
+            if (!WRITE && svf) {
+                const int dense = (dirs[d].info >> 8) - 1;
                 if (dense >= 0) atomicOr(&svf_acc[dense >> 3], 1u << (dense & 7));
             }
         }
     }
     __syncthreads();
-
     int kept = 0;
 #pragma unroll
     for (int k = 0; k < kDirsPerThread; ++k)
         if (hv[k] && mins[slot[k]] == d0 + k) ++kept;
-
     int total;
-    const int excl = block_excl_scan(kept, tmp, &total);
+    const int excl = block_excl_scan(kept, s.tmp, &total);
     if (!WRITE) {
-        if (tid == 0) a.counts[local] = total;
-        if (a.svf)
-            for (int i = tid; i < a.svf_bytes; i += kListThreads)
-                a.svf[(int64_t)i * a.n_total + v] = (uint8_t)svf_acc[i];
+        if (tid == 0) *count_out = total;
+        if (svf)
+            for (int i = tid; i < svf_bytes; i += kListThreads) svf[(int64_t)i * n + v] = (uint8_t)svf_acc[i];
     } else {
-        int32_t* dst = a.ids + a.offsets[local] + excl;
+        int32_t* dst = dst_base + excl;
 #pragma unroll
         for (int k = 0; k < kDirsPerThread; ++k)
             if (hv[k] && mins[slot[k]] == d0 + k) *dst++ = hv[k];
+    }
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kListThreads) void k_lists_batch(BatchListArgs a) {
+    __shared__ ListsLds s;
+    __shared__ int tile_s;
+    const int64_t local = blockIdx.x;
+    const int64_t g = a.v0 + local;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = a.ntiles - 1;   // last tile with vbase <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.tiles[mid].vbase <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        tile_s = lo;
+    }
+    __syncthreads();
+    const TileDesc& t = a.tiles[tile_s];
+    voxel_list<WRITE>(s, a.raw + local * kDirs, t.dirs, t.svf, t.svf_bytes, t.n_total, g - t.vbase,
+                      a.counts + local, WRITE ? a.ids + a.offsets[local] : nullptr);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
+    __shared__ ListsLds s;
+    const int64_t local = blockIdx.x;
+    voxel_list<WRITE>(s, a.raw + local * kDirs, a.dirs, a.svf, a.svf_bytes, a.n_total, a.v0 + local,
+                      a.counts + local, WRITE ? a.ids + a.offsets[local] : nullptr);
+}
+
+/* Persistent multi-tile DDA: every workgroup takes (256-voxel block, 32-direction group)
+ * items from a global counter until the chunk is exhausted; each lane finds its voxel's
+ * tile (tiles are contiguous in the global voxel order) and traces with that tile's grid. */
+__global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(BatchTraceArgs a) {
+    __shared__ long long item_s;
+    constexpr int G = 32;
+    const int64_t groups = kDirs / G;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = (long long)atomicAdd(a.next, 1ull);
+        __syncthreads();
+        const int64_t item = item_s;
+        __syncthreads();
+        if (item >= a.items) break;   // every wave of the workgroup leaves together
+        const int64_t blk = item / groups, grp = item % groups;
+        const int64_t local = blk * kTraceBlock + threadIdx.x;
+        if (local >= a.count) continue;
+        const int64_t g = a.v0 + local;
+        int lo = 0, hi = a.ntiles - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.tiles[mid].vbase <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const TileDesc& t = a.tiles[lo];
+        const int64_t v = g - t.vbase;
+        const int top = t.pix_parent[v];
+        const int gx = top % t.W, gy = top / t.W;
+        const float base_z = (v < t.n_top) ? t.grid[top] : t.wall_z[v - t.n_top];
+        unsigned long long steps = 0;
+        int32_t* out = a.raw + local * kDirs + grp * G;
+        for (int k = 0; k < G; ++k) {
+            const DirConst dc = t.dirs[grp * G + k];
+            out[k] = dda_first_hit<false>(t.grid, t.W, (unsigned)t.W, (unsigned)t.H, t.maxz, (float)gx, (float)gy,
+                                          base_z, dc, steps);
+        }
     }
 }
 
@@ -245,6 +315,19 @@ void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s) {
     dim3 grid((unsigned)a.count);
     if (write_pass) hipLaunchKernelGGL(k_lists<true>, grid, dim3(kListThreads), 0, s, a);
     else hipLaunchKernelGGL(k_lists<false>, grid, dim3(kListThreads), 0, s, a);
+    SGPU_HIP(hipGetLastError());
+}
+
+void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s) {
+    if (a.count <= 0) return;
+    hipLaunchKernelGGL(k_trace_persistent, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s, a);
+    SGPU_HIP(hipGetLastError());
+}
+
+void launch_lists_batch(const BatchListArgs& a, bool write_pass, hipStream_t s) {
+    if (a.count <= 0) return;
+    if (write_pass) hipLaunchKernelGGL(k_lists_batch<true>, dim3((unsigned)a.count), dim3(kListThreads), 0, s, a);
+    else hipLaunchKernelGGL(k_lists_batch<false>, dim3((unsigned)a.count), dim3(kListThreads), 0, s, a);
     SGPU_HIP(hipGetLastError());
 }
 

@@ -102,6 +102,41 @@ struct ListArgs {
 };
 void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s);
 
+/* Batched viewshed over many tiles: tiles' voxels are concatenated into one index space
+   (tile t owns [vbase, vbase + n_total)); each tile keeps its own heightmap, sky (dense SVF
+   bins in its DirConst table) and SVF bitmap. */
+struct TileDesc {
+    const float* grid;
+    int W, H;
+    int64_t n_top, n_total;
+    const int32_t* pix_parent;
+    const float* wall_z;
+    float maxz;
+    const DirConst* dirs;
+    uint8_t* svf;
+    int svf_bytes;
+    int64_t vbase;
+};
+struct BatchTraceArgs {
+    const TileDesc* tiles;
+    int ntiles;
+    int64_t v0, count;          // global voxel range of this chunk
+    int32_t* raw;               // count x kDirs
+    unsigned long long* next;   // work-item counter (zeroed before the launch)
+    int64_t items;              // ceil(count / 256) blocks x (kDirs / 32) direction groups
+};
+void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s);
+struct BatchListArgs {
+    const TileDesc* tiles;
+    int ntiles;
+    const int32_t* raw;
+    int64_t v0, count;
+    int32_t* counts;
+    const int64_t* offsets;
+    int32_t* ids;
+};
+void launch_lists_batch(const BatchListArgs& a, bool write_pass, hipStream_t s);
+
 /* Exclusive scan of n int32 counts into n+1 int64 offsets (offsets[n] = total). */
 void launch_scan_counts(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* scratch_blocks,
                         hipStream_t s);
