@@ -3,30 +3,35 @@
  * A pass covers K consecutive sunlit hours (K = 1 is the reference's one hour
  * per launch).  Per pass:
  *   k_solar_top  one lane per top cell: direct (SVF-gated) + Perez diffuse for
- *                the K hours -> hourly_top[cell][K] (cell-major: a neighbour
- *                gather is one contiguous K-float load).
- *   k_reflect    one wave per 64 consecutive voxels.
+ *                the K hours -> hourly_top[cell][K] (cell-major: one cell's K
+ *                hours are one contiguous row).
+ *   k_reflect    one wave per 64 consecutive voxels (receivers).
  *                  phase 1, lane = voxel: receiver geometry and the voxel's own
- *                           direct + diffuse for the K hours (registers);
- *                  phase 2, lane = neighbour: for each of the 64 voxels in turn
- *                           the whole wave walks its list -- CSR ids read
- *                           coalesced, or the ba-RLE stream decoded wave-parallel
- *                           (run lookup per element, delta prefix sum across
- *                           lanes) -- computes the hour-invariant Lambertian
- *                           weight once per neighbour, applies it to the K
- *                           gathered hours, and reduces across lanes with a
- *                           butterfly transpose (K + log2(64/K) shuffles);
- *                  phase 3, lane = voxel: FP32 Kahan-compensated annual
- *                           accumulation, hour by hour in hour order
- *                           (total += reflected + hourly).
+ *                           direct + diffuse summed over the K hours;
+ *                  phase 2, one receiver of the wave at a time:
+ *                           lane = neighbour: CSR ids read coalesced, or the
+ *                           ba-RLE stream decoded wave-parallel out of an
+ *                           LDS-staged window (run lookup per element, delta
+ *                           prefix sum across lanes); the hour-invariant
+ *                           Lambertian weight; neighbours with a non-zero weight
+ *                           are compacted into a per-wave LDS list (ballot +
+ *                           prefix count);
+ *                           lane = (neighbour slot, hour): the list is applied to
+ *                           the gathered hourly_top rows, so one load instruction
+ *                           reads 64/K whole rows, into one FP32 accumulator per
+ *                           lane; a butterfly over the wave gives the receiver's
+ *                           reflected irradiance summed over the pass;
+ *                  phase 3, lane = voxel: the pass total (reflected + own)
+ *                           enters the FP32 Kahan-compensated annual sum.
  * Workgroups are remapped so each XCD walks one contiguous eighth of the
  * voxels, keeping its neighbour gathers in its own L2.
  *
  * Formulas: solar_kernel.cu:54-77 (+ Perez :8-50; hour-only terms hoisted into
  * HourConst), reflection_kernels.cu:18-65 (geometry), :70-89 (CSR walk),
- * :95-183 (ba-RLE decode), accumulate solar_kernel.cu:79-85.  Transcendentals
- * and reciprocals use the device's fast forms; the parity bar is 1e-4 on the
- * annual sums against the FP64 oracle. */
+ * :95-183 (ba-RLE decode), accumulate solar_kernel.cu:79-85 (total +=
+ * reflected + hourly, here once per pass of K hours with Kahan compensation).
+ * Transcendentals and reciprocals use the device's fast forms; the parity bar
+ * is 1e-4 on the annual sums against the FP64 oracle. */
 #include "sgpu_internal.h"
 
 namespace sgpu {
@@ -96,86 +101,66 @@ __device__ __forceinline__ void cell_xy(int id, int W, float inv_w, int& x, int&
     y = q;
 }
 
-/* Contribution of neighbour cell `id` (0-based) to the K hourly sums of receiver r
- * (reflection_kernels.cu:18-38). */
-template <int K>
-__device__ __forceinline__ void add_neighbour(int id, const Recv& r, const HourArgs& a, float inv_w, float* R) {
-    if (id < 0 || id >= a.W * a.H) return;
+/* Hour-invariant weight of neighbour cell `id` (0-based) for receiver r, 0 when the pair is
+ * culled (reflection_kernels.cu:18-38): 0.2 cos_recv cos_send / (pi d^2), dx and dy in cells,
+ * dz in metres. */
+__device__ __forceinline__ float neighbour_weight(int id, const Recv& r, const float* __restrict__ grid, int W,
+                                                  int cells, float inv_w) {
+    if (id < 0 || id >= cells) return 0.0f;
     int nx, ny;
-    cell_xy(id, a.W, inv_w, nx, ny);
-    const float dx = (float)nx - r.gx, dy = (float)ny - r.gy, dz = a.grid[id] - r.z;
+    cell_xy(id, W, inv_w, nx, ny);
+    const float dx = (float)nx - r.gx, dy = (float)ny - r.gy, dz = grid[id] - r.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 < 1e-4f) return;
+    if (d2 < 1e-4f) return 0.0f;
     const float inv_d = rsqrtf(d2);
     const float cr = (r.nx * dx + r.ny * dy + r.nz * dz) * inv_d;
-    if (cr <= 0.0f) return;
+    if (cr <= 0.0f) return 0.0f;
     const float cs = fmaxf(fabsf(dz) * inv_d, 0.1f);
-    const float wgt = (0.2f / 3.14159265f) * cr * cs * (inv_d * inv_d);
-    const float* hs = a.hourly_top + (int64_t)id * K;
-    if constexpr (K % 4 == 0) {
-#pragma unroll
-        for (int h = 0; h < K; h += 4) {
-            const float4 q = *reinterpret_cast<const float4*>(hs + h);
-            R[h] += wgt * q.x;
-            R[h + 1] += wgt * q.y;
-            R[h + 2] += wgt * q.z;
-            R[h + 3] += wgt * q.w;
-        }
-    } else {
-#pragma unroll
-        for (int h = 0; h < K; ++h) R[h] += wgt * hs[h];
-    }
+    return (0.2f / 3.14159265f) * cr * cs * (inv_d * inv_d);
 }
 
-/* Butterfly transpose-reduce of R[K] over the 64 lanes.  Afterwards lane L holds the
- * full sum of hour hour_of_lane(L) in R[0]. */
+/* Number of set bits of m below this lane. */
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+/* Lane (g, h) = (lane / K, lane % K) adds w[i] * top[id[i]][h] over the list entries
+ * i = g, g + 64/K, ...: each load instruction reads 64/K whole K-hour rows. */
 template <int K>
-__device__ __forceinline__ void wave_transpose_reduce(float* R, int lane) {
-    int width = K;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        if (width > 1) {
-            const int half = width >> 1;
-            const bool upper = lane & m;
-#pragma unroll
-            for (int h = 0; h < K / 2; ++h) {
-                if (h < half) {
-                    const float keep = upper ? R[h + half] : R[h];
-                    const float give = upper ? R[h] : R[h + half];
-                    R[h] = keep + __shfl_xor(give, m, 64);
-                }
-            }
-            width = half;
-        } else {
-            R[0] += __shfl_xor(R[0], m, 64);
-        }
+__device__ __forceinline__ float apply_list(const int* ids, const float* ws, int cnt, const float* __restrict__ top,
+                                            int lane) {
+    constexpr int G = 64 / K;
+    const float* row = top + (lane & (K - 1));
+    float acc0 = 0.0f, acc1 = 0.0f;
+    int i = lane / K;
+#pragma unroll 4
+    for (; i + G < cnt; i += 2 * G) {
+        acc0 += ws[i] * row[(size_t)(unsigned)ids[i] * K];
+        acc1 += ws[i + G] * row[(size_t)(unsigned)ids[i + G] * K];
     }
+    if (i < cnt) acc0 += ws[i] * row[(size_t)(unsigned)ids[i] * K];
+    return acc0 + acc1;
 }
 
-/* Which hour lane L holds after wave_transpose_reduce<K>. */
-template <int K>
-__device__ __forceinline__ int hour_of_lane(int lane) {
-    int h = 0, width = K;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        if (width > 1) {
-            width >>= 1;
-            if (lane & m) h += width;
-        }
-    }
-    return h;
-}
-
-/* ba-RLE stream of one voxel, decoded by the whole wave; calls f(id) per non-zero value.
+/* ba-RLE stream of one voxel, decoded by the whole wave 64 elements at a time; every lane
+ * calls f(valid, id) once per step (valid: a non-zero value, id = value - 1).
  * Block layout: 3 bytes of 3-bit width codes (7 = unused), one length byte per run,
- * payload (4-bit deltas two per byte, high nibble first; wider ones big-endian). */
+ * payload (4-bit deltas two per byte, high nibble first; wider ones big-endian).
+ * p points into LDS (staged window) or global memory (streams longer than it). */
 template <typename F>
-__device__ __forceinline__ void wave_decode(const uint8_t* __restrict__ p, int size, int lane, F&& f) {
+__device__ __forceinline__ void wave_decode(const uint8_t* p, int size, int lane, F&& f) {
     int i = 0;
     unsigned prev = 0;
     bool first = true;
     while (i < size) {
-        const unsigned bits = ((unsigned)p[i] << 16) | ((unsigned)p[i + 1] << 8) | (unsigned)p[i + 2];
+        const unsigned bits = (unsigned)__builtin_amdgcn_readfirstlane(
+            (int)(((unsigned)p[i] << 16) | ((unsigned)p[i + 1] << 8) | (unsigned)p[i + 2]));
         i += 3;
         int nc = 0;
         unsigned wid[8];
@@ -186,32 +171,35 @@ __device__ __forceinline__ void wave_decode(const uint8_t* __restrict__ p, int s
             if (cd != 7u && nc == j) nc = j + 1;
         }
         /* run lengths, element prefix and payload offsets (wave-uniform) */
-        int len[8], estart[9], pstart[8];
+        int estart[9], pstart[8];
         estart[0] = 0;
         int pay = i + nc;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            len[j] = (j < nc) ? (int)p[i + j] : 0;
-            estart[j + 1] = estart[j] + len[j];
+            const int len = (j < nc) ? __builtin_amdgcn_readfirstlane((int)p[i + j]) : 0;
+            estart[j + 1] = estart[j] + len;
             pstart[j] = pay;
-            pay += (wid[j] == 0u) ? (len[j] + 1) >> 1 : len[j] * (int)wid[j];
+            pay += (wid[j] == 0u) ? (len + 1) >> 1 : len * (int)wid[j];
         }
         const int total = estart[8];
         for (int e0 = 0; e0 < total; e0 += 64) {
             const int e = e0 + lane;
             unsigned delta = 0;
             if (e < total) {
-                int r = 0;
+                /* the element's run: the last run starting at or before e (a select chain,
+                   so the per-run tables stay in registers) */
+                unsigned w = wid[0];
+                int es = 0, ps = pstart[0];
 #pragma unroll
-                for (int j = 1; j < 8; ++j) r += (e >= estart[j]) ? 1 : 0;
-                const int k = e - estart[r];
-                const unsigned w = wid[r];
+                for (int j = 1; j < 8; ++j)
+                    if (e >= estart[j]) { w = wid[j]; es = estart[j]; ps = pstart[j]; }
+                const int k = e - es;
                 if (w == 0u) {
-                    const unsigned byte = p[pstart[r] + (k >> 1)];
+                    const unsigned byte = p[ps + (k >> 1)];
                     delta = (k & 1) ? (byte & 15u) : (byte >> 4);
                 } else {
-                    const uint8_t* q = p + pstart[r] + k * (int)w;
-                    for (unsigned b = 0; b < w; ++b) delta = (delta << 8) | q[b];
+                    const int q = ps + k * (int)w;
+                    for (unsigned b = 0; b < w; ++b) delta = (delta << 8) | p[q + (int)b];
                 }
                 if (first && e == 0) delta = 0u - delta;   // first value is absolute
             }
@@ -223,7 +211,7 @@ __device__ __forceinline__ void wave_decode(const uint8_t* __restrict__ p, int s
                 if (lane >= o) s += y;
             }
             const unsigned val = prev - s;
-            if (e < total && val > 0u) f((int)val - 1);
+            f(e < total && val > 0u, (int)val - 1);
             prev = __shfl(val, 63, 64);   // lanes past the end add zero deltas
             first = false;
         }
@@ -231,12 +219,42 @@ __device__ __forceinline__ void wave_decode(const uint8_t* __restrict__ p, int s
     }
 }
 
-/* WAVES waves per workgroup (2 or 4): the LDS footprint / occupancy knob the autotuner picks. */
+/* ba-RLE bytes staged in LDS per workgroup, split evenly over its waves. */
+constexpr int kStageBytes = 8192;
+/* compacted (id, weight) entries per wave; the list is applied once it may not take another 64 */
+constexpr int kListCap = 128;
+
+/* Wave-wide copy of src[lo, hi) into LDS (lo and src 16-byte aligned). */
+__device__ __forceinline__ void stage_copy(uint8_t* dst, const uint8_t* __restrict__ src, int64_t lo, int64_t hi,
+                                           int lane) {
+    const int n = (int)(hi - lo);
+    const int full = n & ~15;
+    const uint8_t* s = src + lo;
+    for (int o = lane * 16; o < full; o += 64 * 16)
+        *reinterpret_cast<uint4*>(dst + o) = *reinterpret_cast<const uint4*>(s + o);
+    for (int o = full + lane; o < n; o += 64) dst[o] = s[o];
+}
+
+/* LDS writes of one lane visible to the other lanes of its wave (and no reordering across). */
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+/* WAVES waves per workgroup (2 or 4): the knob the autotuner picks.
+ * ba-RLE: the 64 voxels of a wave own one contiguous stretch of the stream; it is staged in
+ * LDS windows of kStageBytes / WAVES bytes with 16-byte loads, and each voxel is decoded from
+ * LDS (a voxel whose stream is longer than a window is decoded from global memory).
+ * DBG (K = 1 only): write the hour's own and reflected irradiance instead of accumulating. */
 template <int K, int FMT, bool DBG, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
-    __shared__ float rbuf[WAVES][64][K + 1];
+    constexpr int kCap = (FMT == SGPU_FMT_BARLE) ? kStageBytes / WAVES : 16;
     __shared__ Recv rec[WAVES][64];
     __shared__ int64_t beg[WAVES][64 + 1];
+    __shared__ int list_id[WAVES][kListCap];
+    __shared__ float list_w[WAVES][kListCap];
+    __shared__ __align__(16) uint8_t stage[WAVES][kCap];
 
     /* XCD-aware block order: block b runs on XCD b % 8; give each XCD a contiguous range. */
     const unsigned nb = gridDim.x;
@@ -253,12 +271,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
     const bool live = lane < nv;
     const int64_t v = live ? (a.subset ? (int64_t)a.subset[p] : a.v_base + p) : 0;
     const float inv_w = 1.0f / (float)a.W;
+    const int cells = a.W * a.H;
 
     /* phase 1: lane = voxel */
-    float own[K];
+    float own = 0.0f;
     {
         Recv r{};
-        float ss = 0, cs = 1, sa = 0, ca = 1;
         if (live) {
             const int top = a.pix_parent[v];
             int gx, gy;
@@ -266,6 +284,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
             r.gx = (float)gx;
             r.gy = (float)gy;
             r.z = (v < a.n_top) ? a.grid[top] : a.wall_z[v - a.n_top];
+            float ss, cs, sa, ca;
             sincosf(a.slopes[v], &ss, &cs);
             sincosf(a.aspects[v], &sa, &ca);
             r.nx = ss * sa;
@@ -273,57 +292,87 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
             r.nz = cs;
             beg[wave][lane] = a.nb_off[p];
             if (lane == nv - 1) beg[wave][nv] = a.nb_off[p + 1];
-        }
-        rec[wave][lane] = r;
-#pragma unroll
-        for (int h = 0; h < K; ++h) {
-            own[h] = 0.0f;
-            if (live && h < a.nh) {
-                const HourConst& hc = a.hours[h];
-                own[h] = hour_irradiance(hc, cs, ss, ca, sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
+            for (int h = 0; h < K; ++h) {
+                if (h < a.nh) {
+                    const HourConst& hc = a.hours[h];
+                    own += hour_irradiance(hc, cs, ss, ca, sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
+                }
             }
         }
+        rec[wave][lane] = r;
     }
     __syncthreads();
 
-    /* phase 2: lane = neighbour, one voxel of the wave at a time */
-    const int my_hour = hour_of_lane<K>(lane);
-    const bool writer = (lane & ((64 / K) - 1)) == 0;   // one of the 64/K replicas of each hour
+    /* phase 2: one receiver at a time */
+    float refl = 0.0f;   // lane j: receiver j's reflected irradiance summed over the pass
+    int64_t wlo = 0, whi = 0;   // staged stream window [wlo, whi)
+    const int64_t wend = (nv > 0) ? uniform64(beg[wave][nv]) : 0;
     for (int j = 0; j < nv; ++j) {
         const Recv r = rec[wave][j];
-        const int64_t s0 = beg[wave][j], s1 = beg[wave][j + 1];
-        float R[K];
-#pragma unroll
-        for (int h = 0; h < K; ++h) R[h] = 0.0f;
+        const int64_t s0 = uniform64(beg[wave][j]), s1 = uniform64(beg[wave][j + 1]);
+        float acc = 0.0f;
+        int cnt = 0;
+        auto take = [&](bool valid, int id) __attribute__((always_inline)) {
+            const float w = valid ? neighbour_weight(id, r, a.grid, a.W, cells, inv_w) : 0.0f;
+            const bool keep = w > 0.0f;
+            const uint64_t m = __ballot(keep);
+            if (keep) {
+                const int at = cnt + lanes_below(m);
+                list_id[wave][at] = id;
+                list_w[wave][at] = w;
+            }
+            cnt += __popcll(m);
+            if (cnt > kListCap - 64) {
+                wave_lds_sync();
+                acc += apply_list<K>(list_id[wave], list_w[wave], cnt, a.hourly_top, lane);
+                wave_lds_sync();
+                cnt = 0;
+            }
+        };
         if constexpr (FMT == SGPU_FMT_CSR) {
-            for (int64_t k = s0 + lane; k < s1; k += 64) add_neighbour<K>(a.nb_ids[k] - 1, r, a, inv_w, R);
+            for (int64_t b = s0; b < s1; b += 64) {
+                const int64_t k = b + lane;
+                const bool valid = k < s1;
+                take(valid, valid ? a.nb_ids[k] - 1 : -1);
+            }
         } else {
-            wave_decode(a.nb_bytes + s0, (int)(s1 - s0), lane,
-                        [&](int id) { add_neighbour<K>(id, r, a, inv_w, R); });
+            const int size = (int)(s1 - s0);
+            if (size <= kCap - 15) {   // fits a window that starts at s0 rounded down to 16
+                if (s0 < wlo || s1 > whi) {
+                    wave_lds_sync();
+                    wlo = s0 & ~(int64_t)15;
+                    whi = min(wlo + (int64_t)kCap, wend);
+                    stage_copy(stage[wave], a.nb_bytes, wlo, whi, lane);
+                    wave_lds_sync();
+                }
+                wave_decode(&stage[wave][s0 - wlo], size, lane, take);
+            } else {
+                wave_decode(a.nb_bytes + s0, size, lane, take);
+            }
         }
-        wave_transpose_reduce<K>(R, lane);
-        if (writer) rbuf[wave][j][my_hour] = R[0];
+        if (cnt > 0) {
+            wave_lds_sync();
+            acc += apply_list<K>(list_id[wave], list_w[wave], cnt, a.hourly_top, lane);
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
+        if (lane == j) refl = acc;
     }
-    __syncthreads();
 
     /* phase 3: lane = voxel */
     if (!live) return;
     if constexpr (DBG) {
-        a.dbg_hourly[p] = own[0];
-        a.dbg_reflected[p] = rbuf[wave][lane][0];
+        a.dbg_hourly[p] = own;
+        a.dbg_reflected[p] = refl;
         return;
     }
     float s = a.sum[p], c = a.comp[p];
-#pragma unroll
-    for (int h = 0; h < K; ++h) {
-        if (h < a.nh) {
-            const float x = rbuf[wave][lane][h] + own[h];
-            const float y = x - c;
-            const float t = s + y;
-            c = (t - s) - y;
-            s = t;
-        }
-    }
+    const float x = refl + own;
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
     a.sum[p] = s;
     a.comp[p] = c;
 }
@@ -334,32 +383,28 @@ void solar_top_k(const HourArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_solar_top<K>, dim3(g), dim3(kSolarBlock), 0, s, a);
 }
 
-template <int K, int WAVES>
+template <int K, bool DBG, int WAVES>
 void reflect_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
     const int64_t n = a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total);
     const int64_t blocks = (n + 64 * WAVES - 1) / (64 * WAVES);
     /* round up to a multiple of the XCD count so the remap covers every block */
     const unsigned g = (unsigned)(((blocks + kXcds - 1) / kXcds) * kXcds);
-    const bool dbg = a.dbg_hourly != nullptr;
     const dim3 blk(64 * WAVES);
-    if (fmt == SGPU_FMT_CSR) {
-        if (dbg) hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_CSR, true, WAVES>), dim3(g), blk, 0, s, a);
-        else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_CSR, false, WAVES>), dim3(g), blk, 0, s, a);
-    } else {
-        if (dbg) hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, true, WAVES>), dim3(g), blk, 0, s, a);
-        else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, false, WAVES>), dim3(g), blk, 0, s, a);
-    }
+    if (fmt == SGPU_FMT_CSR) hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_CSR, DBG, WAVES>), dim3(g), blk, 0, s, a);
+    else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, DBG, WAVES>), dim3(g), blk, 0, s, a);
 }
 
 template <int K>
 void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
-    if (a.waves == 2) reflect_kw<K, 2>(a, fmt, s);
-    else reflect_kw<K, 4>(a, fmt, s);
+    if (a.waves == 2) reflect_kw<K, false, 2>(a, fmt, s);
+    else reflect_kw<K, false, 4>(a, fmt, s);
 }
 
 }  // namespace
 
-bool supported_hours_per_pass(int K) { return K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32; }
+bool supported_hours_per_pass(int K) {
+    return K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64;
+}
 
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
     if (a.n_top <= 0) return;
@@ -370,6 +415,7 @@ void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
         case 8: solar_top_k<8>(a, s); break;
         case 16: solar_top_k<16>(a, s); break;
         case 32: solar_top_k<32>(a, s); break;
+        case 64: solar_top_k<64>(a, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());
@@ -378,6 +424,14 @@ void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
 void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s) {
     if ((a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total)) <= 0) return;
     if (fmt != SGPU_FMT_CSR && fmt != SGPU_FMT_BARLE) fail(SGPU_ERR_ARG, "format must be CSR or BARLE");
+    if (fmt == SGPU_FMT_BARLE && (reinterpret_cast<uintptr_t>(a.nb_bytes) & 15u) != 0)
+        fail(SGPU_ERR_ARG, "ba-RLE stream must be 16-byte aligned");
+    if (a.dbg_hourly != nullptr) {
+        if (K != 1 || a.dbg_reflected == nullptr) fail(SGPU_ERR_ARG, "single-hour outputs need one hour per pass");
+        reflect_kw<1, true, 4>(a, fmt, s);
+        SGPU_HIP(hipGetLastError());
+        return;
+    }
     switch (K) {
         case 1: reflect_k<1>(a, fmt, s); break;
         case 2: reflect_k<2>(a, fmt, s); break;
@@ -385,6 +439,7 @@ void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s)
         case 8: reflect_k<8>(a, fmt, s); break;
         case 16: reflect_k<16>(a, fmt, s); break;
         case 32: reflect_k<32>(a, fmt, s); break;
+        case 64: reflect_k<64>(a, fmt, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());

@@ -3,7 +3,7 @@
  *
  *   DDA tile       directions per 256-voxel workgroup, G in {16, 32, 48}:
  *                  timed on a 2^17-voxel slice from the middle of the scene;
- *   stage-2 pass   hours per pass K in {8, 16, 32} x waves per reflection
+ *   stage-2 pass   hours per pass K in {16, 32, 64} x waves per reflection
  *                  workgroup in {2, 4} (LDS footprint and occupancy): three
  *                  passes each on the live lists, scratch accumulators.
  * Every shape computes identical results; only time differs.  The winners are
@@ -133,10 +133,10 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
     t.rays_per_s = (double)n * kDirs / (best * 1e-3);
     /* stage-2 pass shape */
     if (lists && c->svf_ok && !c->hours.empty()) {
-        ensure_accumulators(c, 32);
+        ensure_accumulators(c, 64);
         ensure_hours_d(c, c->hours);
         double best_rate = 0.0;
-        for (int K : {8, 16, 32})
+        for (int K : {16, 32, 64})
             for (int waves : {2, 4}) {
                 const int passes = 3;
                 const double ms = time_passes(c, fmt, K, waves, passes);
@@ -173,7 +173,7 @@ int sgpu_set_tuning(sgpu_ctx* c, const sgpu_tuning* t) {
         need(c && t, SGPU_ERR_ARG, "null argument");
         need(t->dirs_per_group == 16 || t->dirs_per_group == 32 || t->dirs_per_group == 48, SGPU_ERR_ARG,
              "dirs_per_group must be 16, 32 or 48");
-        need(supported_hours_per_pass(t->hours_per_pass), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16 or 32");
+        need(supported_hours_per_pass(t->hours_per_pass), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32 or 64");
         need(t->waves_per_group == 2 || t->waves_per_group == 4, SGPU_ERR_ARG, "waves_per_group must be 2 or 4");
         c->tune = *t;
     });

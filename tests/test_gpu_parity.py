@@ -118,8 +118,8 @@ def test_single_hour_matches_oracle(fmt):
             assert (np.abs(rv - er) <= HOUR_RTOL * np.abs(er) + 1e-4).all(), f"reflected, hour {hi}"
 
 
-@pytest.mark.parametrize("fmt,K", [(sg.FMT_CSR, 1), (sg.FMT_CSR, 16), (sg.FMT_BARLE, 1), (sg.FMT_BARLE, 8),
-                                   (sg.FMT_BARLE, 32)])
+@pytest.mark.parametrize("fmt,K", [(sg.FMT_CSR, 1), (sg.FMT_CSR, 16), (sg.FMT_CSR, 64), (sg.FMT_BARLE, 1),
+                                   (sg.FMT_BARLE, 8), (sg.FMT_BARLE, 32), (sg.FMT_BARLE, 64)])
 def test_annual_within_1e4_of_fp64(fmt, K):
     sc = synth_scene(12, 64, 64)
     ref = oracle_lists(12, 64, 64)
@@ -187,3 +187,44 @@ def test_host_streamed_lists_match_device_resident(fmt):
         np.testing.assert_array_equal(data, edata)
         ctx.hour_loop(fmt, 8)
         np.testing.assert_array_equal(ctx.irradiance(), resident)
+
+
+@pytest.mark.parametrize("waves", [2, 4])
+def test_long_barle_streams_match_cpu_decoder(waves):
+    """Injected lists of distinct random cells: 3000 voxels with 0..2592 cells each, 10% of the
+    rest with 0..60, the others empty.  The ba-RLE streams run from nothing to several KB, so
+    the reflection kernel reloads its LDS stream window and also decodes the voxels whose stream
+    is longer than a window straight from global memory.  The single-hour reflected irradiance
+    matches the CPU ba-RLE decoder + reflection, and the annual sums of the ba-RLE and CSR walks
+    agree."""
+    sc = synth_scene(14, 1024, 1024)
+    rng = np.random.default_rng(7)
+    cells = sc.width * sc.height
+    n = sc.num_total
+    counts = np.where(rng.random(n) < 0.1, rng.integers(0, 60, n), 0)
+    longv = rng.choice(n, 3000, replace=False)
+    counts[longv] = rng.integers(0, 2593, 3000)   # at most one hit per ray direction
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    ids = np.concatenate([rng.choice(cells, int(c), replace=False) + 1 for c in counts if c]).astype(np.int32)
+    boff, data = orc.barle_encode_all(off, ids)
+    sizes = np.diff(boff)
+    window = 8192 // waves
+    assert (sizes > window).sum() > 100 and ((sizes > 0) & (sizes <= window - 15)).sum() > 100_000
+    az_alt, dni, dhi, bins = sky()
+    hours = orc.sunlit_hours(bins["nudged"])
+    with _ctx(sc, formats=sg.FMT_CSR) as ctx:
+        svf = ctx.svf()
+        ctx.set_csr(off, ids)
+        ctx.set_barle(boff, data)
+        ctx.set_tuning(32, 16, waves)
+        hi = int(hours[len(hours) // 2])
+        hv, rv = ctx.hour_single(sg.FMT_BARLE, hi)
+        eh = orc.solar_hour(sc, bins, dni, dhi, svf, hi)
+        er = orc.reflect_hour_barle(sc, boff, data, eh)
+        assert (np.abs(rv - er) <= HOUR_RTOL * np.abs(er) + 1e-4).all()
+        ctx.hour_loop(sg.FMT_BARLE, 0)
+        a_barle = ctx.irradiance()
+        ctx.hour_loop(sg.FMT_CSR, 0)
+        a_csr = ctx.irradiance()
+    assert _rel(a_barle, a_csr).max() <= 1e-5
