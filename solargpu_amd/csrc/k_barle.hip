@@ -1,7 +1,7 @@
 /* GPU ba-RLE encoder (format of the reference's barle_compression.h:7-44 and
  * encoder barle_compression.cpp:12-207), one wave per voxel:
- *   - the voxel's CSR ids are staged in LDS and rank-sorted descending by
- *     the whole wave (ties by index, so duplicates end up adjacent);
+ *   - the voxel's CSR ids are staged in LDS and bitonic-sorted descending by
+ *     the whole wave (padded with zeros to a power of two);
  *   - duplicates are dropped with a wave-ordered compaction;
  *   - delta width classes are computed in parallel;
  *   - lane 0 runs the short serial part: run detection, the single-element
@@ -14,7 +14,7 @@
 namespace sgpu {
 namespace {
 
-constexpr int kMaxList = kDirs;   // a voxel has at most one distinct hit per direction
+static_assert(kDirs <= 4096, "a voxel has at most one distinct hit per direction");
 
 __device__ __forceinline__ unsigned width_class(unsigned x) {   // barle_compression.cpp:12-18
     return x <= 15u ? 0u : x <= 255u ? 1u : x <= 65535u ? 2u : x <= 16777215u ? 3u : 4u;
@@ -45,54 +45,67 @@ __device__ __forceinline__ bool next_split(const unsigned* runs, int nruns, Spli
     return true;
 }
 
-template <bool WRITE>
+/* Descending bitonic sort of keys[0, P) (P a power of two <= MAXP) by one wave. */
+__device__ __forceinline__ void wave_bitonic_desc(unsigned* keys, int P, int lane) {
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < P; i += 64) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned x = keys[i], y = keys[ixj];
+                    const bool desc = (i & k) == 0;   // this k-block ends up descending
+                    if (desc ? (x < y) : (x > y)) {
+                        keys[i] = y;
+                        keys[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+/* MAXP: the list capacity of this instantiation (a power of two); lists with n <= SKIP are
+   left to the smaller instantiation, lists with n > MAXP never reach this one. */
+template <bool WRITE, int MAXP, int SKIP>
 __global__ __launch_bounds__(64) void k_barle(BarleArgs a) {
-    __shared__ unsigned vals[kMaxList];   // sorted-descending unique ids
-    __shared__ unsigned tmp[kMaxList];    // unsorted ids, then the run list
-    __shared__ unsigned char wcl[kMaxList];
+    __shared__ unsigned keys[MAXP];   // ids, then the run list
+    __shared__ unsigned vals[MAXP];   // sorted-descending unique ids
+    __shared__ unsigned char wcl[MAXP];
 
     const int64_t v = blockIdx.x;
     const int lane = threadIdx.x;
     const int64_t b = a.csr_off[v];
     const int n = (int)(a.csr_off[v + 1] - b);
+    if (n <= SKIP || n > MAXP) return;
     if (n <= 0) {
         if (!WRITE && lane == 0) a.sizes[v] = 0;
         return;
     }
-    for (int i = lane; i < n; i += 64) tmp[i] = (unsigned)a.csr_ids[b + i];
+    int P = 64;
+    while (P < n) P <<= 1;
+    for (int i = lane; i < P; i += 64) keys[i] = (i < n) ? (unsigned)a.csr_ids[b + i] : 0u;   // ids >= 1
     __syncthreads();
-    for (int i = lane; i < n; i += 64) {
-        const unsigned x = tmp[i];
-        int r = 0;
-        for (int j = 0; j < n; ++j) {
-            const unsigned y = tmp[j];
-            r += (y > x) || (y == x && j < i);
-        }
-        vals[r] = x;
-    }
-    __syncthreads();
-    /* Ordered compaction of the first of each equal group, chunks of 64. */
+    wave_bitonic_desc(keys, P, lane);
+    /* Ordered compaction of the first of each equal group, chunks of 64 (padding zeros last). */
     int m = 0;
     for (int base = 0; base < n; base += 64) {
         const int i = base + lane;
-        const bool keep = (i < n) && (i == 0 || vals[i] != vals[i - 1]);
-        const unsigned x = (i < n) ? vals[i] : 0u;
+        const bool keep = (i < n) && (i == 0 || keys[i] != keys[i - 1]);
         const unsigned long long mask = __ballot(keep);
         const int pos = m + __popcll(mask & ((1ull << lane) - 1ull));
-        __syncthreads();
-        if (keep) tmp[pos] = x;
+        if (keep) vals[pos] = keys[i];
         m += __popcll(mask);
-        __syncthreads();
     }
+    __syncthreads();
     for (int i = lane; i < m; i += 64) {
-        const unsigned x = tmp[i];
-        vals[i] = x;
-        wcl[i] = (unsigned char)width_class(i == 0 ? x : tmp[i - 1] - x);
+        const unsigned x = vals[i];
+        wcl[i] = (unsigned char)width_class(i == 0 ? x : vals[i - 1] - x);
     }
     __syncthreads();
     if (lane != 0) return;
 
-    unsigned* runs = tmp;   // ids no longer needed
+    unsigned* runs = keys;   // sorted ids no longer needed
     /* detect_runs (barle_compression.cpp:90-105) */
     int nr = 0;
     for (int i = 0; i < m;) {
@@ -186,8 +199,16 @@ __global__ __launch_bounds__(64) void k_barle(BarleArgs a) {
 
 void launch_barle(const BarleArgs& a, bool write_pass, hipStream_t s) {
     if (a.n <= 0) return;
-    if (write_pass) hipLaunchKernelGGL(k_barle<true>, dim3((unsigned)a.n), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(k_barle<false>, dim3((unsigned)a.n), dim3(64), 0, s, a);
+    /* most lists are short: a small-LDS instantiation for them (high occupancy), then the
+       full-size one for the rest */
+    const dim3 g((unsigned)a.n), blk(64);
+    if (write_pass) {
+        hipLaunchKernelGGL((k_barle<true, 512, -1>), g, blk, 0, s, a);
+        hipLaunchKernelGGL((k_barle<true, 4096, 512>), g, blk, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_barle<false, 512, -1>), g, blk, 0, s, a);
+        hipLaunchKernelGGL((k_barle<false, 4096, 512>), g, blk, 0, s, a);
+    }
     SGPU_HIP(hipGetLastError());
 }
 

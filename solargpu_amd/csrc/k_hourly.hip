@@ -148,77 +148,6 @@ __device__ __forceinline__ float apply_list(const int* ids, const float* ws, int
     return acc0 + acc1;
 }
 
-/* ba-RLE stream of one voxel, decoded by the whole wave 64 elements at a time; every lane
- * calls f(valid, id) once per step (valid: a non-zero value, id = value - 1).
- * Block layout: 3 bytes of 3-bit width codes (7 = unused), one length byte per run,
- * payload (4-bit deltas two per byte, high nibble first; wider ones big-endian).
- * p points into LDS (staged window) or global memory (streams longer than it). */
-template <typename F>
-__device__ __forceinline__ void wave_decode(const uint8_t* p, int size, int lane, F&& f) {
-    int i = 0;
-    unsigned prev = 0;
-    bool first = true;
-    while (i < size) {
-        const unsigned bits = (unsigned)__builtin_amdgcn_readfirstlane(
-            (int)(((unsigned)p[i] << 16) | ((unsigned)p[i + 1] << 8) | (unsigned)p[i + 2]));
-        i += 3;
-        int nc = 0;
-        unsigned wid[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const unsigned cd = (bits >> (21 - 3 * j)) & 7u;
-            wid[j] = cd;
-            if (cd != 7u && nc == j) nc = j + 1;
-        }
-        /* run lengths, element prefix and payload offsets (wave-uniform) */
-        int estart[9], pstart[8];
-        estart[0] = 0;
-        int pay = i + nc;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int len = (j < nc) ? __builtin_amdgcn_readfirstlane((int)p[i + j]) : 0;
-            estart[j + 1] = estart[j] + len;
-            pstart[j] = pay;
-            pay += (wid[j] == 0u) ? (len + 1) >> 1 : len * (int)wid[j];
-        }
-        const int total = estart[8];
-        for (int e0 = 0; e0 < total; e0 += 64) {
-            const int e = e0 + lane;
-            unsigned delta = 0;
-            if (e < total) {
-                /* the element's run: the last run starting at or before e (a select chain,
-                   so the per-run tables stay in registers) */
-                unsigned w = wid[0];
-                int es = 0, ps = pstart[0];
-#pragma unroll
-                for (int j = 1; j < 8; ++j)
-                    if (e >= estart[j]) { w = wid[j]; es = estart[j]; ps = pstart[j]; }
-                const int k = e - es;
-                if (w == 0u) {
-                    const unsigned byte = p[ps + (k >> 1)];
-                    delta = (k & 1) ? (byte & 15u) : (byte >> 4);
-                } else {
-                    const int q = ps + k * (int)w;
-                    for (unsigned b = 0; b < w; ++b) delta = (delta << 8) | p[q + (int)b];
-                }
-                if (first && e == 0) delta = 0u - delta;   // first value is absolute
-            }
-            /* inclusive prefix sum of deltas (mod 2^32) across the wave */
-            unsigned s = delta;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned y = __shfl_up(s, o, 64);
-                if (lane >= o) s += y;
-            }
-            const unsigned val = prev - s;
-            f(e < total && val > 0u, (int)val - 1);
-            prev = __shfl(val, 63, 64);   // lanes past the end add zero deltas
-            first = false;
-        }
-        i = pay;
-    }
-}
-
 /* ba-RLE bytes staged in LDS per workgroup, split evenly over its waves. */
 constexpr int kStageBytes = 8192;
 /* compacted (id, weight) entries per wave; the list is applied once it may not take another 64 */
@@ -242,10 +171,103 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+/* One wave's window on its stretch of the ba-RLE stream: bytes [lo, hi) held in LDS. */
+struct StreamWindow {
+    uint8_t* lds;
+    const uint8_t* g;   // the whole stream (16-byte aligned)
+    int64_t lo, hi;
+    int64_t end;        // end of the wave's stretch
+    int cap;
+
+    __device__ __forceinline__ bool fits(int64_t a, int64_t b) const { return b - a <= cap - 15; }
+    /* make [a, b) resident (b - a <= cap - 15, b <= end) */
+    __device__ __forceinline__ void cover(int64_t a, int64_t b, int lane) {
+        if (a < lo || b > hi) {
+            wave_lds_sync();
+            lo = a & ~(int64_t)15;
+            hi = min(lo + (int64_t)cap, end);
+            stage_copy(lds, g, lo, hi, lane);
+            wave_lds_sync();
+        }
+    }
+};
+
+/* The ba-RLE stream [s0, s1) of one voxel, decoded by the whole wave block by block; each
+ * block is made resident in the LDS window first (a block larger than the window is read
+ * from global memory).  Block layout: 3 bytes of 3-bit width codes (7 = unused), one length
+ * byte per run, payload (4-bit deltas two per byte, high nibble first; wider ones big-endian). */
+template <typename F>
+__device__ __forceinline__ void wave_decode(StreamWindow& win, int64_t s0, int64_t s1, int lane, F&& f) {
+    unsigned prev = 0;
+    bool first = true;
+    for (int64_t pos = s0; pos < s1;) {
+        win.cover(pos, min(pos + 11, s1), lane);   // codes + at most 8 lengths
+        const uint8_t* h = win.lds + (pos - win.lo);
+        const unsigned bits = (unsigned)__builtin_amdgcn_readfirstlane(
+            (int)(((unsigned)h[0] << 16) | ((unsigned)h[1] << 8) | (unsigned)h[2]));
+        int nc = 0;
+        unsigned wid[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const unsigned cd = (bits >> (21 - 3 * j)) & 7u;
+            wid[j] = cd;
+            if (cd != 7u && nc == j) nc = j + 1;
+        }
+        /* run lengths, element prefix and payload offsets within the block (wave-uniform) */
+        int estart[9], pstart[8];
+        estart[0] = 0;
+        int pay = 3 + nc;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int len = (j < nc) ? __builtin_amdgcn_readfirstlane((int)h[3 + j]) : 0;
+            estart[j + 1] = estart[j] + len;
+            pstart[j] = pay;
+            pay += (wid[j] == 0u) ? (len + 1) >> 1 : len * (int)wid[j];
+        }
+        const bool resident = win.fits(pos, pos + pay);
+        if (resident) win.cover(pos, pos + pay, lane);
+        const uint8_t* p = resident ? win.lds + (pos - win.lo) : win.g + pos;
+        const int total = estart[8];
+        for (int e0 = 0; e0 < total; e0 += 64) {   // 64 elements per step
+            const int e = e0 + lane;
+            unsigned delta = 0;
+            if (e < total) {
+                /* the element's run: the last run starting at or before e (a select chain,
+                   so the per-run tables stay in registers) */
+                unsigned w = wid[0];
+                int es = 0, ps = pstart[0];
+#pragma unroll
+                for (int j = 1; j < 8; ++j)
+                    if (e >= estart[j]) { w = wid[j]; es = estart[j]; ps = pstart[j]; }
+                const int k = e - es;
+                if (w == 0u) {
+                    const unsigned byte = p[ps + (k >> 1)];
+                    delta = (k & 1) ? (byte & 15u) : (byte >> 4);
+                } else {
+                    const int q = ps + k * (int)w;
+                    for (unsigned b = 0; b < w; ++b) delta = (delta << 8) | p[q + (int)b];
+                }
+                if (first && e == 0) delta = 0u - delta;   // first value is absolute
+            }
+            /* inclusive prefix sum of deltas (mod 2^32) across the wave */
+            unsigned sc = delta;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned y = __shfl_up(sc, o, 64);
+                if (lane >= o) sc += y;
+            }
+            const unsigned val = prev - sc;
+            f(e < total && val > 0u, (int)val - 1);   // valid: a non-zero value, id = value - 1
+            prev = __shfl(val, 63, 64);   // lanes past the end add zero deltas
+            first = false;
+        }
+        pos += pay;
+    }
+}
+
 /* WAVES waves per workgroup (2 or 4): the knob the autotuner picks.
- * ba-RLE: the 64 voxels of a wave own one contiguous stretch of the stream; it is staged in
- * LDS windows of kStageBytes / WAVES bytes with 16-byte loads, and each voxel is decoded from
- * LDS (a voxel whose stream is longer than a window is decoded from global memory).
+ * ba-RLE: the 64 voxels of a wave own one contiguous stretch of the stream; it passes through
+ * an LDS window of kStageBytes / WAVES bytes filled with 16-byte loads, block by block.
  * DBG (K = 1 only): write the hour's own and reflected irradiance instead of accumulating. */
 template <int K, int FMT, bool DBG, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
@@ -305,8 +327,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
 
     /* phase 2: one receiver at a time */
     float refl = 0.0f;   // lane j: receiver j's reflected irradiance summed over the pass
-    int64_t wlo = 0, whi = 0;   // staged stream window [wlo, whi)
-    const int64_t wend = (nv > 0) ? uniform64(beg[wave][nv]) : 0;
+    StreamWindow win{stage[wave], a.nb_bytes, 0, 0, (nv > 0) ? uniform64(beg[wave][nv]) : 0, kCap};
     for (int j = 0; j < nv; ++j) {
         const Recv r = rec[wave][j];
         const int64_t s0 = uniform64(beg[wave][j]), s1 = uniform64(beg[wave][j + 1]);
@@ -336,19 +357,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
                 take(valid, valid ? a.nb_ids[k] - 1 : -1);
             }
         } else {
-            const int size = (int)(s1 - s0);
-            if (size <= kCap - 15) {   // fits a window that starts at s0 rounded down to 16
-                if (s0 < wlo || s1 > whi) {
-                    wave_lds_sync();
-                    wlo = s0 & ~(int64_t)15;
-                    whi = min(wlo + (int64_t)kCap, wend);
-                    stage_copy(stage[wave], a.nb_bytes, wlo, whi, lane);
-                    wave_lds_sync();
-                }
-                wave_decode(&stage[wave][s0 - wlo], size, lane, take);
-            } else {
-                wave_decode(a.nb_bytes + s0, size, lane, take);
-            }
+            wave_decode(win, s0, s1, lane, take);
         }
         if (cnt > 0) {
             wave_lds_sync();

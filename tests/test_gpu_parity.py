@@ -192,9 +192,10 @@ def test_host_streamed_lists_match_device_resident(fmt):
 @pytest.mark.parametrize("waves", [2, 4])
 def test_long_barle_streams_match_cpu_decoder(waves):
     """Injected lists of distinct random cells: 3000 voxels with 0..2592 cells each, 10% of the
-    rest with 0..60, the others empty.  The ba-RLE streams run from nothing to several KB, so
-    the reflection kernel reloads its LDS stream window and also decodes the voxels whose stream
-    is longer than a window straight from global memory.  The single-hour reflected irradiance
+    rest with 0..60, the others empty; and 200 voxels with 2000 cells spaced 300 apart (runs of
+    2-byte deltas: ~4 KB ba-RLE blocks).  The streams run from nothing to several KB, so the
+    reflection kernel slides its LDS stream window many times per wave and decodes the blocks
+    larger than a window straight from global memory.  The single-hour reflected irradiance
     matches the CPU ba-RLE decoder + reflection, and the annual sums of the ba-RLE and CSR walks
     agree."""
     sc = synth_scene(14, 1024, 1024)
@@ -204,13 +205,20 @@ def test_long_barle_streams_match_cpu_decoder(waves):
     counts = np.where(rng.random(n) < 0.1, rng.integers(0, 60, n), 0)
     longv = rng.choice(n, 3000, replace=False)
     counts[longv] = rng.integers(0, 2593, 3000)   # at most one hit per ray direction
+    lists = {int(v): rng.choice(cells, int(counts[v]), replace=False) + 1 for v in np.flatnonzero(counts)}
+    for v in longv[:200]:
+        lists[int(v)] = 1 + int(rng.integers(0, 1000)) + 300 * np.arange(2000)
+    counts = np.zeros(n, np.int64)
+    for v, x in lists.items():
+        counts[v] = x.size
     off = np.zeros(n + 1, np.int64)
     off[1:] = np.cumsum(counts)
-    ids = np.concatenate([rng.choice(cells, int(c), replace=False) + 1 for c in counts if c]).astype(np.int32)
+    ids = np.concatenate([lists[v] for v in sorted(lists)]).astype(np.int32)
     boff, data = orc.barle_encode_all(off, ids)
     sizes = np.diff(boff)
     window = 8192 // waves
     assert (sizes > window).sum() > 100 and ((sizes > 0) & (sizes <= window - 15)).sum() > 100_000
+    assert (sizes[longv[:200]] >= 2000 * 2).all()   # blocks of 8 x 255 two-byte deltas
     az_alt, dni, dhi, bins = sky()
     hours = orc.sunlit_hours(bins["nudged"])
     with _ctx(sc, formats=sg.FMT_CSR) as ctx:
