@@ -23,8 +23,8 @@
  *                           reflected irradiance summed over the pass;
  *                  phase 3, lane = voxel: the pass total (reflected + own)
  *                           enters the FP32 Kahan-compensated annual sum.
- * Workgroups are remapped so each XCD walks one contiguous eighth of the
- * voxels, keeping its neighbour gathers in its own L2.
+ * Workgroups are remapped XCD-aware: each XCD walks one contiguous eighth of
+ * the voxels, or the eight XCDs walk adjacent bands side by side (xcd_block).
  *
  * Formulas: solar_kernel.cu:54-77 (+ Perez :8-50; hour-only terms hoisted into
  * HourConst), reflection_kernels.cu:18-65 (geometry), :70-89 (CSR walk),
@@ -117,6 +117,21 @@ __device__ __forceinline__ float neighbour_weight(int id, const Recv& r, const f
     if (cr <= 0.0f) return 0.0f;
     const float cs = fmaxf(fabsf(dz) * inv_d, 0.1f);
     return (0.2f / 3.14159265f) * cr * cs * (inv_d * inv_d);
+}
+
+/* XCD-aware block order: block b runs on XCD b % 8 (nb, the grid, is a multiple of 8).
+ * band == 0: each XCD walks one contiguous eighth of the voxels (its L2 holds its own
+ * region); band > 0: the XCDs walk consecutive bands of `band` blocks side by side (XCD x
+ * takes bands x, x + 8, x + 16, ...), so their neighbour gathers overlap in the shared
+ * last-level cache.  A bijection onto [0, nb). */
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned band) {
+    const unsigned per = nb / kXcds;
+    const unsigned x = b % kXcds, s = b / kXcds;
+    if (band == 0 || band >= per) return x * per + s;
+    const unsigned full = per / band;
+    if (s < full * band) return ((s / band) * kXcds + x) * band + s % band;
+    const unsigned r = per - full * band;
+    return full * band * kXcds + x * r + (s - full * band);
 }
 
 /* Number of set bits of m below this lane. */
@@ -278,11 +293,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
     __shared__ float list_w[WAVES][kListCap];
     __shared__ __align__(16) uint8_t stage[WAVES][kCap];
 
-    /* XCD-aware block order: block b runs on XCD b % 8; give each XCD a contiguous range. */
-    const unsigned nb = gridDim.x;
-    const unsigned per = (nb + kXcds - 1) / kXcds;
-    const unsigned xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
-    const unsigned lb = xcd * per + slot;   // the grid is a multiple of 8, so lb < gridDim.x
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x, (unsigned)a.band);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     /* positions p index the neighbour offsets and the accumulators; p is the voxel id
        itself unless a subset of voxel ids is given */

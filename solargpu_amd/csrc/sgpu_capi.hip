@@ -108,6 +108,7 @@ HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
     a.sum = c->sum;
     a.comp = c->comp;
     a.waves = c->tune.waves_per_group;
+    a.band = c->tune.xcd_band;
     if (fmt == SGPU_FMT_CSR) {
         a.nb_off = c->csr_off;
         a.nb_ids = c->csr_ids;

@@ -156,7 +156,7 @@ struct sgpu_ctx {
     bool count_steps = false;
     sgpu_info info{};
     sgpu::SurrogateState sur;
-    sgpu_tuning tune{32, 16, 4, 0, 0.0, 0.0};   // trace group, hours per pass, waves; from_cache
+    sgpu_tuning tune{16, 64, 4, 0, 0, 0.0, 0.0};   // trace group, hours per pass, waves, XCD band; from_cache
     int residency = SGPU_LISTS_DEVICE;          // where sgpu_viewshed leaves the lists
     sgpu::HostLists host;                       // the lists when host-resident
     hipStream_t copy_stream = nullptr;          // H2D tile copies of the streamed hour loop

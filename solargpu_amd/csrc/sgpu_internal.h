@@ -177,6 +177,7 @@ struct HourArgs {
     int64_t n_sub;
     int64_t v_base;
     int waves;                 // waves per k_reflect workgroup: 2 or 4 (0 = 4)
+    int band;                  // k_reflect blocks per XCD band (0 = one contiguous eighth per XCD)
 };
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s);
 void launch_reflect_accum(const HourArgs& a, int K, unsigned format, hipStream_t s);

@@ -225,7 +225,7 @@ def test_long_barle_streams_match_cpu_decoder(waves):
         svf = ctx.svf()
         ctx.set_csr(off, ids)
         ctx.set_barle(boff, data)
-        ctx.set_tuning(32, 16, waves)
+        ctx.set_tuning(32, 16, waves, 64 if waves == 4 else 0)   # XCDs side by side / contiguous
         hi = int(hours[len(hours) // 2])
         hv, rv = ctx.hour_single(sg.FMT_BARLE, hi)
         eh = orc.solar_hour(sc, bins, dni, dhi, svf, hi)

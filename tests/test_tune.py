@@ -24,8 +24,8 @@ def test_every_shape_is_results_neutral():
     az_alt, dni, dhi, bins = sky()
     exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
     with _ctx(sc) as ctx:
-        for G, K, waves in ((16, 8, 2), (48, 32, 4), (32, 16, 2), (32, 64, 4)):
-            ctx.set_tuning(G, K, waves)
+        for G, K, waves, band in ((16, 8, 2, 0), (48, 32, 4, 16), (32, 16, 2, 0), (32, 64, 4, 0)):
+            ctx.set_tuning(G, K, waves, band)
             ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
             off, ids = ctx.csr()
             np.testing.assert_array_equal(ids, ref["ids"])
