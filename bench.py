@@ -10,7 +10,10 @@ inter-reflection decoded in-kernel, FP32 Kahan annual sums).
 
 value = voxel-hours/s = N_voxels x sunlit hours / stage-2 seconds (HIP
 events).  Extra keys: DDA rays/s (N x 2592 / stage-1 seconds), the CSR
-variant, the reflection kernel's HBM roofline and the CPU baseline.
+variant, the reflection kernel's HBM roofline, the CPU baseline and the
+surrogate form on the same scene (trained weights under
+solargpu_amd/weights/): its GPU-seconds against the plain form's and its
+per-voxel annual error against the plain result.
 """
 from __future__ import annotations
 
@@ -27,6 +30,8 @@ sys.path.insert(0, ROOT)
 PROFILES = os.path.join(ROOT, "profiles")
 
 HBM_PEAK_GBPS = 8000.0
+WEIGHTS = os.path.join(ROOT, "solargpu_amd", "weights", "surrogate.npz")
+METRIC = "voxel-hours/s + DDA rays/s, plain vs surrogate at <=2% p95 annual error"
 SITE = (15.65, 46.56, 1)
 SEED = 20260102
 
@@ -96,6 +101,33 @@ def cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, threads, budget_s):
             "rays_per_s": rays_rate}
 
 
+def surrogate_leg(sg, sc, az_alt, dni, dhi, cell, tuning, exact, plain_s, hours):
+    """The surrogate form on the bench scene in a fresh context, with both exact fallbacks."""
+    from solargpu_amd import surrogate as sm
+    params, gate, _ = sm.load(WEIGHTS)
+    mlp = sm.MLP(seed=1)
+    mlp.set_params(params)
+    out = {"weights": os.path.relpath(WEIGHTS, ROOT), "plain_gpu_s": plain_s}
+    with sg.Context(0) as c2:
+        c2.set_scene(sc)
+        c2.set_sky(az_alt, dni, dhi)
+        c2.set_tuning(tuning["dirs_per_group"], tuning["hours_per_pass"], tuning["waves_per_group"],
+                      tuning["xcd_band"])
+        sm.prepare(c2, SITE[1], cell)
+        for name, k in (("annual_fallback", sm.FALLBACK_ANNUAL), ("hourly_fallback", 0)):
+            tot, fb, st = sm.run(c2, mlp, gate, k)
+            rel = np.abs(tot.astype(np.float64) - exact) / np.maximum(np.abs(exact.astype(np.float64)), 1.0)
+            gs = st["t_total_ms"] / 1e3
+            out[name] = {"gpu_s": gs, "gpu_seconds_ratio": plain_s / gs,
+                         "voxel_hours_per_s": sc.num_total * hours / gs, "fallback_share": float(fb.mean()),
+                         "p50": float(np.percentile(rel, 50)), "p95": float(np.percentile(rel, 95)),
+                         "max": float(rel.max())}
+            log(f"surrogate ({name}): {gs * 1e3:.0f} ms, p95 {out[name]['p95']:.4f}, "
+                f"fallback {out[name]['fallback_share']:.3f}")
+    mlp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +142,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-csr", action="store_true", help="skip the CSR hour-loop report")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
 
@@ -186,6 +219,10 @@ def main():
     traffic = read_traffic()
 
     extra = {}
+    exact = ctx.irradiance()
+    if not args.no_surrogate and os.path.exists(WEIGHTS) and args.hour_stride == 1:
+        extra["surrogate"] = surrogate_leg(sg, sc, az_alt, dni, dhi, cell, ctx.tuning(), exact,
+                                           (t_vs + t_hr) / args.steps, H)
     copy_gbps = ctx.copy_bandwidth(1 << 30, 20)
     if not args.no_csr:
         ctx.hour_loop(sg.FMT_CSR, 16, args.hour_stride)
@@ -195,7 +232,7 @@ def main():
     if tuning is not None:
         extra["autotune"] = tuning
     res = {
-        "metric": "voxel-hours/s (plain ba-RLE pipeline, stage 2); DDA rays/s (stage 1)",
+        "metric": METRIC,
         "value": vh,
         "unit": "voxel-hours/s",
         "n_gpus": 1,

@@ -194,7 +194,7 @@ int sgpu_get_tuning(sgpu_ctx* ctx, sgpu_tuning* out);
 int sgpu_copy_bandwidth(sgpu_ctx* ctx, int64_t bytes, int reps, double* gbps);
 
 /* ---- per-voxel surrogate MLP (new; no reference counterpart) ----
-   64 inputs (48 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and
+   64 inputs (49 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and
    activations, FP32 accumulation on MFMA; Adam.  Parameters as one fp32 vector: per layer
    W[out][in] then b[out]. */
 typedef struct sgpu_mlp sgpu_mlp;
@@ -231,13 +231,20 @@ typedef struct sgpu_surrogate_stats {
 } sgpu_surrogate_stats;
 
 /* Sun-bin SVF (identical bits to sgpu_viewshed's), exact annual direct + diffuse and the
-   feature matrix, on the device.  lat is a feature (site latitude, degrees). */
-int sgpu_surrogate_prepare(sgpu_ctx* ctx, float lat, sgpu_surrogate_stats* stats);
+   feature matrix, on the device.  lat (site latitude, degrees) and cell_m (DSM cell size,
+   metres) are features; sgpu_surrogate_run reuses them. */
+int sgpu_surrogate_prepare(sgpu_ctx* ctx, float lat, float cell_m, sgpu_surrogate_stats* stats);
 /* Download features (N x 64 bf16), annual direct+diffuse and diffuse (any may be NULL). */
 int sgpu_surrogate_get(sgpu_ctx* ctx, uint16_t* X, float* dd, float* diff);
 /* Annual reflected irradiance of the plain method (needs the neighbour lists). */
 int sgpu_surrogate_labels(sgpu_ctx* ctx, unsigned format, float* reflected);
-/* The whole surrogate path; out: annual totals (N), fallback: per-voxel flag (N, optional). */
+/* The whole surrogate path; out: annual totals (N), fallback: per-voxel flag (N, optional).
+   The gated voxels get the exact plain result from their own 2592-direction lists (ba-RLE):
+   hours_per_pass > 0 runs the sunlit-hour loop on them, K hours per pass;
+   hours_per_pass == SGPU_FALLBACK_ANNUAL walks each list once with the neighbours' annual
+   direct + diffuse totals in place of one hour's values (the reflection is linear in them
+   and its weights do not depend on the hour, so the annual sums are the same). */
+#define SGPU_FALLBACK_ANNUAL (-1)
 int sgpu_surrogate_run(sgpu_ctx* ctx, sgpu_mlp* mlp, const sgpu_gate* gate, int hours_per_pass, float* out,
                        int32_t* fallback, sgpu_surrogate_stats* stats);
 

@@ -4,7 +4,7 @@
  *   k_annual_dd  exact annual direct + diffuse per voxel (FP32 Kahan over all
  *                sunlit hours) and the diffuse part alone -- the solar kernel
  *                of solar_kernel.cu:54-77 without the reflection term;
- *   k_features   32-sector horizon march + the 48 features of the MLP, bf16;
+ *   k_features   32-sector horizon march + the 49 features of the MLP, bf16;
  *   k_gate       surrogate estimate, predicted error, fallback flags;
  *   k_combine    scatter of the exact fallback totals. */
 #include <cmath>
@@ -160,6 +160,7 @@ __global__ __launch_bounds__(256) void k_features(SurArgs a) {
     row[43] = f2bf(dd > 0.0f ? a.diff[v] / dd : 0.0f);
 #pragma unroll
     for (int q = 0; q < 4; ++q) row[44 + q] = f2bf(quad[q] * (1e-6f / 8.0f));
+    row[48] = f2bf(a.cell_m);   // horizontal distances of the ray geometry are in cells
 #pragma unroll
     for (int f = kSurFeatures; f < kSurInPad; ++f) row[f] = 0;
 }
@@ -188,6 +189,16 @@ __global__ __launch_bounds__(256) void k_scatter(const int32_t* ids, const float
     if (i < n) out[ids[i]] = vals[i];
 }
 
+/* out[ids[i]] = base[ids[i]] + vals[i] (exact direct + diffuse plus the fallback's reflected sum) */
+__global__ __launch_bounds__(256) void k_scatter_sum(const int32_t* ids, const float* vals, const float* base,
+                                                     int64_t n, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const int32_t v = ids[i];
+        out[v] = vals[i] + base[v];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids) {
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (v < n && flags[v]) ids[off[v]] = (int32_t)v;
@@ -214,6 +225,12 @@ void launch_gate(const SurArgs& a, hipStream_t s) {
 void launch_scatter(const int32_t* ids, const float* vals, int64_t n, float* out, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ids, vals, n, out);
+    SGPU_HIP(hipGetLastError());
+}
+void launch_scatter_sum(const int32_t* ids, const float* vals, const float* base, int64_t n, float* out,
+                        hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ids, vals, base, n, out);
     SGPU_HIP(hipGetLastError());
 }
 void launch_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids, hipStream_t s) {

@@ -643,8 +643,38 @@ void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const Subse
     dfree(comp);
 }
 
+void subset_reflected_linear(sgpu_ctx* c, const int32_t* subset, int64_t n, const SubsetLists& L, unsigned fmt,
+                             const float* dd, float* out) {
+    if (n <= 0) return;
+    ensure_hours_d(c, c->hours);
+    HourArgs a = hour_args(c, fmt);
+    a.subset = subset;
+    a.n_sub = n;
+    if (fmt == SGPU_FMT_CSR) {
+        a.nb_off = L.csr_off;
+        a.nb_ids = L.csr_ids;
+    } else {
+        a.nb_off = L.br_off;
+        a.nb_bytes = L.br_bytes;
+    }
+    a.hourly_top = const_cast<float*>(dd);   // K = 1: the top voxels' annual totals
+    a.hours = c->hours_d;
+    a.nh = 1;
+    float* own = dalloc<float>(n);
+    try {
+        a.dbg_hourly = own;
+        a.dbg_reflected = out;
+        launch_reflect_accum(a, 1, fmt, c->stream);
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        dfree(own);
+        throw;
+    }
+    dfree(own);
+}
+
 /* Annual reflected irradiance via single-bounce linearity: one reflection walk with the
-   annual top-cell totals in place of one hour's values (labels and checks only). */
+   annual top-cell totals in place of one hour's values. */
 void annual_reflected_linear(sgpu_ctx* c, unsigned fmt, const float* dd, float* out_dev) {
     check_stage2_ready(c, fmt);
     need(!lists_on_host(c, fmt), SGPU_ERR_STATE, "needs device-resident lists");

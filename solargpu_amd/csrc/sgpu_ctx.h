@@ -133,6 +133,7 @@ struct sgpu_ctx {
     std::vector<sgpu::HourConst> hours;
     sgpu::DirConst* dirs = nullptr;
     float lat = 0.0f;
+    float cell_m = 1.0f;     // DSM cell size of the scene (surrogate feature)
 
     uint8_t* svf = nullptr;
     bool svf_ok = false;
@@ -230,5 +231,8 @@ void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const Subse
 /* Annual reflected irradiance per voxel from the annual direct + diffuse of the top cells
    (device arrays), by single-bounce linearity. */
 void annual_reflected_linear(sgpu_ctx* c, unsigned fmt, const float* dd, float* out_dev);
+/* Same walk over a subset's own lists (positions of `subset`); out: n reflected sums. */
+void subset_reflected_linear(sgpu_ctx* c, const int32_t* subset, int64_t n, const SubsetLists& L, unsigned fmt,
+                             const float* dd, float* out);
 
 }  // namespace sgpu

@@ -8,7 +8,7 @@
 namespace sgpu {
 
 constexpr int kSectors = 32;
-constexpr int kSurFeatures = 48;
+constexpr int kSurFeatures = 49;
 constexpr int kSurInPad = 64;
 
 struct SurArgs {
@@ -31,7 +31,7 @@ struct SurArgs {
     float* dd;                 // annual direct + diffuse
     float* diff;               // annual diffuse
     uint16_t* X;               // features [n_total][64] bf16
-    float lat, albedo;
+    float lat, albedo, cell_m;
     const float* Y;            // MLP output [n_total][2]
     float* out;                // surrogate annual total
     int32_t* fallback;         // 1 = take the exact plain result
@@ -44,6 +44,8 @@ void launch_features(const SurArgs& a, hipStream_t s);
 void launch_gate(const SurArgs& a, hipStream_t s);
 void launch_scatter(const int32_t* ids, const float* vals, int64_t n, float* out, hipStream_t s);
 /* ids[off[v]] = v for every flagged v (off = exclusive scan of the flags). */
+void launch_scatter_sum(const int32_t* ids, const float* vals, const float* base, int64_t n, float* out,
+                        hipStream_t s);
 void launch_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids, hipStream_t s);
 
 }  // namespace sgpu

@@ -2,7 +2,8 @@
  *   prepare   sun-bin SVF + exact annual direct/diffuse + MLP features, on the GPU;
  *   labels    annual reflected irradiance of the plain method (training targets);
  *   run       features -> MLP (BF16 MFMA) -> gate -> exact plain fallback (full
- *             2592-direction trace, ba-RLE lists and the per-hour reflection loop)
+ *             2592-direction trace and ba-RLE lists, then either the per-hour
+ *             reflection loop or one walk over the neighbours' annual totals)
  *             for the voxels whose predicted error exceeds the threshold. */
 #include <algorithm>
 #include <vector>
@@ -39,15 +40,18 @@ SurArgs sur_args(sgpu_ctx* c) {
     a.diff = c->sur.diff;
     a.X = c->sur.X;
     a.lat = c->lat;
+    a.cell_m = c->cell_m;
     a.albedo = 0.2f;
     return a;
 }
 
 /* Sun-bin SVF, annual direct + diffuse and the feature matrix; returns the kernel ms of each. */
-void prepare(sgpu_ctx* c, float lat, sgpu_surrogate_stats* st) {
+void prepare(sgpu_ctx* c, float lat, float cell_m, sgpu_surrogate_stats* st) {
     need(c->scene && c->sky_set, SGPU_ERR_STATE, "scene/sky not set");
+    need(cell_m > 0.0f, SGPU_ERR_ARG, "cell size must be positive");
     const int64_t N = c->n_total;
     c->lat = lat;
+    c->cell_m = cell_m;
     if (!c->svf) c->svf = dalloc<uint8_t>((size_t)std::max(c->sky.svf_bytes, 1) * N);
     SurrogateState& s = c->sur;
     if (!s.X) {
@@ -90,11 +94,11 @@ void prepare(sgpu_ctx* c, float lat, sgpu_surrogate_stats* st) {
 
 extern "C" {
 
-int sgpu_surrogate_prepare(sgpu_ctx* c, float lat, sgpu_surrogate_stats* st) {
+int sgpu_surrogate_prepare(sgpu_ctx* c, float lat, float cell_m, sgpu_surrogate_stats* st) {
     return guarded([&] {
         need(c != nullptr, SGPU_ERR_ARG, "null context");
         c->set_device();
-        prepare(c, lat, st);
+        prepare(c, lat, cell_m, st);
     });
 }
 
@@ -135,13 +139,15 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, const sgpu_gate* gate, int ho
         need(c && mlp && gate && out_host, SGPU_ERR_ARG, "null argument");
         c->set_device();
         const int64_t N = c->n_total;
-        const int K = hours_per_pass > 0 ? hours_per_pass : 16;
+        const bool annual = hours_per_pass == SGPU_FALLBACK_ANNUAL;
+        need(annual || hours_per_pass >= 0, SGPU_ERR_ARG, "hours_per_pass must be >= 0 or SGPU_FALLBACK_ANNUAL");
+        const int K = hours_per_pass > 0 ? hours_per_pass : c->tune.hours_per_pass;
         sgpu_surrogate_stats local{};
         sgpu_surrogate_stats* s = st ? st : &local;
         *s = sgpu_surrogate_stats{};
         EventPair all, ef;
         SGPU_HIP(hipEventRecord(all.a, c->stream));
-        prepare(c, c->lat, s);
+        prepare(c, c->lat, c->cell_m, s);
         float* Y = dalloc<float>((size_t)N * 2);
         float* out = nullptr;
         int32_t* flags = nullptr;
@@ -180,8 +186,13 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, const sgpu_gate* gate, int ho
                 exact = dalloc<float>(nf);
                 launch_compact(flags, off, N, ids, c->stream);
                 L = subset_viewshed(c, ids, nf, true);
-                subset_hour_loop(c, ids, nf, L, SGPU_FMT_BARLE, K, exact);
-                launch_scatter(ids, exact, nf, out, c->stream);
+                if (annual) {
+                    subset_reflected_linear(c, ids, nf, L, SGPU_FMT_BARLE, c->sur.dd, exact);
+                    launch_scatter_sum(ids, exact, c->sur.dd, nf, out, c->stream);
+                } else {
+                    subset_hour_loop(c, ids, nf, L, SGPU_FMT_BARLE, K, exact);
+                    launch_scatter(ids, exact, nf, out, c->stream);
+                }
             }
             SGPU_HIP(hipEventRecord(ef.b, c->stream));
             SGPU_HIP(hipEventRecord(all.b, c->stream));

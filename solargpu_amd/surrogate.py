@@ -1,7 +1,7 @@
 """Per-voxel surrogate for the reflection pass: the MLP binding (sgpu_mlp_* of
 include/sgpu.h) and the surrogate pipeline helpers.
 
-The network (64 -> 4 x 256 GELU -> 2, BF16 weights/activations, FP32
+The network (49 features padded to 64 -> 4 x 256 GELU -> 2, BF16 weights/activations, FP32
 accumulation on MFMA) is trained by the library's own forward, backward and
 Adam kernels on labels produced by the plain pipeline.
 """
@@ -13,8 +13,9 @@ import numpy as np
 
 from . import _check, _lib, _ptr, _c, _I64
 
+FALLBACK_ANNUAL = -1   # run(): exact fallback by one walk over the neighbours' annual totals
 IN_PAD = 64
-FEATURES = 48
+FEATURES = 49
 HIDDEN = 256
 OUTPUTS = 2
 
@@ -132,9 +133,9 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-def prepare(ctx, lat: float) -> dict:
+def prepare(ctx, lat: float, cell_m: float = 1.0) -> dict:
     st = Stats()
-    _check(_lib.sgpu_surrogate_prepare(ctx._h, ctypes.c_float(lat), ctypes.byref(st)))
+    _check(_lib.sgpu_surrogate_prepare(ctx._h, ctypes.c_float(lat), ctypes.c_float(cell_m), ctypes.byref(st)))
     return st.as_dict()
 
 
@@ -160,7 +161,9 @@ def targets(refl, dd, diff) -> np.ndarray:
     return np.stack([refl / scale, diff / scale], 1).astype(np.float32)
 
 
-def run(ctx, mlp: MLP, gate: Gate, hours_per_pass: int = 16):
+def run(ctx, mlp: MLP, gate: Gate, hours_per_pass: int = 0):
+    """Surrogate annual totals.  hours_per_pass > 0: per-hour exact fallback with K hours per
+    pass (0 = the context's tuned K); FALLBACK_ANNUAL: one-walk annual fallback."""
     n = ctx.info()["num_total"]
     out = np.zeros(n, np.float32)
     fb = np.zeros(n, np.int32)

@@ -82,7 +82,10 @@ def test_box_raises_the_horizon_towards_it():
     assert north.min() > 60.0 and np.abs(south).max() < 1.0
 
 
-def test_all_fallback_reproduces_plain():
+@pytest.mark.parametrize("fallback", [16, sm.FALLBACK_ANNUAL])
+def test_all_fallback_reproduces_plain(fallback):
+    """Every voxel gated: the per-hour fallback loop and the one-walk annual fallback both give
+    the plain method's annual totals (FP64 oracle, 1e-4)."""
     sc = synth_scene(11, 48, 40)
     ref = oracle_lists(11, 48, 40)
     az_alt, dni, dhi, bins = sky()
@@ -91,7 +94,7 @@ def test_all_fallback_reproduces_plain():
     with _ctx(sc, lists=False) as ctx:
         ctx_lat = SITE[1]
         sm.prepare(ctx, ctx_lat)
-        out, fb, st = sm.run(ctx, mlp, sm.Gate(-1.0, 1.0, 1.0, 0.0), 16)
+        out, fb, st = sm.run(ctx, mlp, sm.Gate(-1.0, 1.0, 1.0, 0.0), fallback)
     assert fb.all() and st["fallback"] == sc.num_total
     assert _rel(out, exact).max() <= 1e-4
 
