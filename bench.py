@@ -225,10 +225,11 @@ def main():
                                            (t_vs + t_hr) / args.steps, H)
     copy_gbps = ctx.copy_bandwidth(1 << 30, 20)
     if not args.no_csr:
-        ctx.hour_loop(sg.FMT_CSR, 16, args.hour_stride)
+        tc = ctx.autotune(sg.FMT_CSR) if not args.no_autotune else ctx.tuning()
+        ctx.hour_loop(sg.FMT_CSR, 0, args.hour_stride)
         c = ctx.info()
         extra["csr_voxel_hours_per_s"] = N * c["hours_computed"] / (c["t_hours_ms"] / 1e3)
-        extra["csr_hours_per_pass"] = 16
+        extra["csr_tuning"] = {k: tc[k] for k in ("hours_per_pass", "waves_per_group", "xcd_band")}
     if tuning is not None:
         extra["autotune"] = tuning
     res = {

@@ -116,6 +116,9 @@ def main(argv=None):
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--sample", type=int, default=65536, help="voxels per training tile")
     ap.add_argument("--calib-tiles", type=int, default=8)
+    ap.add_argument("--calib-size", type=int, default=0, help="calibration tile side (0 = --size)")
+    ap.add_argument("--calibrate-only", metavar="WEIGHTS", default=None,
+                    help="keep the network of WEIGHTS, fit only the gate, write --out")
     ap.add_argument("--epochs", type=int, default=24, help="split in three phases at lr, lr/3, lr/10")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -124,39 +127,47 @@ def main(argv=None):
     args = ap.parse_args(argv)
     rng = np.random.default_rng(1)
     t0 = time.perf_counter()
-    Xs, Ts = [], []
-    for k in range(args.tiles):
-        X, T, _, _ = labelled(TRAIN_SEED + 1000 * k, args.size, args.sample, rng)
-        Xs.append(X)
-        Ts.append(T)
-        if k % 20 == 0:
-            print(f"tile {k}: {X.shape[0]} samples, mean reflected share {T[:, 0].mean():.4f} "
-                  f"({time.perf_counter() - t0:.0f} s)", flush=True)
-    X = np.concatenate(Xs)
-    T = np.concatenate(Ts)
-    del Xs, Ts
-    t1 = time.perf_counter()
     mlp = sm.MLP(seed=11)
-    mlp.set_loss_weights(100.0, 1.0)
-    phases = [args.epochs - 2 * (args.epochs // 3), args.epochs // 3, args.epochs // 3]
-    losses = np.concatenate([mlp.train(X, T, epochs=e, batch=args.batch, lr=args.lr * f, seed=5 + i)
-                             for i, (e, f) in enumerate(zip(phases, (1.0, 1.0 / 3.0, 0.1))) if e > 0])
-    t2 = time.perf_counter()
-    print(f"trained on {X.shape[0]} samples: epoch losses {np.round(losses, 6).tolist()} ({t2 - t1:.1f} s)",
-          flush=True)
+    if args.calibrate_only:
+        params, _, old_meta = sm.load(args.calibrate_only)
+        mlp.set_params(params)
+        meta = {"network_from": old_meta}
+        t1 = t2 = time.perf_counter()
+    else:
+        Xs, Ts = [], []
+        for k in range(args.tiles):
+            X, T, _, _ = labelled(TRAIN_SEED + 1000 * k, args.size, args.sample, rng)
+            Xs.append(X)
+            Ts.append(T)
+            if k % 20 == 0:
+                print(f"tile {k}: {X.shape[0]} samples, mean reflected share {T[:, 0].mean():.4f} "
+                      f"({time.perf_counter() - t0:.0f} s)", flush=True)
+        X = np.concatenate(Xs)
+        T = np.concatenate(Ts)
+        del Xs, Ts
+        t1 = time.perf_counter()
+        mlp.set_loss_weights(100.0, 1.0)
+        phases = [args.epochs - 2 * (args.epochs // 3), args.epochs // 3, args.epochs // 3]
+        losses = np.concatenate([mlp.train(X, T, epochs=e, batch=args.batch, lr=args.lr * f, seed=5 + i)
+                                 for i, (e, f) in enumerate(zip(phases, (1.0, 1.0 / 3.0, 0.1))) if e > 0])
+        t2 = time.perf_counter()
+        print(f"trained on {X.shape[0]} samples: epoch losses {np.round(losses, 6).tolist()} ({t2 - t1:.1f} s)",
+              flush=True)
     Yc, Tc = [], []
     for k in range(args.calib_tiles):
-        Xk, Tk, _, _ = labelled(CALIB_SEED + 1000 * k, args.size, None, rng)
+        Xk, Tk, _, _ = labelled(CALIB_SEED + 1000 * k, args.calib_size or args.size, None, rng)
         Yc.append(mlp.forward(Xk))
         Tc.append(Tk)
     Yc = np.concatenate(Yc)
     Tc = np.concatenate(Tc)
     gate, f = calibrate(Yc, Tc, args.target_p95)
     share_err = np.abs(np.maximum(Yc[:, 0], 0) - Tc[:, 0])
-    meta = {"tiles": args.tiles, "size": args.size, "samples": int(X.shape[0]), "epochs": args.epochs,
-            "losses": [float(x) for x in losses], "calib_fallback": f,
-            "calib_share_abs_err_p95": float(np.percentile(share_err, 95)),
-            "train_s": t2 - t1, "label_s": t1 - t0}
+    if not args.calibrate_only:
+        meta = {"tiles": args.tiles, "size": args.size, "samples": int(X.shape[0]), "epochs": args.epochs,
+                "losses": [float(x) for x in losses], "train_s": t2 - t1, "label_s": t1 - t0}
+    meta.update({"calib_tiles": args.calib_tiles, "calib_size": args.calib_size or args.size,
+                 "target_p95": args.target_p95, "calib_fallback": f,
+                 "calib_share_abs_err_p95": float(np.percentile(share_err, 95))})
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     sm.save(args.out, mlp.params(), gate, meta)
     print(json.dumps({"out": args.out, "gate": [gate.tau, gate.refl_rel_err, gate.diff_miss_weight], **meta}),
