@@ -112,7 +112,7 @@ def surrogate_leg(sg, sc, az_alt, dni, dhi, cell, tuning, exact, plain_s, hours)
         c2.set_scene(sc)
         c2.set_sky(az_alt, dni, dhi)
         c2.set_tuning(tuning["dirs_per_group"], tuning["hours_per_pass"], tuning["waves_per_group"],
-                      tuning["xcd_band"])
+                      tuning["xcd_band"], tuning["group_blocks"])
         sm.prepare(c2, SITE[1], cell)
         for name, k in (("annual_fallback", sm.FALLBACK_ANNUAL), ("hourly_fallback", 0)):
             tot, fb, st = sm.run(c2, mlp, gate, k)
@@ -229,7 +229,7 @@ def main():
         ctx.hour_loop(sg.FMT_CSR, 0, args.hour_stride)
         c = ctx.info()
         extra["csr_voxel_hours_per_s"] = N * c["hours_computed"] / (c["t_hours_ms"] / 1e3)
-        extra["csr_tuning"] = {k: tc[k] for k in ("hours_per_pass", "waves_per_group", "xcd_band")}
+        extra["csr_tuning"] = {k: tc[k] for k in ("hours_per_pass", "waves_per_group", "xcd_band", "group_blocks")}
     if tuning is not None:
         extra["autotune"] = tuning
     res = {

@@ -171,8 +171,8 @@ int sgpu_sync(sgpu_ctx* ctx);
 
 /* Autotuner (new).  Launch shapes of the hot kernels, measured on the context's own scene:
    the DDA tile (directions per workgroup of 256 voxels), the hours per stage-2 pass, the
-   waves per reflection workgroup (LDS footprint / occupancy) and the order in which the
-   eight XCDs walk the voxels (cache sharing).  Results persist in a
+   waves per reflection workgroup (LDS footprint / occupancy), the order in which the
+   eight XCDs walk the voxels and how the resident waves share receivers (cache sharing).  Results persist in a
    text cache ($SGPU_AUTOTUNE_CACHE, else ~/.cache/solargpu/autotune.txt) keyed by device,
    format and scene size class; a cached key is applied without measuring.  Stage-2 shapes
    are tuned only when the lists of `format` exist.  Every shape gives identical results. */
@@ -182,6 +182,9 @@ typedef struct sgpu_tuning {
     int32_t waves_per_group;     /* 2 or 4 */
     int32_t xcd_band;            /* reflection workgroups per XCD band: 0 (one contiguous eighth
                                     of the voxels per XCD) or 16..4096 (XCDs side by side) */
+    int32_t group_blocks;        /* reflection workgroups whose waves interleave their receivers
+                                    over a tile-major voxel order: 0 (each wave 64 consecutive
+                                    voxels) or 2..1024 */
     int32_t from_cache;
     double rays_per_s;           /* best measured trace rate (0 if from cache) */
     double voxel_hours_per_s;    /* best measured stage-2 rate (0 if from cache / not tuned) */

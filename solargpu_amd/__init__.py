@@ -70,7 +70,8 @@ class Info(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("dirs_per_group", ctypes.c_int32), ("hours_per_pass", ctypes.c_int32),
-                ("waves_per_group", ctypes.c_int32), ("xcd_band", ctypes.c_int32), ("from_cache", ctypes.c_int32),
+                ("waves_per_group", ctypes.c_int32), ("xcd_band", ctypes.c_int32), ("group_blocks", ctypes.c_int32),
+                ("from_cache", ctypes.c_int32),
                 ("rays_per_s", ctypes.c_double), ("voxel_hours_per_s", ctypes.c_double)]
 
     def as_dict(self) -> dict:
@@ -396,8 +397,8 @@ class Context:
         return t.as_dict()
 
     def set_tuning(self, dirs_per_group: int = 16, hours_per_pass: int = 64, waves_per_group: int = 4,
-                   xcd_band: int = 0) -> None:
-        t = Tuning(dirs_per_group, hours_per_pass, waves_per_group, xcd_band, 0, 0.0, 0.0)
+                   xcd_band: int = 0, group_blocks: int = 0) -> None:
+        t = Tuning(dirs_per_group, hours_per_pass, waves_per_group, xcd_band, group_blocks, 0, 0.0, 0.0)
         _check(_lib.sgpu_set_tuning(self._h, ctypes.byref(t)))
 
     def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:

@@ -67,4 +67,28 @@ void top_slope_aspect(const float* H, int w, int h, double sx, double sy, float*
     }
 }
 
+std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32_t* pp, int tile) {
+    const int64_t n_top = (int64_t)w * h;
+    /* walls grouped by parent cell, stable (counting sort) */
+    std::vector<int64_t> start(n_top + 1, 0);
+    for (int64_t v = n_top; v < n_total; ++v) ++start[(int64_t)pp[v] + 1];
+    for (int64_t c = 0; c < n_top; ++c) start[c + 1] += start[c];
+    std::vector<int32_t> walls((size_t)(n_total - n_top));
+    {
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t v = n_top; v < n_total; ++v) walls[(size_t)fill[pp[v]]++] = (int32_t)v;
+    }
+    std::vector<int32_t> order;
+    order.reserve((size_t)n_total);
+    for (int ty = 0; ty < h; ty += tile)
+        for (int tx = 0; tx < w; tx += tile)
+            for (int y = ty; y < std::min(h, ty + tile); ++y)
+                for (int x = tx; x < std::min(w, tx + tile); ++x) {
+                    const int64_t c = (int64_t)y * w + x;
+                    order.push_back((int32_t)c);
+                    for (int64_t k = start[c]; k < start[c + 1]; ++k) order.push_back(walls[(size_t)k]);
+                }
+    return order;
+}
+
 }  // namespace sgpu

@@ -171,6 +171,7 @@ constexpr int kListCap = 128;
 /* Wave-wide copy of src[lo, hi) into LDS (lo and src 16-byte aligned). */
 __device__ __forceinline__ void stage_copy(uint8_t* dst, const uint8_t* __restrict__ src, int64_t lo, int64_t hi,
                                            int lane) {
+    if (hi <= lo) return;
     const int n = (int)(hi - lo);
     const int full = n & ~15;
     const uint8_t* s = src + lo;
@@ -191,7 +192,7 @@ struct StreamWindow {
     uint8_t* lds;
     const uint8_t* g;   // the whole stream (16-byte aligned)
     int64_t lo, hi;
-    int64_t end;        // end of the wave's stretch
+    int64_t end;        // the window may run ahead up to here (valid stream bytes)
     int cap;
 
     __device__ __forceinline__ bool fits(int64_t a, int64_t b) const { return b - a <= cap - 15; }
@@ -200,7 +201,7 @@ struct StreamWindow {
         if (a < lo || b > hi) {
             wave_lds_sync();
             lo = a & ~(int64_t)15;
-            hi = min(lo + (int64_t)cap, end);
+            hi = min(lo + (int64_t)cap, max(end, b));
             stage_copy(lds, g, lo, hi, lane);
             wave_lds_sync();
         }
@@ -288,21 +289,39 @@ template <int K, int FMT, bool DBG, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
     constexpr int kCap = (FMT == SGPU_FMT_BARLE) ? kStageBytes / WAVES : 16;
     __shared__ Recv rec[WAVES][64];
-    __shared__ int64_t beg[WAVES][64 + 1];
+    __shared__ int64_t beg[WAVES][64];   // each receiver's list [beg, fin)
+    __shared__ int64_t fin[WAVES][64];
     __shared__ int list_id[WAVES][kListCap];
     __shared__ float list_w[WAVES][kListCap];
     __shared__ __align__(16) uint8_t stage[WAVES][kCap];
 
     const unsigned lb = xcd_block(blockIdx.x, gridDim.x, (unsigned)a.band);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    /* positions p index the neighbour offsets and the accumulators; p is the voxel id
-       itself unless a subset of voxel ids is given */
     const int64_t n = a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total);
-    const int64_t p0 = ((int64_t)lb * WAVES + wave) * 64;
-    const int nv = (p0 < n) ? (int)min<int64_t>(64, n - p0) : 0;
-    const int64_t p = p0 + lane;
+    /* this wave's receivers: positions first + j * S, j = 0..63 */
+    int64_t first, S;
+    if (a.group > 1) {
+        S = (int64_t)a.group * WAVES;
+        first = (int64_t)(lb / (unsigned)a.group) * S * 64 + (int64_t)(lb % (unsigned)a.group) * WAVES + wave;
+    } else {
+        S = 1;
+        first = ((int64_t)lb * WAVES + wave) * 64;
+    }
+    const int nv = (first < n) ? (int)min<int64_t>(64, (n - first + S - 1) / S) : 0;
+    const int64_t p = first + (int64_t)lane * S;
     const bool live = lane < nv;
-    const int64_t v = live ? (a.subset ? (int64_t)a.subset[p] : a.v_base + p) : 0;
+    /* v: the voxel (geometry); q: its index into nb_off, sum, comp and dbg_* -- the voxel id
+       in full-scene order mode, else the position (subset / chunk-local lists) */
+    int64_t v = 0, q = 0;
+    if (live) {
+        if (a.order) {
+            v = a.order[p];
+            q = v;
+        } else {
+            v = a.subset ? (int64_t)a.subset[p] : a.v_base + p;
+            q = p;
+        }
+    }
     const float inv_w = 1.0f / (float)a.W;
     const int cells = a.W * a.H;
 
@@ -323,8 +342,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
             r.nx = ss * sa;
             r.ny = -ss * ca;
             r.nz = cs;
-            beg[wave][lane] = a.nb_off[p];
-            if (lane == nv - 1) beg[wave][nv] = a.nb_off[p + 1];
+            beg[wave][lane] = a.nb_off[q];
+            fin[wave][lane] = a.nb_off[q + 1];
             for (int h = 0; h < K; ++h) {
                 if (h < a.nh) {
                     const HourConst& hc = a.hours[h];
@@ -338,10 +357,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
 
     /* phase 2: one receiver at a time */
     float refl = 0.0f;   // lane j: receiver j's reflected irradiance summed over the pass
-    StreamWindow win{stage[wave], a.nb_bytes, 0, 0, (nv > 0) ? uniform64(beg[wave][nv]) : 0, kCap};
+    /* consecutive receivers (S == 1) mostly have neighbouring streams: the window runs ahead
+       over the buffer; interleaved receivers (S > 1) each stage only their own list */
+    const int64_t wend = (S == 1) ? a.nb_len : -1;
+    StreamWindow win{stage[wave], a.nb_bytes, 0, 0, 0, kCap};
     for (int j = 0; j < nv; ++j) {
         const Recv r = rec[wave][j];
-        const int64_t s0 = uniform64(beg[wave][j]), s1 = uniform64(beg[wave][j + 1]);
+        const int64_t s0 = uniform64(beg[wave][j]), s1 = uniform64(fin[wave][j]);
+        win.end = wend >= 0 ? wend : s1;
         float acc = 0.0f;
         int cnt = 0;
         auto take = [&](bool valid, int id) __attribute__((always_inline)) {
@@ -383,18 +406,18 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
     /* phase 3: lane = voxel */
     if (!live) return;
     if constexpr (DBG) {
-        a.dbg_hourly[p] = own;
-        a.dbg_reflected[p] = refl;
+        a.dbg_hourly[q] = own;
+        a.dbg_reflected[q] = refl;
         return;
     }
-    float s = a.sum[p], c = a.comp[p];
+    float s = a.sum[q], c = a.comp[q];
     const float x = refl + own;
     const float y = x - c;
     const float t = s + y;
     c = (t - s) - y;
     s = t;
-    a.sum[p] = s;
-    a.comp[p] = c;
+    a.sum[q] = s;
+    a.comp[q] = c;
 }
 
 template <int K>
@@ -406,7 +429,8 @@ void solar_top_k(const HourArgs& a, hipStream_t s) {
 template <int K, bool DBG, int WAVES>
 void reflect_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
     const int64_t n = a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total);
-    const int64_t blocks = (n + 64 * WAVES - 1) / (64 * WAVES);
+    int64_t blocks = (n + 64 * WAVES - 1) / (64 * WAVES);
+    if (a.group > 1) blocks = (blocks + a.group - 1) / a.group * a.group;   // whole receiver groups
     /* round up to a multiple of the XCD count so the remap covers every block */
     const unsigned g = (unsigned)(((blocks + kXcds - 1) / kXcds) * kXcds);
     const dim3 blk(64 * WAVES);

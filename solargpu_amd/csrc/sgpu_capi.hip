@@ -109,12 +109,15 @@ HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
     a.comp = c->comp;
     a.waves = c->tune.waves_per_group;
     a.band = c->tune.xcd_band;
+    a.order = c->order;   // subset and streamed callers clear it
+    a.group = c->tune.group_blocks;
     if (fmt == SGPU_FMT_CSR) {
         a.nb_off = c->csr_off;
         a.nb_ids = c->csr_ids;
     } else {
         a.nb_off = c->br_off;
         a.nb_bytes = c->br_bytes;
+        a.nb_len = c->br_size;
     }
     return a;
 }
@@ -192,10 +195,12 @@ void run_hour_passes_streamed(sgpu_ctx* c, unsigned fmt, int K, int nsel, std::v
                 SGPU_HIP(hipStreamWaitEvent(c->stream, ready[b], 0));
                 HourArgs t = a;
                 t.v_base = p.v0;
+                t.order = nullptr;
                 t.n_sub = p.count;
                 t.nb_off = doff[b];
                 t.nb_ids = reinterpret_cast<const int32_t*>(ddata[b]);
                 t.nb_bytes = ddata[b];
+                t.nb_len = p.bytes;
                 t.sum = c->sum + p.v0;
                 t.comp = c->comp + p.v0;
                 launch_reflect_accum(t, K, fmt, c->stream);
@@ -618,6 +623,7 @@ void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const Subse
         zero_floats(comp, n, c->stream);
         HourArgs a = hour_args(c, fmt);
         a.subset = subset;
+        a.order = nullptr;
         a.n_sub = n;
         a.sum = out;
         a.comp = comp;
@@ -627,6 +633,7 @@ void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const Subse
         } else {
             a.nb_off = L.br_off;
             a.nb_bytes = L.br_bytes;
+            a.nb_len = L.br_size;
         }
         const int nsel = (int)c->hours.size();
         for (int h0 = 0; h0 < nsel; h0 += K) {
@@ -649,6 +656,7 @@ void subset_reflected_linear(sgpu_ctx* c, const int32_t* subset, int64_t n, cons
     ensure_hours_d(c, c->hours);
     HourArgs a = hour_args(c, fmt);
     a.subset = subset;
+    a.order = nullptr;
     a.n_sub = n;
     if (fmt == SGPU_FMT_CSR) {
         a.nb_off = L.csr_off;
@@ -656,6 +664,7 @@ void subset_reflected_linear(sgpu_ctx* c, const int32_t* subset, int64_t n, cons
     } else {
         a.nb_off = L.br_off;
         a.nb_bytes = L.br_bytes;
+        a.nb_len = L.br_size;
     }
     a.hourly_top = const_cast<float*>(dd);   // K = 1: the top voxels' annual totals
     a.hours = c->hours_d;
@@ -863,6 +872,9 @@ int sgpu_set_scene(sgpu_ctx* c, const float* grid, int w, int h, int64_t n_total
         SGPU_HIP(hipMemcpyAsync(c->aspects, aspects, n_total * sizeof(float), hipMemcpyHostToDevice, c->stream));
         if (nw > 0)
             SGPU_HIP(hipMemcpyAsync(c->wall_z, wall_z, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        const std::vector<int32_t> order = processing_order(w, h, n_total, pp, 32);
+        c->order = dalloc<int32_t>(n_total);
+        SGPU_HIP(hipMemcpyAsync(c->order, order.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
         SGPU_HIP(hipStreamSynchronize(c->stream));
         c->scene = true;
     });

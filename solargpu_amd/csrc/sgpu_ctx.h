@@ -126,6 +126,7 @@ struct sgpu_ctx {
     float* wall_z = nullptr;
     float* slopes = nullptr;
     float* aspects = nullptr;
+    int32_t* order = nullptr;    // reflection-pass processing order (tile-major cells, walls after their cell)
     bool scene = false;
 
     bool sky_set = false;
@@ -157,7 +158,7 @@ struct sgpu_ctx {
     bool count_steps = false;
     sgpu_info info{};
     sgpu::SurrogateState sur;
-    sgpu_tuning tune{16, 64, 4, 0, 0, 0.0, 0.0};   // trace group, hours per pass, waves, XCD band; from_cache
+    sgpu_tuning tune{16, 64, 4, 0, 0, 0, 0.0, 0.0};   // trace group, hours/pass, waves, XCD band, group; from_cache
     int residency = SGPU_LISTS_DEVICE;          // where sgpu_viewshed leaves the lists
     sgpu::HostLists host;                       // the lists when host-resident
     hipStream_t copy_stream = nullptr;          // H2D tile copies of the streamed hour loop
@@ -183,6 +184,7 @@ struct sgpu_ctx {
         sgpu::dfree(wall_z);
         sgpu::dfree(slopes);
         sgpu::dfree(aspects);
+        sgpu::dfree(order);
         sgpu::dfree(sum);
         sgpu::dfree(comp);
         sgpu::dfree(hourly_top);

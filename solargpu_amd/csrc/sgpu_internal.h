@@ -71,6 +71,9 @@ Walls detect_walls(const float* heights, int w, int h, float voxel);
 void top_slope_aspect(const float* heights, int w, int h, double sx, double sy, float* slope, float* aspect);
 
 void synth_dsm(uint64_t seed, int w, int h, float cell_m, float density, int trees, float* out);
+/* Reflection-pass processing order: cells in tiles of tile x tile (tiles and cells in raster
+   order), each top voxel followed by the wall voxels it parents. */
+std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32_t* pix_parent, int tile);
 void synth_weather(uint64_t seed, const float* az_alt, float* dni, float* dhi);
 
 /* ---- device kernels (k_viewshed.hip, k_barle.hip, k_hourly.hip) ---- */
@@ -167,6 +170,7 @@ struct HourArgs {
     const int64_t* nb_off;     // CSR offsets or ba-RLE byte offsets
     const int32_t* nb_ids;
     const uint8_t* nb_bytes;
+    int64_t nb_len;            // bytes in nb_bytes (ba-RLE): the stream window never reads past it
     float* sum;                // Kahan annual sum
     float* comp;               // Kahan compensation
     float* dbg_hourly;         // optional single-hour outputs
@@ -178,6 +182,13 @@ struct HourArgs {
     int64_t v_base;
     int waves;                 // waves per k_reflect workgroup: 2 or 4 (0 = 4)
     int band;                  // k_reflect blocks per XCD band (0 = one contiguous eighth per XCD)
+    /* Receiver assignment.  group <= 1: wave w of logical block b takes the 64 consecutive
+       positions from (b * waves + w) * 64.  group > 1: the waves of `group` consecutive blocks
+       (S = group * waves of them) share 64 * S positions, wave i taking every S-th from i, so
+       at any moment the resident waves work on S adjacent positions.  order (full scene only):
+       position p is voxel order[p], and voxel ids index nb_off, sum, comp and dbg_*. */
+    const int32_t* order;
+    int group;
 };
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s);
 void launch_reflect_accum(const HourArgs& a, int K, unsigned format, hipStream_t s);

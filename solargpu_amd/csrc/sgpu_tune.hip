@@ -6,8 +6,9 @@
  *   stage-2 pass   hours per pass K in {16, 32, 64} x waves per reflection
  *                  workgroup in {2, 4} (LDS footprint and occupancy) x XCD
  *                  band in {0, 64, 256} (contiguous eighths or side-by-side
- *                  bands): three passes each on the live lists, scratch
- *                  accumulators.
+ *                  bands), then receiver groups in {0, 8, 32, 128} x band in
+ *                  {0, 256} for the best of those: three passes each on the
+ *                  live lists, scratch accumulators.
  * Every shape computes identical results; only time differs.  The winners are
  * appended to a text cache keyed by device, format and scene size class. */
 #include <cmath>
@@ -57,7 +58,8 @@ bool lookup(const std::string& key, sgpu_tuning& t) {
         std::istringstream in(line);
         std::string k;
         sgpu_tuning r{};
-        if (in >> k >> r.dirs_per_group >> r.hours_per_pass >> r.waves_per_group >> r.xcd_band && k == key) {
+        if (in >> k >> r.dirs_per_group >> r.hours_per_pass >> r.waves_per_group >> r.xcd_band >> r.group_blocks &&
+            k == key) {
             t = r;
             found = true;   // the last entry for a key wins
         }
@@ -68,7 +70,7 @@ bool lookup(const std::string& key, sgpu_tuning& t) {
 void store(const std::string& key, const sgpu_tuning& t) {
     std::ofstream f(cache_path(), std::ios::app);
     if (f) f << key << " " << t.dirs_per_group << " " << t.hours_per_pass << " " << t.waves_per_group << " "
-             << t.xcd_band << "\n";
+             << t.xcd_band << " " << t.group_blocks << "\n";
 }
 
 double time_trace(sgpu_ctx* c, int G, int64_t v0, int64_t n, int32_t* raw) {
@@ -83,10 +85,11 @@ double time_trace(sgpu_ctx* c, int G, int64_t v0, int64_t n, int32_t* raw) {
     return e.ms();
 }
 
-double time_passes(sgpu_ctx* c, unsigned fmt, int K, int waves, int band, int passes) {
+double time_passes(sgpu_ctx* c, unsigned fmt, int K, int waves, int band, int group, int passes) {
     HourArgs a = stage2_args(c, fmt);
     a.waves = waves;
     a.band = band;
+    a.group = group;
     const int nsel = (int)c->hours.size();
     auto run = [&](int np) {
         for (int p = 0, h0 = 0; p < np && h0 < nsel; ++p, h0 += K) {
@@ -140,20 +143,25 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
         ensure_accumulators(c, 64);
         ensure_hours_d(c, c->hours);
         double best_rate = 0.0;
+        const int passes = 3;
+        auto trial = [&](int K, int waves, int band, int group) {
+            const double ms = time_passes(c, fmt, K, waves, band, group, passes);
+            const int hours = std::min<int>(passes * K, (int)c->hours.size());
+            const double rate = (double)c->n_total * hours / (ms * 1e-3);
+            if (rate > best_rate) {
+                best_rate = rate;
+                t.hours_per_pass = K;
+                t.waves_per_group = waves;
+                t.xcd_band = band;
+                t.group_blocks = group;
+            }
+        };
         for (int K : {16, 32, 64})
             for (int waves : {2, 4})
-                for (int band : {0, 64, 256}) {
-                    const int passes = 3;
-                    const double ms = time_passes(c, fmt, K, waves, band, passes);
-                    const int hours = std::min<int>(passes * K, (int)c->hours.size());
-                    const double rate = (double)c->n_total * hours / (ms * 1e-3);
-                    if (rate > best_rate) {
-                        best_rate = rate;
-                        t.hours_per_pass = K;
-                        t.waves_per_group = waves;
-                        t.xcd_band = band;
-                    }
-                }
+                for (int band : {0, 64, 256}) trial(K, waves, band, 0);
+        const int K = t.hours_per_pass, waves = t.waves_per_group;
+        for (int group : {8, 32, 128})
+            for (int band : {0, 256}) trial(K, waves, band, group);
         t.voxel_hours_per_s = best_rate;
         c->irr_ok = false;   // the scratch passes overwrote the annual accumulators
     }
@@ -183,6 +191,8 @@ int sgpu_set_tuning(sgpu_ctx* c, const sgpu_tuning* t) {
         need(t->waves_per_group == 2 || t->waves_per_group == 4, SGPU_ERR_ARG, "waves_per_group must be 2 or 4");
         need(t->xcd_band == 0 || (t->xcd_band >= 16 && t->xcd_band <= 4096), SGPU_ERR_ARG,
              "xcd_band must be 0 or 16..4096");
+        need(t->group_blocks == 0 || (t->group_blocks >= 2 && t->group_blocks <= 1024), SGPU_ERR_ARG,
+             "group_blocks must be 0 or 2..1024");
         c->tune = *t;
     });
 }

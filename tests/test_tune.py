@@ -24,8 +24,9 @@ def test_every_shape_is_results_neutral():
     az_alt, dni, dhi, bins = sky()
     exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
     with _ctx(sc) as ctx:
-        for G, K, waves, band in ((16, 8, 2, 0), (48, 32, 4, 16), (32, 16, 2, 0), (32, 64, 4, 0)):
-            ctx.set_tuning(G, K, waves, band)
+        for G, K, waves, band, group in ((16, 8, 2, 0, 0), (48, 32, 4, 16, 0), (32, 16, 2, 0, 4),
+                                         (32, 64, 4, 0, 2)):
+            ctx.set_tuning(G, K, waves, band, group)
             ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
             off, ids = ctx.csr()
             np.testing.assert_array_equal(ids, ref["ids"])
@@ -49,3 +50,20 @@ def test_autotune_measures_then_hits_the_cache(tmp_path, monkeypatch):
         assert t2["from_cache"] == 1
         assert (t2["dirs_per_group"], t2["hours_per_pass"], t2["waves_per_group"]) == \
             (t["dirs_per_group"], t["hours_per_pass"], t["waves_per_group"])
+
+
+def test_receiver_assignment_is_bit_identical():
+    """Each voxel's reflection walk and Kahan sum are its own: consecutive receivers per wave,
+    receiver groups over the tile-major order and XCD bands give bit-identical annual sums."""
+    sc = synth_scene(13, 150, 120)
+    with _ctx(sc) as ctx:
+        ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
+        out = []
+        for fmt in (sg.FMT_CSR, sg.FMT_BARLE):
+            for waves, band, group in ((4, 0, 0), (4, 16, 8), (2, 0, 32), (2, 64, 3)):
+                ctx.set_tuning(16, 32, waves, band, group)
+                ctx.hour_loop(fmt, 32)
+                out.append(ctx.irradiance())
+            for x in out[1:]:
+                np.testing.assert_array_equal(x, out[0])
+            out.clear()
