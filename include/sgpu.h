@@ -251,6 +251,46 @@ int sgpu_surrogate_labels(sgpu_ctx* ctx, unsigned format, float* reflected);
 int sgpu_surrogate_run(sgpu_ctx* ctx, sgpu_mlp* mlp, const sgpu_gate* gate, int hours_per_pass, float* out,
                        int32_t* fallback, sgpu_surrogate_stats* stats);
 
+/* ---- batched tiles (new; no reference counterpart: the reference runs one DSM per call,
+   solar_gpu_api.h:37-54) ----
+   Many DSM tiles of different sizes and sites in one object: stage 1 is one persistent DDA
+   launch per voxel chunk over all tiles (workgroups take (256-voxel block, 32-direction)
+   items from a global counter) plus one list launch per chunk; stage 2 is one solar and one
+   reflection launch per pass of K hours covering every tile, the whole sunlit-hour loop
+   recorded as one hipGraph.  Each tile's lists, SVF bits and annual sums are the ones a
+   context of its own computes (bit-identical). */
+typedef struct sgpu_batch sgpu_batch;
+typedef struct sgpu_batch_info {
+    int32_t tiles;
+    int32_t chunks;              /* stage-1 voxel chunks */
+    int64_t voxels;              /* all tiles */
+    int64_t nnz;                 /* neighbour ids, all tiles */
+    int64_t barle_bytes;
+    int32_t hours_per_pass, passes;
+    int64_t voxel_hours;         /* sum over tiles of voxels x sunlit hours */
+    double t_viewshed_ms, t_trace_ms, t_lists_ms, t_encode_ms, t_hours_ms;
+} sgpu_batch_info;
+int sgpu_batch_create(int device, sgpu_batch** out);
+int sgpu_batch_destroy(sgpu_batch* b);
+/* Append one tile: the arrays of sgpu_set_scene plus its sky (sgpu_set_sky); index = its number. */
+int sgpu_batch_add_tile(sgpu_batch* b, const float* grid, int w, int h, int64_t num_total, int64_t num_top,
+                        const int32_t* pixel_parent, const float* wall_z, float max_height, const float* slopes,
+                        const float* aspects, const float* az_alt, const float* dni, const float* dhi,
+                        int32_t* index);
+/* Stage 1 for every tile: SVF bits, CSR lists and ba-RLE streams. */
+int sgpu_batch_viewshed(sgpu_batch* b);
+/* Stage 2 for every tile (format CSR or BARLE, K hours per pass, 2 or 4 waves per reflection
+   workgroup); graph != 0 records the loop as one hipGraph and launches that. */
+int sgpu_batch_hour_loop(sgpu_batch* b, unsigned format, int hours_per_pass, int waves, int graph);
+int sgpu_batch_get_info(sgpu_batch* b, sgpu_batch_info* out);
+/* Per tile: sizes, then the reference-layout products (offsets rebased to the tile). */
+int sgpu_batch_tile_sizes(sgpu_batch* b, int32_t tile, int64_t* num_total, int64_t* nnz, int64_t* barle_bytes,
+                          int32_t* svf_bytes);
+int sgpu_batch_get_svf(sgpu_batch* b, int32_t tile, uint8_t* svf);
+int sgpu_batch_get_csr(sgpu_batch* b, int32_t tile, int64_t* offsets, int32_t* ids);
+int sgpu_batch_get_barle(sgpu_batch* b, int32_t tile, int64_t* offsets, uint8_t* bytes);
+int sgpu_batch_get_irradiance(sgpu_batch* b, int32_t tile, float* out);
+
 /* ---- drop-in whole pipelines (solar_gpu_api.h:37-54) ----
    sun_az_alt: 2*8760 floats; dni/dhi: 8760 floats each. */
 int sgpu_uncompressed_irradiance(const float* grid_data, int grid_width, int grid_height, int num_total_voxels,

@@ -408,6 +408,96 @@ class Context:
         return g.value
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("tiles", ctypes.c_int32), ("chunks", ctypes.c_int32), ("voxels", ctypes.c_int64),
+                ("nnz", ctypes.c_int64), ("barle_bytes", ctypes.c_int64), ("hours_per_pass", ctypes.c_int32),
+                ("passes", ctypes.c_int32), ("voxel_hours", ctypes.c_int64), ("t_viewshed_ms", ctypes.c_double),
+                ("t_trace_ms", ctypes.c_double), ("t_lists_ms", ctypes.c_double), ("t_encode_ms", ctypes.c_double),
+                ("t_hours_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Batch:
+    """Many DSM tiles (own sizes and sites) through one batched pipeline (sgpu_batch_*)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        self._keep = []
+        _check(_lib.sgpu_batch_create(device, ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            _check(_lib.sgpu_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_tile(self, sc: Scene, az_alt, dni, dhi) -> int:
+        idx = ctypes.c_int32()
+        _check(_lib.sgpu_batch_add_tile(self._h, _ptr(sc.grid), sc.width, sc.height, _I64(sc.num_total),
+                                        _I64(sc.num_top), _ptr(sc.pixel_parent),
+                                        _ptr(sc.wall_z) if sc.wall_z.size else None, _F(sc.max_height),
+                                        _ptr(sc.slopes), _ptr(sc.aspects), _ptr(_c(az_alt, np.float32)),
+                                        _ptr(_c(dni, np.float32)), _ptr(_c(dhi, np.float32)), ctypes.byref(idx)))
+        return idx.value
+
+    def viewshed(self) -> None:
+        _check(_lib.sgpu_batch_viewshed(self._h))
+
+    def hour_loop(self, fmt: int = FMT_BARLE, hours_per_pass: int = 64, waves: int = 4, graph: bool = True) -> None:
+        _check(_lib.sgpu_batch_hour_loop(self._h, ctypes.c_uint(fmt), hours_per_pass, waves, int(graph)))
+
+    def info(self) -> dict:
+        i = BatchInfo()
+        _check(_lib.sgpu_batch_get_info(self._h, ctypes.byref(i)))
+        return i.as_dict()
+
+    def _sizes(self, t: int):
+        n, nnz, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        sb = ctypes.c_int32()
+        _check(_lib.sgpu_batch_tile_sizes(self._h, t, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nb),
+                                          ctypes.byref(sb)))
+        return n.value, nnz.value, nb.value, sb.value
+
+    def svf(self, t: int) -> np.ndarray:
+        n, _, _, sb = self._sizes(t)
+        out = np.zeros(max(sb * n, 1), np.uint8)
+        _check(_lib.sgpu_batch_get_svf(self._h, t, _ptr(out)))
+        return out[: sb * n]
+
+    def csr(self, t: int):
+        n, nnz, _, _ = self._sizes(t)
+        off = np.zeros(n + 1, np.int64)
+        ids = np.zeros(max(nnz, 1), np.int32)
+        _check(_lib.sgpu_batch_get_csr(self._h, t, _ptr(off), _ptr(ids)))
+        return off, ids[:nnz]
+
+    def barle(self, t: int):
+        n, _, nb, _ = self._sizes(t)
+        off = np.zeros(n + 1, np.int64)
+        data = np.zeros(max(nb, 1), np.uint8)
+        _check(_lib.sgpu_batch_get_barle(self._h, t, _ptr(off), _ptr(data)))
+        return off, data[:nb]
+
+    def irradiance(self, t: int) -> np.ndarray:
+        n = self._sizes(t)[0]
+        out = np.zeros(n, np.float32)
+        _check(_lib.sgpu_batch_get_irradiance(self._h, t, _ptr(out)))
+        return out
+
+
 def _whole(fn, sc: Scene, az_alt, dni, dhi) -> np.ndarray:
     out = np.zeros(sc.num_total, np.float32)
     _check(fn(_ptr(sc.grid), sc.width, sc.height, sc.num_total, sc.num_top, _ptr(sc.pixel_parent),

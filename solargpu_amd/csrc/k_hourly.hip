@@ -59,8 +59,7 @@ __device__ __forceinline__ bool svf_blocked(const uint8_t* svf, int64_t v, int d
 }
 
 template <int K>
-__global__ __launch_bounds__(kSolarBlock) void k_solar_top(HourArgs a) {
-    const int64_t c = (int64_t)blockIdx.x * kSolarBlock + threadIdx.x;
+__device__ __forceinline__ void solar_top_cell(const HourArgs& a, int64_t c) {
     if (c >= a.n_top) return;
     const float slope = a.slopes[c], aspect = a.aspects[c];
     float ss, cs, sa, ca;
@@ -84,6 +83,11 @@ __global__ __launch_bounds__(kSolarBlock) void k_solar_top(HourArgs a) {
 #pragma unroll
         for (int h = 0; h < K; ++h) dst[h] = out[h];
     }
+}
+
+template <int K>
+__global__ __launch_bounds__(kSolarBlock) void k_solar_top(HourArgs a) {
+    solar_top_cell<K>(a, (int64_t)blockIdx.x * kSolarBlock + threadIdx.x);
 }
 
 struct Recv {
@@ -286,7 +290,7 @@ __device__ __forceinline__ void wave_decode(StreamWindow& win, int64_t s0, int64
  * an LDS window of kStageBytes / WAVES bytes filled with 16-byte loads, block by block.
  * DBG (K = 1 only): write the hour's own and reflected irradiance instead of accumulating. */
 template <int K, int FMT, bool DBG, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
+__device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
     constexpr int kCap = (FMT == SGPU_FMT_BARLE) ? kStageBytes / WAVES : 16;
     __shared__ Recv rec[WAVES][64];
     __shared__ int64_t beg[WAVES][64];   // each receiver's list [beg, fin)
@@ -295,7 +299,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
     __shared__ float list_w[WAVES][kListCap];
     __shared__ __align__(16) uint8_t stage[WAVES][kCap];
 
-    const unsigned lb = xcd_block(blockIdx.x, gridDim.x, (unsigned)a.band);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t n = a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total);
     /* this wave's receivers: positions first + j * S, j = 0..63 */
@@ -420,6 +423,51 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
     a.comp[q] = c;
 }
 
+template <int K, int FMT, bool DBG, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
+    reflect_block<K, FMT, DBG, WAVES>(a, xcd_block(blockIdx.x, gridDim.x, (unsigned)a.band));
+}
+
+/* ---- batched tiles: one launch per pass covers every tile of a batch ---- */
+
+/* The tile owning block b: the last tile whose first block is <= b (wave-uniform). */
+__device__ __forceinline__ int tile_of_block(const BatchHour* d, int ntiles, int64_t b, bool solar) {
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((solar ? d[mid].top_block0 : d[mid].block0) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+/* The tile's arguments for pass `pass` (its hours pass * K ...); false once its hours are done. */
+template <int K>
+__device__ __forceinline__ bool tile_pass_args(const BatchHour& t, int pass, HourArgs& a) {
+    const int h0 = pass * K;
+    if (h0 >= t.nsel) return false;
+    a = t.a;
+    a.hours = t.a.hours + h0;
+    a.nh = min(K, t.nsel - h0);
+    return true;
+}
+
+template <int K>
+__global__ __launch_bounds__(kSolarBlock) void k_solar_top_batch(const BatchHour* d, int ntiles, int pass) {
+    const int t = tile_of_block(d, ntiles, blockIdx.x, true);
+    HourArgs a;
+    if (!tile_pass_args<K>(d[t], pass, a)) return;
+    solar_top_cell<K>(a, ((int64_t)blockIdx.x - d[t].top_block0) * kSolarBlock + threadIdx.x);
+}
+
+template <int K, int FMT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_reflect_batch(const BatchHour* d, int ntiles, int pass) {
+    const int t = tile_of_block(d, ntiles, blockIdx.x, false);
+    HourArgs a;
+    if (!tile_pass_args<K>(d[t], pass, a)) return;   // the whole block leaves together
+    reflect_block<K, FMT, false, WAVES>(a, (unsigned)((int64_t)blockIdx.x - d[t].block0));
+}
+
 template <int K>
 void solar_top_k(const HourArgs& a, hipStream_t s) {
     const unsigned g = (unsigned)((a.n_top + kSolarBlock - 1) / kSolarBlock);
@@ -436,6 +484,27 @@ void reflect_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
     const dim3 blk(64 * WAVES);
     if (fmt == SGPU_FMT_CSR) hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_CSR, DBG, WAVES>), dim3(g), blk, 0, s, a);
     else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, DBG, WAVES>), dim3(g), blk, 0, s, a);
+}
+
+template <int K>
+void batch_k(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, unsigned fmt, int waves, int pass,
+             hipStream_t s) {
+    hipLaunchKernelGGL(k_solar_top_batch<K>, dim3((unsigned)top_blocks), dim3(kSolarBlock), 0, s, d, ntiles, pass);
+    if (fmt == SGPU_FMT_CSR) {
+        if (waves == 2)
+            hipLaunchKernelGGL((k_reflect_batch<K, SGPU_FMT_CSR, 2>), dim3((unsigned)blocks), dim3(128), 0, s, d,
+                               ntiles, pass);
+        else
+            hipLaunchKernelGGL((k_reflect_batch<K, SGPU_FMT_CSR, 4>), dim3((unsigned)blocks), dim3(256), 0, s, d,
+                               ntiles, pass);
+    } else {
+        if (waves == 2)
+            hipLaunchKernelGGL((k_reflect_batch<K, SGPU_FMT_BARLE, 2>), dim3((unsigned)blocks), dim3(128), 0, s, d,
+                               ntiles, pass);
+        else
+            hipLaunchKernelGGL((k_reflect_batch<K, SGPU_FMT_BARLE, 4>), dim3((unsigned)blocks), dim3(256), 0, s, d,
+                               ntiles, pass);
+    }
 }
 
 template <int K>
@@ -484,6 +553,24 @@ void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s)
         case 16: reflect_k<16>(a, fmt, s); break;
         case 32: reflect_k<32>(a, fmt, s); break;
         case 64: reflect_k<64>(a, fmt, s); break;
+        default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
+    }
+    SGPU_HIP(hipGetLastError());
+}
+
+void launch_hour_pass_batch(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, int K,
+                            unsigned fmt, int waves, int pass, hipStream_t s) {
+    if (ntiles <= 0 || blocks <= 0) return;
+    if (fmt != SGPU_FMT_CSR && fmt != SGPU_FMT_BARLE) fail(SGPU_ERR_ARG, "format must be CSR or BARLE");
+    if (waves != 2 && waves != 4) fail(SGPU_ERR_ARG, "waves per group must be 2 or 4");
+    switch (K) {
+        case 1: batch_k<1>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 2: batch_k<2>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 4: batch_k<4>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 8: batch_k<8>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 16: batch_k<16>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 32: batch_k<32>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 64: batch_k<64>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());

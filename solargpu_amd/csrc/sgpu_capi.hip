@@ -404,9 +404,16 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
     return out;
 }
 
-/* ba-RLE streams of N CSR lists: size pass -> int64 scan -> write pass. */
 void encode_barle(sgpu_ctx* c, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,
                   uint8_t** br_bytes, int64_t* size) {
+    encode_barle_on(c->stream, off, ids, N, br_off, br_bytes, size);
+}
+
+}  // namespace
+
+/* ba-RLE streams of N CSR lists: size pass -> int64 scan -> write pass. */
+void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,
+                           uint8_t** br_bytes, int64_t* size) {
     BarleArgs ba{};
     ba.csr_off = off;
     ba.csr_ids = ids;
@@ -419,16 +426,16 @@ void encode_barle(sgpu_ctx* c, const int64_t* off, const int32_t* ids, int64_t N
         scr = dalloc<int64_t>(scan_scratch_elems(N));
         boff = dalloc<int64_t>(N + 1);
         ba.sizes = sizes;
-        launch_barle(ba, false, c->stream);
-        launch_scan_counts(sizes, N, boff, scr, c->stream);
+        launch_barle(ba, false, stream);
+        launch_scan_counts(sizes, N, boff, scr, stream);
         int64_t total = 0;
-        SGPU_HIP(hipMemcpyAsync(&total, boff + N, sizeof total, hipMemcpyDeviceToHost, c->stream));
-        SGPU_HIP(hipStreamSynchronize(c->stream));
+        SGPU_HIP(hipMemcpyAsync(&total, boff + N, sizeof total, hipMemcpyDeviceToHost, stream));
+        SGPU_HIP(hipStreamSynchronize(stream));
         bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
         ba.out_off = boff;
         ba.out = bytes;
-        launch_barle(ba, true, c->stream);
-        SGPU_HIP(hipStreamSynchronize(c->stream));
+        launch_barle(ba, true, stream);
+        SGPU_HIP(hipStreamSynchronize(stream));
         *size = total;
     } catch (...) {
         dfree(sizes);
@@ -442,6 +449,8 @@ void encode_barle(sgpu_ctx* c, const int64_t* off, const int32_t* ids, int64_t N
     *br_off = boff;
     *br_bytes = bytes;
 }
+
+namespace {
 
 void do_viewshed(sgpu_ctx* c, unsigned formats) {
     need(c->scene, SGPU_ERR_STATE, "scene not set");

@@ -191,6 +191,18 @@ struct HourArgs {
     int group;
 };
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s);
+
+/* Batched tiles in stage 2: one k_solar_top and one k_reflect launch per pass cover every
+   tile.  Tile t's arguments address its own scene, sky and lists (positions = its voxel ids,
+   n_sub = its voxel count, nb_off/sum/comp pointing at its slice of the batch arrays); its
+   reflection blocks start at block0, its solar blocks at top_block0. */
+struct BatchHour {
+    HourArgs a;          // hours: the tile's first sunlit-hour constant; nh unused
+    int nsel;            // the tile's sunlit hours
+    int64_t block0, top_block0;
+};
+void launch_hour_pass_batch(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, int K,
+                            unsigned fmt, int waves, int pass, hipStream_t s);
 void launch_reflect_accum(const HourArgs& a, int K, unsigned format, hipStream_t s);
 bool supported_hours_per_pass(int K);
 
