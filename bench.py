@@ -72,13 +72,13 @@ def cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, threads, budget_s):
     svf = ctx.svf()
     N, ntop = sc.num_total, sc.num_top
     hours_total = orc.sunlit_hours(bins["nudged"]).size
-    # stage 2 sample: 4000 top + 4000 wall voxels, every 200th sunlit hour
-    stride = 200
+    # stage 2 sample: 20000 top + 20000 wall voxels, every 80th sunlit hour
+    stride = 80
     rates = []
     t_top = t_rng = 0.0
     nvox = 0
     for v0 in (ntop // 2, ntop + (N - ntop) // 2):
-        cnt = min(4000, N - v0)
+        cnt = min(20000, N - v0)
         off, ids = ctx.csr_range(v0, cnt)
         _, nh, tt, tr = orc.annual_fp32_range(sc, bins, dni, dhi, svf, v0, off, ids, stride=stride,
                                               nthreads=threads)
@@ -88,15 +88,15 @@ def cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, threads, budget_s):
     # extrapolate: per sampled hour, all-top solar once + per-voxel cost for all N voxels
     per_hour = t_top / (2 * nh) + (t_rng / nh) * (N / nvox)
     vh_rate = N / per_hour
-    # stage 1 sample: DDA rays of 600 voxels (all 2592 directions)
+    # stage 1 sample: DDA rays of 3000 voxels (all 2592 directions)
     t0 = time.perf_counter()
-    orc.trace_raw(sc, bins, ntop // 2 + 1000, 300, nthreads=threads)
-    orc.trace_raw(sc, bins, ntop + (N - ntop) // 3, 300, nthreads=threads)
+    orc.trace_raw(sc, bins, ntop // 2 + 1000, 1500, nthreads=threads)
+    orc.trace_raw(sc, bins, ntop + (N - ntop) // 3, 1500, nthreads=threads)
     t1 = time.perf_counter()
-    rays_rate = 600 * 2592 / (t1 - t0)
-    sample = (f"oracle FP32 hour loop on 2 x 4000 voxels (top, wall) x {nh} sunlit hours (every {stride}th of "
+    rays_rate = 3000 * 2592 / (t1 - t0)
+    sample = (f"oracle FP32 hour loop on 2 x 20000 voxels (top, wall) x {nh} sunlit hours (every {stride}th of "
               f"{hours_total}), all-top solar per hour, extrapolated linearly to {N} voxels; "
-              f"DDA on 600 voxels x 2592 rays; {t_top + t_rng + (t1 - t0):.1f} s CPU")
+              f"DDA on 3000 voxels x 2592 rays; {t_top + t_rng + (t1 - t0):.1f} s CPU")
     return {"value": vh_rate, "unit": "voxel-hours/s", "cores": threads, "kind": "port", "sample": sample,
             "rays_per_s": rays_rate}
 
@@ -143,6 +143,7 @@ def main():
     ap.add_argument("--no-csr", action="store_true", help="skip the CSR hour-loop report")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer drop-in timing")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
 
@@ -224,6 +225,14 @@ def main():
         extra["surrogate"] = surrogate_leg(sg, sc, az_alt, dni, dhi, cell, ctx.tuning(), exact,
                                            (t_vs + t_hr) / args.steps, H)
     copy_gbps = ctx.copy_bandwidth(1 << 30, 20)
+    if not args.no_dropin:
+        # the drop-in entry point end to end: host arrays in (PCIe), viewshed, hour loop, host out
+        t0 = time.perf_counter()
+        out = sg.compressed_gpu_irradiance(sc, az_alt, dni, dhi)
+        dw = time.perf_counter() - t0
+        extra["dropin"] = {"entry": "compressed_gpu_irradiance", "wall_s": dw, "voxel_hours_per_s": N * H / dw,
+                           "max_rel_vs_bench": float((np.abs(out.astype(np.float64) - exact)
+                                                      / np.maximum(np.abs(exact.astype(np.float64)), 1.0)).max())}
     if not args.no_csr:
         tc = ctx.autotune(sg.FMT_CSR) if not args.no_autotune else ctx.tuning()
         ctx.hour_loop(sg.FMT_CSR, 0, args.hour_stride)

@@ -1246,6 +1246,7 @@ static int whole_pipeline(unsigned fmt, const float* grid, int w, int h, int n_t
     st = sgpu_set_scene(c, grid, w, h, n_total, n_top, pp, wall_z, maxz, slopes, aspects);
     if (!st) st = sgpu_set_sky(c, az_alt, dni, dhi);
     if (!st) st = sgpu_viewshed(c, fmt);
+    if (!st) st = guarded([&] { apply_cached_tuning(c, fmt); });   // shapes a past sgpu_autotune measured
     if (!st) st = sgpu_hour_loop(c, fmt, 0, 1);
     if (!st) st = sgpu_get_irradiance(c, out);
     std::string keep = last_error();

@@ -233,6 +233,8 @@ void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const Subse
 /* Annual reflected irradiance per voxel from the annual direct + diffuse of the top cells
    (device arrays), by single-bounce linearity. */
 void annual_reflected_linear(sgpu_ctx* c, unsigned fmt, const float* dd, float* out_dev);
+/* Apply the autotuner's cached launch shapes for this scene class, if any (no measuring). */
+bool apply_cached_tuning(sgpu_ctx* c, unsigned fmt);
 /* ba-RLE streams of N CSR lists on a stream (size pass, int64 scan, write pass); the caller
    owns *br_off (N + 1) and *br_bytes (*size bytes). */
 void encode_barle_on(hipStream_t s, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,

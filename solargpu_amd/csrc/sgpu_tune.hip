@@ -172,6 +172,16 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
 
 }  // namespace
 
+/* Apply a cached tuning of this context's scene class, if any, without measuring. */
+bool sgpu::apply_cached_tuning(sgpu_ctx* c, unsigned fmt) {
+    sgpu_tuning t = c->tune;
+    if (!lookup(tune_key(c, fmt), t)) return false;
+    t.from_cache = 1;
+    t.rays_per_s = t.voxel_hours_per_s = 0.0;
+    c->tune = t;
+    return true;
+}
+
 extern "C" {
 
 int sgpu_autotune(sgpu_ctx* c, unsigned format, sgpu_tuning* out) {
