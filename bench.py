@@ -72,13 +72,13 @@ def cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, threads, budget_s):
     svf = ctx.svf()
     N, ntop = sc.num_total, sc.num_top
     hours_total = orc.sunlit_hours(bins["nudged"]).size
-    # stage 2 sample: 20000 top + 20000 wall voxels, every 80th sunlit hour
-    stride = 80
+    # stage 2 sample: 50000 top + 50000 wall voxels, every 40th sunlit hour
+    stride = 40
     rates = []
     t_top = t_rng = 0.0
     nvox = 0
     for v0 in (ntop // 2, ntop + (N - ntop) // 2):
-        cnt = min(20000, N - v0)
+        cnt = min(50000, N - v0)
         off, ids = ctx.csr_range(v0, cnt)
         _, nh, tt, tr = orc.annual_fp32_range(sc, bins, dni, dhi, svf, v0, off, ids, stride=stride,
                                               nthreads=threads)
@@ -88,15 +88,15 @@ def cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, threads, budget_s):
     # extrapolate: per sampled hour, all-top solar once + per-voxel cost for all N voxels
     per_hour = t_top / (2 * nh) + (t_rng / nh) * (N / nvox)
     vh_rate = N / per_hour
-    # stage 1 sample: DDA rays of 3000 voxels (all 2592 directions)
+    # stage 1 sample: DDA rays of 4 x 40000 voxels (all 2592 directions)
     t0 = time.perf_counter()
-    orc.trace_raw(sc, bins, ntop // 2 + 1000, 1500, nthreads=threads)
-    orc.trace_raw(sc, bins, ntop + (N - ntop) // 3, 1500, nthreads=threads)
+    for v0 in (ntop // 3, 2 * ntop // 3, ntop + (N - ntop) // 3, ntop + 2 * (N - ntop) // 3):
+        orc.trace_raw(sc, bins, v0, 40000, nthreads=threads)
     t1 = time.perf_counter()
-    rays_rate = 3000 * 2592 / (t1 - t0)
-    sample = (f"oracle FP32 hour loop on 2 x 20000 voxels (top, wall) x {nh} sunlit hours (every {stride}th of "
+    rays_rate = 160000 * 2592 / (t1 - t0)
+    sample = (f"oracle FP32 hour loop on 2 x 50000 voxels (top, wall) x {nh} sunlit hours (every {stride}th of "
               f"{hours_total}), all-top solar per hour, extrapolated linearly to {N} voxels; "
-              f"DDA on 3000 voxels x 2592 rays; {t_top + t_rng + (t1 - t0):.1f} s CPU")
+              f"DDA on 160000 voxels x 2592 rays; {t_top + t_rng + (t1 - t0):.1f} s wall on {threads} threads")
     return {"value": vh_rate, "unit": "voxel-hours/s", "cores": threads, "kind": "port", "sample": sample,
             "rays_per_s": rays_rate}
 
