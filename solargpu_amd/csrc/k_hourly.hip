@@ -138,6 +138,27 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned 
     return full * band * kXcds + x * r + (s - full * band);
 }
 
+/* Lane-permuted copy of x (CDNA DPP); lanes the control does not write, or whose source lies
+   outside their row, read 0. */
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
+__device__ __forceinline__ unsigned dpp0(unsigned x) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, BANK_MASK, true);
+}
+
+/* Inclusive prefix sum (mod 2^32) over the 64 lanes: row shifts 1, 2, 3 of the input, then
+   4 and 8 of the partial sums build the 16-lane row prefix; row_bcast15 / row_bcast31 carry
+   the row totals into the rows above. */
+__device__ __forceinline__ unsigned wave_scan_add(unsigned x) {
+    unsigned s = x + dpp0<0x111>(x);        // row_shr:1
+    s += dpp0<0x112>(x);                    // row_shr:2
+    s += dpp0<0x113>(x);                    // row_shr:3
+    s += dpp0<0x114, 0xf, 0xe>(s);          // row_shr:4, lanes 4..15 of each row
+    s += dpp0<0x118, 0xf, 0xc>(s);          // row_shr:8, lanes 8..15 of each row
+    s += dpp0<0x142, 0xa, 0xf>(s);          // row_bcast:15 into rows 1 and 3
+    s += dpp0<0x143, 0xc, 0xf>(s);          // row_bcast:31 into rows 2 and 3
+    return s;
+}
+
 /* Number of set bits of m below this lane. */
 __device__ __forceinline__ int lanes_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -269,16 +290,9 @@ __device__ __forceinline__ void wave_decode(StreamWindow& win, int64_t s0, int64
                 }
                 if (first && e == 0) delta = 0u - delta;   // first value is absolute
             }
-            /* inclusive prefix sum of deltas (mod 2^32) across the wave */
-            unsigned sc = delta;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned y = __shfl_up(sc, o, 64);
-                if (lane >= o) sc += y;
-            }
-            const unsigned val = prev - sc;
+            const unsigned val = prev - wave_scan_add(delta);   // deltas are summed mod 2^32
             f(e < total && val > 0u, (int)val - 1);   // valid: a non-zero value, id = value - 1
-            prev = __shfl(val, 63, 64);   // lanes past the end add zero deltas
+            prev = (unsigned)__builtin_amdgcn_readlane((int)val, 63);   // lanes past the end add zero
             first = false;
         }
         pos += pay;
@@ -298,6 +312,7 @@ __device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
     __shared__ int list_id[WAVES][kListCap];
     __shared__ float list_w[WAVES][kListCap];
     __shared__ __align__(16) uint8_t stage[WAVES][kCap];
+    __shared__ int pend_id[WAVES][FMT == SGPU_FMT_BARLE ? 128 : 1];   // decoded ids not yet weighed
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t n = a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total);
@@ -394,7 +409,30 @@ __device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
                 take(valid, valid ? a.nb_ids[k] - 1 : -1);
             }
         } else {
-            wave_decode(win, s0, s1, lane, take);
+            /* a ba-RLE block holds ~30 ids: decoded ids wait in pend_id so that the weights are
+               computed on full 64-id chunks */
+            int pend = 0;
+            auto gather = [&](bool valid, int id) __attribute__((always_inline)) {
+                const uint64_t m = __ballot(valid);
+                if (valid) pend_id[wave][pend + lanes_below(m)] = id;
+                pend += __popcll(m);
+                if (pend >= 64) {
+                    wave_lds_sync();
+                    const int x = pend_id[wave][lane];
+                    const bool more = 64 + lane < pend;
+                    const int y = more ? pend_id[wave][64 + lane] : 0;
+                    wave_lds_sync();
+                    if (more) pend_id[wave][lane] = y;
+                    pend -= 64;
+                    take(true, x);
+                }
+            };
+            wave_decode(win, s0, s1, lane, gather);
+            if (pend > 0) {
+                wave_lds_sync();
+                const bool v = lane < pend;
+                take(v, v ? pend_id[wave][lane] : -1);
+            }
         }
         if (cnt > 0) {
             wave_lds_sync();
