@@ -422,9 +422,15 @@ void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t
     int64_t* scr = nullptr;
     int64_t* boff = nullptr;
     uint8_t* bytes = nullptr;
+    int32_t* long_ids = nullptr;
+    unsigned* long_count = nullptr;
     try {
         scr = dalloc<int64_t>(scan_scratch_elems(N));
         boff = dalloc<int64_t>(N + 1);
+        long_ids = dalloc<int32_t>(N);
+        long_count = dalloc<unsigned>(1);
+        ba.long_ids = long_ids;
+        ba.long_count = long_count;
         ba.sizes = sizes;
         launch_barle(ba, false, stream);
         launch_scan_counts(sizes, N, boff, scr, stream);
@@ -442,10 +448,14 @@ void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t
         dfree(scr);
         dfree(boff);
         dfree(bytes);
+        dfree(long_ids);
+        dfree(long_count);
         throw;
     }
     dfree(sizes);
     dfree(scr);
+    dfree(long_ids);
+    dfree(long_count);
     *br_off = boff;
     *br_bytes = bytes;
 }

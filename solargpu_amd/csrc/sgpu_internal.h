@@ -152,6 +152,8 @@ struct BarleArgs {
     int32_t* sizes;         // size pass output
     const int64_t* out_off; // write pass input
     uint8_t* out;           // write pass output
+    int32_t* long_ids;      // queue of voxels with more than 512 ids (n entries; filled by the size pass)
+    unsigned* long_count;   // its length (zeroed by the size pass)
 };
 void launch_barle(const BarleArgs& a, bool write_pass, hipStream_t s);
 
