@@ -252,6 +252,9 @@ struct ListsOut {
     int64_t* off = nullptr;
     int32_t* ids = nullptr;
     int64_t nnz = 0;
+    int64_t* br_off = nullptr;   // barle_only builds
+    uint8_t* br_bytes = nullptr;
+    int64_t br_size = 0;
 };
 
 void encode_barle(sgpu_ctx* c, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,
@@ -305,16 +308,21 @@ void host_take(sgpu_ctx* c, bool csr, bool barle, int64_t v0, int64_t cnt, const
 }
 
 /* host_csr / host_barle: send every chunk's lists to pinned host memory instead of keeping
-   them on the device (then out.off / out.ids stay null). */
+   them on the device (then out.off / out.ids stay null).  barle_only: encode every chunk's
+   lists to ba-RLE at once and keep only the streams (out.br_*), so the whole CSR never
+   sits on the device (scenes whose CSR exceeds HBM). */
 ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf, unsigned long long* steps,
-                     bool host_csr = false, bool host_barle = false) {
+                     bool host_csr = false, bool host_barle = false, bool barle_only = false) {
     const bool to_host = host_csr || host_barle;
     const int64_t chunk = choose_chunk(c, N);
     c->info.chunk_voxels = chunk;
     c->info.viewshed_chunks = (int32_t)((N + chunk - 1) / chunk);
     EventPair ev_trace, ev_lists;
     ListsOut out;
-    if (!to_host) out.off = dalloc<int64_t>(N + 1);
+    std::vector<std::pair<uint8_t*, int64_t>> bpieces;
+    int64_t bbase = 0;
+    if (!to_host && !barle_only) out.off = dalloc<int64_t>(N + 1);
+    if (barle_only) out.br_off = dalloc<int64_t>(N + 1);
     int32_t* raw = nullptr;
     int32_t* counts = nullptr;
     int64_t* off = nullptr;
@@ -344,10 +352,23 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
             pieces.push_back({ids, piece});
             la.ids = ids;
             launch_lists(la, true, c->stream);
-            if (!to_host) {
+            if (!to_host && !barle_only) {
                 const unsigned g = (unsigned)((cnt + 255) / 256);
                 hipLaunchKernelGGL(k_add_base, dim3(g), dim3(256), 0, c->stream, off, cnt, base, out.off + v0);
                 SGPU_HIP(hipGetLastError());
+            }
+            if (barle_only) {
+                int64_t* boff = nullptr;
+                uint8_t* bytes = nullptr;
+                int64_t size = 0;
+                encode_barle_on(c->stream, off, ids, cnt, &boff, &bytes, &size);
+                bpieces.push_back({bytes, size});
+                const unsigned g = (unsigned)((cnt + 255) / 256);
+                hipLaunchKernelGGL(k_add_base, dim3(g), dim3(256), 0, c->stream, boff, cnt, bbase, out.br_off + v0);
+                SGPU_HIP(hipGetLastError());
+                SGPU_HIP(hipStreamSynchronize(c->stream));
+                dfree(boff);
+                bbase += size;
             }
             SGPU_HIP(hipEventRecord(ev_lists.b, c->stream));
             SGPU_HIP(hipStreamSynchronize(c->stream));
@@ -355,10 +376,39 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
             c->info.t_lists_ms += ev_lists.ms();
             if (to_host) {
                 host_take(c, host_csr, host_barle, v0, cnt, off, ids, piece);
+            }
+            if (to_host || barle_only) {
                 dfree(pieces.back().first);
                 pieces.pop_back();
             }
             base += piece;
+        }
+        if (barle_only) {
+            out.nnz = base;
+            dfree(raw);
+            dfree(counts);
+            dfree(off);
+            dfree(scratch);
+            SGPU_HIP(hipMemcpyAsync(out.br_off + N, &bbase, sizeof bbase, hipMemcpyHostToDevice, c->stream));
+            if (bpieces.size() == 1) {
+                out.br_bytes = bpieces[0].first;
+                bpieces.clear();
+            } else {
+                out.br_bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(bbase, 1));
+                int64_t pos = 0;
+                for (auto& p : bpieces) {
+                    if (p.second)
+                        SGPU_HIP(hipMemcpyAsync(out.br_bytes + pos, p.first, (size_t)p.second, hipMemcpyDeviceToDevice,
+                                                c->stream));
+                    pos += p.second;
+                    SGPU_HIP(hipStreamSynchronize(c->stream));
+                    dfree(p.first);
+                }
+                bpieces.clear();
+            }
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            out.br_size = bbase;
+            return out;
         }
         if (to_host) {
             out.nnz = base;
@@ -388,12 +438,15 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
         SGPU_HIP(hipStreamSynchronize(c->stream));
     } catch (...) {
         for (auto& p : pieces) dfree(p.first);
+        for (auto& p : bpieces) dfree(p.first);
         dfree(raw);
         dfree(counts);
         dfree(off);
         dfree(scratch);
         dfree(out.off);
         dfree(out.ids);
+        dfree(out.br_off);
+        dfree(out.br_bytes);
         throw;
     }
     dfree(raw);
@@ -477,9 +530,10 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
     if (steps) SGPU_HIP(hipMemsetAsync(steps, 0, sizeof(unsigned long long), c->stream));
     ListsOut L;
     const bool to_host = c->residency == SGPU_LISTS_HOST;
+    const bool barle_only = !to_host && formats == SGPU_FMT_BARLE;
     try {
         L = build_lists(c, nullptr, N, c->svf, steps, to_host && (formats & SGPU_FMT_CSR),
-                        to_host && (formats & SGPU_FMT_BARLE));
+                        to_host && (formats & SGPU_FMT_BARLE), barle_only);
         if (steps) {
             unsigned long long st = 0;
             SGPU_HIP(hipMemcpyAsync(&st, steps, sizeof st, hipMemcpyDeviceToHost, c->stream));
@@ -498,6 +552,17 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
         c->nnz = (formats & SGPU_FMT_CSR) ? L.nnz : -1;
         c->br_size = (formats & SGPU_FMT_BARLE) ? c->host.total_barle : -1;
         c->info.barle_bytes = c->host.total_barle;
+        SGPU_HIP(hipEventRecord(all.b, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->info.t_viewshed_ms = all.ms();
+        return;
+    }
+    if (barle_only) {
+        c->br_off = L.br_off;
+        c->br_bytes = L.br_bytes;
+        c->br_size = L.br_size;
+        c->nnz = -1;   // no CSR kept
+        c->info.barle_bytes = L.br_size;
         SGPU_HIP(hipEventRecord(all.b, c->stream));
         SGPU_HIP(hipStreamSynchronize(c->stream));
         c->info.t_viewshed_ms = all.ms();
@@ -615,18 +680,18 @@ HourArgs stage2_args(sgpu_ctx* c, unsigned fmt) { return hour_args(c, fmt); }
 SubsetLists subset_viewshed(sgpu_ctx* c, const int32_t* subset, int64_t n, bool barle) {
     SubsetLists s;
     if (n <= 0) return s;
+    if (barle) {   // ba-RLE only, encoded chunk by chunk
+        ListsOut L = build_lists(c, subset, n, nullptr, nullptr, false, false, true);
+        s.nnz = L.nnz;
+        s.br_off = L.br_off;
+        s.br_bytes = L.br_bytes;
+        s.br_size = L.br_size;
+        return s;
+    }
     ListsOut L = build_lists(c, subset, n, nullptr, nullptr);
     s.csr_off = L.off;
     s.csr_ids = L.ids;
     s.nnz = L.nnz;
-    if (barle) {
-        try {
-            encode_barle(c, s.csr_off, s.csr_ids, n, &s.br_off, &s.br_bytes, &s.br_size);
-        } catch (...) {
-            s.release();
-            throw;
-        }
-    }
     return s;
 }
 

@@ -210,8 +210,8 @@ HourArgs stage2_args(sgpu_ctx* c, unsigned fmt);
 void ensure_accumulators(sgpu_ctx* c, int K);
 void ensure_hours_d(sgpu_ctx* c, const std::vector<HourConst>& hs);
 void zero_floats(float* p, int64_t n, hipStream_t s);
-/* Viewshed of a voxel subset (ids on the device) into compact CSR lists and, if
-   barle, ba-RLE streams, indexed by subset position. */
+/* Viewshed of a voxel subset (ids on the device) into compact CSR lists or, with barle,
+   into ba-RLE streams only (encoded chunk by chunk), indexed by subset position. */
 struct SubsetLists {
     int64_t* csr_off = nullptr;
     int32_t* csr_ids = nullptr;

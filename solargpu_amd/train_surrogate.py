@@ -43,9 +43,10 @@ def train_cell(seed: int) -> float:
     return 0.5 if np.random.default_rng(seed + 17).random() < 0.5 else 1.0
 
 
-def make_tile(seed: int, size: int, cell: float = 1.0, lists: bool = True):
-    """Scene + context of one seeded tile; lists=True also runs the plain viewshed."""
-    density, lat, lon, tz = tile_site(seed)
+def make_tile(seed: int, size: int, cell: float = 1.0, lists: bool = True, site=None):
+    """Scene + context of one seeded tile; lists=True also runs the plain viewshed.
+    site = (density, lat, lon, tz) overrides the seeded draw."""
+    density, lat, lon, tz = site if site is not None else tile_site(seed)
     trees = int(round(40 * (size * cell / 256.0) ** 2))
     heights = sg.synth_dsm(seed, size, size, cell, density, trees)
     sc = sg.assemble_scene(heights, cell)

@@ -236,3 +236,26 @@ def test_long_barle_streams_match_cpu_decoder(waves):
         ctx.hour_loop(sg.FMT_CSR, 0)
         a_csr = ctx.irradiance()
     assert _rel(a_barle, a_csr).max() <= 1e-5
+
+
+def test_barle_only_build_encodes_chunk_by_chunk():
+    """A ba-RLE-only viewshed encodes every chunk's lists at once and keeps no CSR: the
+    streams still equal the oracle encoder's byte for byte, the annual sums equal those of the
+    combined CSR + ba-RLE build, and asking for the CSR lists is an error."""
+    sc = synth_scene(13, 37, 53)
+    ref = oracle_lists(13, 37, 53)
+    eoff, edata = orc.barle_encode_all(ref["off"], ref["ids"])
+    with _ctx(sc) as ctx:
+        ctx.hour_loop(sg.FMT_BARLE, 16)
+        combined = ctx.irradiance()
+    with _ctx(sc, formats=0, chunk=300) as ctx:
+        ctx.viewshed(sg.FMT_BARLE)
+        assert ctx.info()["viewshed_chunks"] > 1 and ctx.info()["nnz"] == ref["ids"].size
+        boff, data = ctx.barle()
+        np.testing.assert_array_equal(boff, eoff)
+        np.testing.assert_array_equal(data, edata)
+        np.testing.assert_array_equal(ctx.svf(), ref["svf"])
+        ctx.hour_loop(sg.FMT_BARLE, 16)
+        np.testing.assert_array_equal(ctx.irradiance(), combined)
+        with pytest.raises(sg.SolarGPUError):
+            ctx.csr()
