@@ -197,7 +197,7 @@ int sgpu_get_tuning(sgpu_ctx* ctx, sgpu_tuning* out);
 int sgpu_copy_bandwidth(sgpu_ctx* ctx, int64_t bytes, int reps, double* gbps);
 
 /* ---- per-voxel surrogate MLP (new; no reference counterpart) ----
-   64 inputs (49 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and
+   64 inputs (53 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and
    activations, FP32 accumulation on MFMA; Adam.  Parameters as one fp32 vector: per layer
    W[out][in] then b[out]. */
 typedef struct sgpu_mlp sgpu_mlp;

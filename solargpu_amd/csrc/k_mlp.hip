@@ -2,7 +2,7 @@
  * accumulation, hand-written MFMA GEMMs for forward, backward and the weight
  * gradients, and an Adam kernel.
  *
- * Network: 64 (49 features + zero padding) -> 256 -> 256 -> 256 -> 256 -> 2,
+ * Network: 64 (53 features + zero padding) -> 256 -> 256 -> 256 -> 256 -> 2,
  * exact-erf GELU on the hidden layers.  Layout: every matrix is row-major with
  * the reduction (K) dimension contiguous, so an MFMA operand fragment is one
  * 16-byte load.  Weights are kept as W[out][in] (forward) and Wt[in][out]

@@ -4,7 +4,7 @@
  *   k_annual_dd  exact annual direct + diffuse per voxel (FP32 Kahan over all
  *                sunlit hours) and the diffuse part alone -- the solar kernel
  *                of solar_kernel.cu:54-77 without the reflection term;
- *   k_features   32-sector horizon march + the 49 features of the MLP, bf16;
+ *   k_features   32-sector horizon march + the 53 features of the MLP, bf16;
  *   k_gate       surrogate estimate, predicted error, fallback flags;
  *   k_combine    scatter of the exact fallback totals. */
 #include <cmath>
@@ -124,7 +124,9 @@ __global__ __launch_bounds__(256) void k_features(SurArgs a) {
     uint16_t* row = a.X + v * kSurInPad;
     float svf_h = 0.0f;
     float quad[4] = {0.f, 0.f, 0.f, 0.f};
+    float lamb = 0.0f;   // Lambertian estimate from the horizon cells (weights of the reflection pass)
     const float aspect = a.aspects[v];
+    const float nx = ss * sa, ny = -ss * ca, nz = cs;
     for (int s = 0; s < kSectors; ++s) {
         const float az = (s + 0.5f) * (6.28318531f / kSectors);
         float sx, cy;
@@ -142,6 +144,15 @@ __global__ __launch_bounds__(256) void k_features(SurArgs a) {
             rel -= 6.28318531f * floorf(rel * 0.15915494f);
             const int q = (int)((rel + 0.78539816f) * 0.63661977f) & 3;
             quad[q] += a.dd[cell] * se;
+        }
+        if (cell >= 0) {   /* 0.2 cos_recv cos_send / (pi d^2), dx dy in cells, dz in metres */
+            const float dx = (float)(cell % a.W - gx), dy = (float)(cell / a.W - gy), dz = a.grid[cell] - z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 >= 1e-4f) {
+                const float inv_d = rsqrtf(d2);
+                const float cr = (nx * dx + ny * dy + nz * dz) * inv_d;
+                if (cr > 0.0f) lamb += 0.0636619772f * cr * fmaxf(fabsf(dz) * inv_d, 0.1f) * inv_d * inv_d * a.dd[cell];
+            }
         }
     }
     const bool wall = v >= a.n_top;
@@ -161,6 +172,29 @@ __global__ __launch_bounds__(256) void k_features(SurArgs a) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) row[44 + q] = f2bf(quad[q] * (1e-6f / 8.0f));
     row[48] = f2bf(a.cell_m);   // horizontal distances of the ray geometry are in cells
+    row[49] = f2bf(lamb * 1e-5f);
+    /* the ground in front of a wall (one cell along its outward normal) and the height above it */
+    float front_dd = 0.0f, front_h = 0.0f;
+    if (wall) {
+        const int fx = min(max(gx + (int)rintf(sa), 0), a.W - 1), fy = min(max(gy - (int)rintf(ca), 0), a.H - 1);
+        const int fc = fx + fy * a.W;
+        front_dd = a.dd[fc];
+        front_h = z - a.grid[fc];
+    }
+    row[50] = f2bf(front_dd * 1e-6f);
+    row[51] = f2bf(front_h * 0.02f);
+    /* mean annual direct + diffuse of the 8 cells around the voxel's cell */
+    float ring = 0.0f;
+    int nring = 0;
+    for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox) {
+            const int cx = gx + ox, cy = gy + oy;
+            if ((ox || oy) && cx >= 0 && cy >= 0 && cx < a.W && cy < a.H) {
+                ring += a.dd[cx + cy * a.W];
+                ++nring;
+            }
+        }
+    row[52] = f2bf(nring ? ring / (float)nring * 1e-6f : 0.0f);
 #pragma unroll
     for (int f = kSurFeatures; f < kSurInPad; ++f) row[f] = 0;
 }

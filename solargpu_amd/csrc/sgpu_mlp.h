@@ -7,7 +7,7 @@
 
 namespace sgpu {
 
-constexpr int kFeatures = 49;      // used inputs
+constexpr int kFeatures = 53;      // used inputs
 constexpr int kInPad = 64;         // padded input width (multiple of the MFMA K = 32)
 constexpr int kHidden = 256;
 constexpr int kLayers = 5;         // 4 hidden + output

@@ -8,7 +8,7 @@
 namespace sgpu {
 
 constexpr int kSectors = 32;
-constexpr int kSurFeatures = 49;
+constexpr int kSurFeatures = 53;
 constexpr int kSurInPad = 64;
 
 struct SurArgs {

@@ -1,7 +1,7 @@
 """Per-voxel surrogate for the reflection pass: the MLP binding (sgpu_mlp_* of
 include/sgpu.h) and the surrogate pipeline helpers.
 
-The network (49 features padded to 64 -> 4 x 256 GELU -> 2, BF16 weights/activations, FP32
+The network (53 features padded to 64 -> 4 x 256 GELU -> 2, BF16 weights/activations, FP32
 accumulation on MFMA) is trained by the library's own forward, backward and
 Adam kernels on labels produced by the plain pipeline.
 """
@@ -15,7 +15,7 @@ from . import _check, _lib, _ptr, _c, _I64
 
 FALLBACK_ANNUAL = -1   # run(): exact fallback by one walk over the neighbours' annual totals
 IN_PAD = 64
-FEATURES = 49
+FEATURES = 53
 HIDDEN = 256
 OUTPUTS = 2
 

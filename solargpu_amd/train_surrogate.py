@@ -115,15 +115,15 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--tiles", type=int, default=200)
     ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--sample", type=int, default=65536, help="voxels per training tile")
-    ap.add_argument("--calib-tiles", type=int, default=8)
-    ap.add_argument("--calib-size", type=int, default=0, help="calibration tile side (0 = --size)")
+    ap.add_argument("--sample", type=int, default=98304, help="voxels per training tile")
+    ap.add_argument("--calib-tiles", type=int, default=10)
+    ap.add_argument("--calib-size", type=int, default=1024, help="calibration tile side (0 = --size)")
     ap.add_argument("--calibrate-only", metavar="WEIGHTS", default=None,
                     help="keep the network of WEIGHTS, fit only the gate, write --out")
-    ap.add_argument("--epochs", type=int, default=24, help="split in three phases at lr, lr/3, lr/10")
+    ap.add_argument("--epochs", type=int, default=32, help="split in three phases at lr, lr/3, lr/10")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--target-p95", type=float, default=0.017)
+    ap.add_argument("--target-p95", type=float, default=0.016)
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights", "surrogate.npz"))
     args = ap.parse_args(argv)
     rng = np.random.default_rng(1)
