@@ -1,15 +1,19 @@
 /* GPU ba-RLE encoder (format of the reference's barle_compression.h:7-44 and
- * encoder barle_compression.cpp:12-207), one wave per voxel:
- *   - the voxel's CSR ids are staged in LDS and bitonic-sorted descending by
- *     the whole wave (padded with zeros to a power of two);
+ * encoder barle_compression.cpp:12-207), one wave per voxel, every step
+ * wave-parallel:
+ *   - sort: lists of <= 512 ids are bitonic-sorted descending in registers
+ *     (E = 1..8 ids per lane, cross-lane stages by xor shuffles, stages of
+ *     stride >= 64 inside each lane); longer lists in LDS;
  *   - duplicates are dropped with a wave-ordered compaction (ballot);
  *   - delta width classes and run starts (width changes) in parallel;
- *   - lane 0 runs the short per-run part: the single-element absorption
- *     passes to a fixpoint and the coalescing (optimize_runs); the size pass
- *     totals the stream from the runs, the write pass splits them at 255 in
- *     place;
- *   - the write pass walks the blocks in order (scalar byte position) and all
- *     lanes write each block's header (3 code bytes, length bytes) and payloads.
+ *   - optimize_runs: each absorption sweep is solved with 64-bit ballot
+ *     arithmetic (in a maximal chain of candidate pairs the greedy left-to-right
+ *     sweep merges every other pair from the chain's start), repeated to the
+ *     fixpoint, then equal-width neighbours coalesce by a boundary compaction;
+ *   - pass 1 sums the stream size over the runs; pass 2 splits the runs at 255
+ *     (prefix of piece counts), places every split by a prefix of payload
+ *     bytes (block b of 8 splits starts at 11 b + payload bytes before it), and
+ *     writes headers per split and payload bytes per delta.
  * Pass 1 writes per-voxel byte counts; pass 2 writes the bytes at scanned
  * offsets, so the stream is byte-identical to the reference encoder's.
  * Lists of <= 512 ids (most) take a small-LDS instantiation over all voxels;
@@ -37,8 +41,88 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
     return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-/* Descending bitonic sort of keys[0, P) (P a power of two) by one wave. */
-__device__ __forceinline__ void wave_bitonic_desc(unsigned* keys, int P, int lane) {
+constexpr unsigned long long kEven = 0x5555555555555555ull;
+constexpr unsigned long long kOdd = 0xAAAAAAAAAAAAAAAAull;
+
+__device__ __forceinline__ unsigned readlane(unsigned x, int l) {
+    return (unsigned)__builtin_amdgcn_readlane((int)x, l);
+}
+
+/* Inclusive wave64 prefix sum. */
+__device__ __forceinline__ int wave_incl_sum(int x, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+/* Descending bitonic sort of the wave's 64 E ids, element i = r * 64 + lane in x[r]. */
+template <int E, int LOGP>
+__device__ __forceinline__ void reg_bitonic_desc(unsigned (&x)[E], int lane) {
+#pragma unroll
+    for (int lk = 1; lk <= LOGP; ++lk) {
+        const int k = 1 << lk;
+#pragma unroll
+        for (int lj = lk - 1; lj >= 0; --lj) {
+            const int j = 1 << lj;
+            if (j >= 64) {   // partner in the same lane
+                const int jr = j >> 6;
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    if (r & jr) continue;
+                    const bool desc = ((r * 64) & k) == 0;
+                    const unsigned p = x[r], q = x[r | jr];
+                    x[r] = desc ? max(p, q) : min(p, q);
+                    x[r | jr] = desc ? min(p, q) : max(p, q);
+                }
+            } else {         // partner lane ^ j; the lower index keeps the max in a descending block
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    const unsigned y = (unsigned)__shfl_xor((int)x[r], j, 64);
+                    const bool desc = ((r * 64 + lane) & k) == 0;
+                    x[r] = (lower == desc) ? max(x[r], y) : min(x[r], y);
+                }
+            }
+        }
+    }
+}
+
+/* Sorted-descending unique ids of a list of n <= 64 E ids into vals; returns their count. */
+template <int E, int LOGP>
+__device__ __forceinline__ int sort_unique_regs(const int32_t* ids, int n, unsigned* vals, int lane) {
+    unsigned x[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int i = r * 64 + lane;
+        x[r] = (i < n) ? (unsigned)ids[i] : 0u;   // ids >= 1; padding zeros sort last
+    }
+    reg_bitonic_desc<E, LOGP>(x, lane);
+    int m = 0;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const unsigned up = (unsigned)__shfl_up((int)x[r], 1, 64);
+        const unsigned last = (r > 0) ? readlane(x[r > 0 ? r - 1 : 0], 63) : 0u;
+        const unsigned prev = (lane == 0) ? last : up;
+        const int i = r * 64 + lane;
+        const bool keep = (i < n) && (i == 0 || x[r] != prev);
+        const unsigned long long mask = __ballot(keep);
+        if (keep) vals[m + lanes_below(mask)] = x[r];
+        m += __popcll(mask);
+    }
+    return m;
+}
+
+/* Descending bitonic sort of keys[0, P) (P a power of two) in LDS by one wave. */
+__device__ __forceinline__ void lds_bitonic_desc(unsigned* keys, int P, int lane) {
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = lane; i < P; i += 64) {
@@ -57,46 +141,72 @@ __device__ __forceinline__ void wave_bitonic_desc(unsigned* keys, int P, int lan
     }
 }
 
-/* optimize_runs (barle_compression.cpp:112-152) in place: a single-element run one width
-   class below its left neighbour joins it (skipping past the pair) until nothing merges or one
-   run is left; then equal-width neighbours coalesce.  Returns the new run count. */
-__device__ int optimize_runs(unsigned* runs, int nr) {
+/* optimize_runs (barle_compression.cpp:112-152) on runs[0, nr) covering m deltas, in place,
+   wave-parallel.  A sweep merges run i with a single-element run i + 1 one width class below
+   it, skipping past the pair; over 64 runs the candidate mask C (with position 0 blocked when
+   the previous chunk's last run merged) merges at every other bit of each maximal chain of
+   ones, counted from the chain's start.  Sweeps repeat until nothing merges or one run is
+   left; then equal-width neighbours coalesce.  Returns the new run count. */
+__device__ int optimize_runs_wave(unsigned* runs, int nr, int m, int lane) {
     if (nr < 2) return nr;
     for (;;) {
-        bool merged = false;
-        int o = 0, i = 0;
-        while (i + 1 < nr) {
-            const unsigned r0 = runs[i], r1 = runs[i + 1];
-            if (rlen(r1) == 1u && rw(r1) + 1u == rw(r0)) {
-                runs[o++] = mkrun(rw(r0), rstart(r0), rlen(r0) + 1u);
-                merged = true;
-                i += 2;
-            } else {
-                runs[o++] = r0;
-                i += 1;
-            }
+        int o = 0;
+        unsigned long long carry = 0, any = 0;
+        for (int base = 0; base < nr; base += 64) {
+            const int i = base + lane;
+            const unsigned r0 = (i < nr) ? runs[i] : 0u;
+            const unsigned r1 = (i + 1 < nr) ? runs[i + 1] : 0u;
+            const bool cand = (i + 1 < nr) && rlen(r1) == 1u && rw(r1) + 1u == rw(r0);
+            const unsigned long long C = __ballot(cand) & ~carry;
+            const unsigned long long S = C & ~(C << 1);                 // chain starts
+            const unsigned long long even_chain = C & ~(C + (S & kEven)); // chains starting at even bits
+            const unsigned long long M = (even_chain & kEven) | (C & ~even_chain & kOdd);
+            const unsigned long long emit = __ballot(i < nr) & ~((M << 1) | carry);
+            __syncthreads();   // the chunk's reads before its writes (outputs land at <= i)
+            if ((emit >> lane) & 1ull)
+                runs[o + lanes_below(emit)] = ((M >> lane) & 1ull) ? mkrun(rw(r0), rstart(r0), rlen(r0) + 1u) : r0;
+            o += __popcll(emit);
+            carry = M >> 63;
+            any |= M;
+            __syncthreads();
         }
-        if (i < nr) runs[o++] = runs[i];
-        if (o == 1) return 1;
-        if (!merged) break;
         nr = o;
+        if (nr == 1) return 1;
+        if (!any) break;
     }
+    /* coalesce: keep the first run of every equal-width group, then lengths from the starts */
     int o = 0;
-    for (int i = 0; i < nr; ++i) {
-        if (o > 0 && rw(runs[o - 1]) == rw(runs[i]))
-            runs[o - 1] = mkrun(rw(runs[o - 1]), rstart(runs[o - 1]), rlen(runs[o - 1]) + rlen(runs[i]));
-        else
-            runs[o++] = runs[i];
+    unsigned prev_w = 0xFFFFFFFFu;
+    for (int base = 0; base < nr; base += 64) {
+        const int i = base + lane;
+        const unsigned r0 = (i < nr) ? runs[i] : 0u;
+        const unsigned up = (unsigned)__shfl_up((int)rw(r0), 1, 64);
+        const unsigned pw = (lane == 0) ? prev_w : up;
+        const bool first = (i < nr) && rw(r0) != pw;
+        const unsigned long long B = __ballot(first);
+        prev_w = readlane(rw(r0), 63);
+        __syncthreads();
+        if (first) runs[o + lanes_below(B)] = r0;
+        o += __popcll(B);
+        __syncthreads();
+    }
+    for (int base = 0; base < o; base += 64) {
+        const int j = base + lane;
+        const unsigned r0 = (j < o) ? runs[j] : 0u;
+        const unsigned end = (j + 1 < o) ? rstart(runs[j + 1]) : (unsigned)m;
+        __syncthreads();
+        if (j < o) runs[j] = mkrun(rw(r0), rstart(r0), end - rstart(r0));
+        __syncthreads();
     }
     return o;
 }
 
 template <int MAXP>
 struct EncLds {
-    unsigned keys[MAXP];          // ids, then run starts, runs and (write pass) splits
+    unsigned keys[MAXP];          // LDS sort buffer (long lists), runs, then payload positions
     unsigned vals[MAXP];          // sorted-descending unique ids
-    unsigned char wcl[MAXP];      // width class of each delta
-    int ns;
+    unsigned spl[MAXP];           // splits (write pass)
+    unsigned char wcl[MAXP];      // width class of each delta; 0x80 marks a split start (write pass)
 };
 
 /* Payload bytes of a split of len deltas of width class w. */
@@ -111,19 +221,27 @@ __device__ __forceinline__ unsigned delta_at(const unsigned* vals, unsigned i) {
 /* The ba-RLE stream of voxel v's list (n ids at csr_ids + b): pass 1 its size, pass 2 its bytes. */
 template <bool WRITE, int MAXP>
 __device__ void encode_voxel(const BarleArgs& a, EncLds<MAXP>& s, int64_t v, int64_t b, int n, int lane) {
-    int P = 64;
-    while (P < n) P <<= 1;
-    for (int i = lane; i < P; i += 64) s.keys[i] = (i < n) ? (unsigned)a.csr_ids[b + i] : 0u;   // ids >= 1
-    __syncthreads();
-    wave_bitonic_desc(s.keys, P, lane);
-    /* first of each equal group, in order (padding zeros sort last) */
-    int m = 0;
-    for (int base = 0; base < n; base += 64) {
-        const int i = base + lane;
-        const bool keep = (i < n) && (i == 0 || s.keys[i] != s.keys[i - 1]);
-        const unsigned long long mask = __ballot(keep);
-        if (keep) s.vals[m + lanes_below(mask)] = s.keys[i];
-        m += __popcll(mask);
+    int m;
+    if constexpr (MAXP <= 512) {
+        const int32_t* ids = a.csr_ids + b;
+        if (n <= 64) m = sort_unique_regs<1, 6>(ids, n, s.vals, lane);
+        else if (n <= 128) m = sort_unique_regs<2, 7>(ids, n, s.vals, lane);
+        else if (n <= 256) m = sort_unique_regs<4, 8>(ids, n, s.vals, lane);
+        else m = sort_unique_regs<8, 9>(ids, n, s.vals, lane);
+    } else {
+        int P = 64;
+        while (P < n) P <<= 1;
+        for (int i = lane; i < P; i += 64) s.keys[i] = (i < n) ? (unsigned)a.csr_ids[b + i] : 0u;   // ids >= 1
+        __syncthreads();
+        lds_bitonic_desc(s.keys, P, lane);
+        m = 0;   // first of each equal group, in order (padding zeros sort last)
+        for (int base = 0; base < n; base += 64) {
+            const int i = base + lane;
+            const bool keep = (i < n) && (i == 0 || s.keys[i] != s.keys[i - 1]);
+            const unsigned long long mask = __ballot(keep);
+            if (keep) s.vals[m + lanes_below(mask)] = s.keys[i];
+            m += __popcll(mask);
+        }
     }
     __syncthreads();
     for (int i = lane; i < m; i += 64) s.wcl[i] = (unsigned char)width_class(delta_at(s.vals, i));
@@ -149,76 +267,90 @@ __device__ void encode_voxel(const BarleArgs& a, EncLds<MAXP>& s, int64_t v, int
         if (r < nr) s.keys[r] = mkrun(s.wcl[s0], s0, s1 - s0);
         __syncthreads();
     }
+    nr = optimize_runs_wave(s.keys, nr, m, lane);
     if (!WRITE) {
-        if (lane == 0) {
-            /* size: per split one length byte and its payload, per block of 8 splits 3 code
-               bytes; a run of len splits into ceil(len / 255) pieces of 255 and the rest */
-            nr = optimize_runs(s.keys, nr);
-            int ns = 0, size = 0;
-            for (int r = 0; r < nr; ++r) {
-                const unsigned w = rw(s.keys[r]), len = rlen(s.keys[r]);
-                const unsigned full = len / 255u, rest = len % 255u;
-                ns += (int)(full + (rest ? 1u : 0u));
-                size += (int)(full + (rest ? 1u : 0u)) + (int)full * payload_bytes(w, 255u) + payload_bytes(w, rest);
-            }
-            a.sizes[v] = size + 3 * ((ns + 7) / 8);
+        /* per split one length byte and its payload, per block of 8 splits 3 code bytes; a run
+           of len splits into ceil(len / 255) pieces of 255 and the rest */
+        int bytes = 0, pieces = 0;
+        for (int r = lane; r < nr; r += 64) {
+            const unsigned w = rw(s.keys[r]), len = rlen(s.keys[r]);
+            const unsigned full = len / 255u, rest = len % 255u;
+            const int pc = (int)(full + (rest ? 1u : 0u));
+            pieces += pc;
+            bytes += pc + (int)full * payload_bytes(w, 255u) + payload_bytes(w, rest);
         }
+        bytes = wave_sum(bytes);
+        pieces = wave_sum(pieces);
+        if (lane == 0) a.sizes[v] = bytes + 3 * ((pieces + 7) / 8);
         return;
     }
-    if (lane == 0) {
-        nr = optimize_runs(s.keys, nr);
-        /* split at 255 (barle_compression.cpp:29-44) in place, from the last run backwards
-           (run r's pieces land at or after index r) */
-        int ns = 0;
-        for (int r = 0; r < nr; ++r) ns += (int)((rlen(s.keys[r]) + 254u) / 255u);
-        int at = ns;
-        for (int r = nr - 1; r >= 0; --r) {
-            const unsigned w = rw(s.keys[r]), st = rstart(s.keys[r]), len = rlen(s.keys[r]);
-            const int pieces = (int)((len + 254u) / 255u);
-            at -= pieces;
-            for (int k = 0; k < pieces; ++k) {
-                const unsigned u = 255u * (unsigned)k;
-                s.keys[at + k] = mkrun(w, st + u, min(255u, len - u));
-            }
+    /* split at 255 (barle_compression.cpp:29-44): run r's pieces at the prefix of piece counts */
+    int ns = 0;
+    for (int base = 0; base < nr; base += 64) {
+        const int r = base + lane;
+        const unsigned run = (r < nr) ? s.keys[r] : 0u;
+        const unsigned w = rw(run), st = rstart(run), len = rlen(run);
+        const int pc = (int)((len + 254u) / 255u);
+        const int incl = wave_incl_sum(pc, lane);
+        for (int k = 0; k < pc; ++k) {
+            const unsigned u = 255u * (unsigned)k;
+            s.spl[ns + incl - pc + k] = mkrun(w, st + u, min(255u, len - u));
         }
-        s.ns = ns;
+        ns += (int)readlane((unsigned)incl, 63);
     }
     __syncthreads();
-    const int ns = s.ns;
+    /* headers and payload positions: block b (splits 8b .. 8b+7) starts at 11 b + the payload
+       bytes of the splits before it; its 3 code bytes, then cnt length bytes, then payloads */
     uint8_t* out = a.out + a.out_off[v];
-    int pos = 0;   // wave-uniform byte position
-    for (int t0 = 0; t0 < ns; t0 += 8) {
-        const int cnt = min(8, ns - t0);
-        if (lane < 3 + cnt) {   // block header: 3 code bytes (7 = unused slot), length bytes
-            uint8_t byte;
-            if (lane < 3) {
+    int pay = 0;   // payload bytes before this chunk of splits
+    for (int base = 0; base < ns; base += 64) {
+        const int t = base + lane;
+        const bool valid = t < ns;
+        const unsigned sp = valid ? s.spl[t] : 0u;
+        const unsigned w = rw(sp), len = rlen(sp);
+        const int pb = valid ? payload_bytes(w, len) : 0;
+        const int incl = wave_incl_sum(pb, lane);
+        const int before = pay + incl - pb;                  // payload bytes of splits < t
+        const int at_block = __shfl(before, lane & ~7, 64);  // ... < the block's first split
+        const int blk = t >> 3;
+        const int cnt = min(8, ns - (blk << 3));
+        if (valid) {
+            const int hdr = 11 * blk + at_block;
+            s.keys[t] = (unsigned)(11 * blk + 3 + cnt + before);
+            out[hdr + 3 + (t & 7)] = (uint8_t)len;
+            if ((t & 7) == 0) {
                 unsigned bits = 0;
-                for (int j = 0; j < 8; ++j) bits |= ((j < cnt ? rw(s.keys[t0 + j]) : 7u) & 7u) << ((7 - j) * 3);
-                byte = (uint8_t)(bits >> (16 - 8 * lane));
-            } else {
-                byte = (uint8_t)rlen(s.keys[t0 + lane - 3]);
+                for (int j = 0; j < 8; ++j) bits |= ((j < cnt ? rw(s.spl[t + j]) : 7u) & 7u) << ((7 - j) * 3);
+                out[hdr] = (uint8_t)(bits >> 16);
+                out[hdr + 1] = (uint8_t)(bits >> 8);
+                out[hdr + 2] = (uint8_t)bits;
             }
-            out[pos + lane] = byte;
+            s.wcl[rstart(sp)] = 0x80;
         }
-        pos += 3 + cnt;
-        for (int t = t0; t < t0 + cnt; ++t) {   // payloads, the lanes over one split's bytes
-            const unsigned sp = s.keys[t], w = rw(sp), st = rstart(sp), len = rlen(sp);
-            uint8_t* dst = out + pos;
+        pay += (int)readlane((unsigned)incl, 63);
+    }
+    __syncthreads();
+    /* payloads, one lane per delta: its split is the count of split starts up to it, minus one */
+    int starts = 0;
+    for (int base = 0; base < m; base += 64) {
+        const int i = base + lane;
+        const bool valid = i < m;
+        const unsigned long long F = __ballot(valid && (s.wcl[i] & 0x80));
+        if (valid) {
+            const int t = starts + lanes_below(F) + (int)((F >> lane) & 1ull) - 1;
+            const unsigned sp = s.spl[t], w = rw(sp), k = (unsigned)i - rstart(sp), len = rlen(sp);
+            uint8_t* dst = out + s.keys[t];
+            const unsigned x = delta_at(s.vals, (unsigned)i);
             if (w == 0u) {   // two 4-bit deltas per byte, high nibble first, odd tail low = 0
-                for (unsigned k = lane; 2u * k < len; k += 64u) {
-                    const unsigned i0 = st + 2u * k;
-                    const unsigned hi = delta_at(s.vals, i0);
-                    const unsigned lo = (2u * k + 1u < len) ? delta_at(s.vals, i0 + 1u) : 0u;
-                    dst[k] = (uint8_t)((hi << 4) | (lo & 0x0Fu));
+                if ((k & 1u) == 0u) {
+                    const unsigned lo = (k + 1u < len) ? delta_at(s.vals, (unsigned)i + 1u) : 0u;
+                    dst[k >> 1] = (uint8_t)((x << 4) | (lo & 0x0Fu));
                 }
             } else {         // w bytes big-endian per delta
-                for (unsigned k = lane; k < len; k += 64u) {
-                    const unsigned x = delta_at(s.vals, st + k);
-                    for (unsigned q = 0; q < w; ++q) dst[k * w + q] = (uint8_t)(x >> (8u * (w - 1u - q)));
-                }
+                for (unsigned q = 0; q < w; ++q) dst[k * w + q] = (uint8_t)(x >> (8u * (w - 1u - q)));
             }
-            pos += payload_bytes(w, len);
         }
+        starts += __popcll(F);
     }
 }
 
