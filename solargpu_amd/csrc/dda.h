@@ -11,15 +11,15 @@ namespace sgpu {
 /* Steps whose cells and heights are fetched before any of them is tested: the
  * traversal does not depend on the heights, so kDdaUnroll loads are in flight at
  * once; the tests then run in step order, so the first exit wins as before. */
-constexpr int kDdaUnroll = 4;
+constexpr int kDdaUnroll = 2;
 
-/* First hit (cell id + 1) of the ray from integer cell (gx, gy) at height
- * base_z in direction dc, or 0.  `steps` counts DDA steps when COUNT. */
-template <bool COUNT>
-__device__ __forceinline__ int dda_first_hit(const float* __restrict__ grid, int W, unsigned uW, unsigned uH,
-                                             float maxz, float gxf, float gyf, float base_z, const DirConst& dc,
-                                             unsigned long long& steps) {
-    const bool upper = dc.info & 1;
+/* The walk for one ray, specialised on the direction's hemisphere (UPPER: the ray
+ * leaves through maxz, else through z = 0) and on whether a 32-bit byte offset
+ * addresses the whole grid (OFF32).  Exit tests are selects, not branches. */
+template <bool COUNT, bool UPPER, bool OFF32>
+__device__ __forceinline__ int dda_walk(const float* __restrict__ grid, int W, unsigned uW, unsigned uH, float maxz,
+                                        float gxf, float gyf, float base_z, const DirConst& dc,
+                                        unsigned long long& steps) {
     const float ox = gxf + dc.cx * 2.0f;
     const float oy = gyf + dc.cy * 2.0f;
     const float oz = base_z + dc.cz * 2.0f;
@@ -43,17 +43,38 @@ __device__ __forceinline__ int dda_first_hit(const float* __restrict__ grid, int
             tmy = xs ? tmy : tmy + dc.ry;
             inside[u] = (unsigned)ix < uW && (unsigned)iy < uH;
             cell[u] = inside[u] ? ix + iy * W : 0;   // off-grid steps read a valid cell, unused
-            h[u] = grid[cell[u]];
+            if (OFF32) h[u] = *(const float*)((const char*)grid + ((unsigned)cell[u] << 2));
+            else h[u] = grid[cell[u]];
         }
+        int res = 0;
+        bool done = false;
 #pragma unroll
         for (int u = 0; u < kDdaUnroll; ++u) {
-            if (COUNT) ++steps;
-            if (!inside[u]) return 0;
+            if (COUNT) steps += done ? 0 : 1;
             const float zr = oz + dc.cz * t[u];
-            if (upper ? (zr > maxz) : (zr < 0.0f)) return 0;
-            if (h[u] - zr > 0.01f) return cell[u] + 1;
+            const bool zout = UPPER ? (zr > maxz) : (zr < 0.0f);
+            const bool stop = !inside[u] | zout | (h[u] - zr > 0.01f);
+            res = (stop & !done) ? ((inside[u] & !zout) ? cell[u] + 1 : 0) : res;
+            done |= stop;
         }
+        if (done) return res;
     }
+}
+
+/* First hit (cell id + 1) of the ray from integer cell (gx, gy) at height
+ * base_z in direction dc, or 0.  `steps` counts DDA steps when COUNT.  The
+ * direction is the same for every lane of a wave, so the dispatch is uniform. */
+template <bool COUNT>
+__device__ __forceinline__ int dda_first_hit(const float* __restrict__ grid, int W, unsigned uW, unsigned uH,
+                                             float maxz, float gxf, float gyf, float base_z, const DirConst& dc,
+                                             unsigned long long& steps) {
+    const bool off32 = (unsigned long long)uW * uH <= (1ull << 30);
+    if (dc.info & 1) {
+        return off32 ? dda_walk<COUNT, true, true>(grid, W, uW, uH, maxz, gxf, gyf, base_z, dc, steps)
+                     : dda_walk<COUNT, true, false>(grid, W, uW, uH, maxz, gxf, gyf, base_z, dc, steps);
+    }
+    return off32 ? dda_walk<COUNT, false, true>(grid, W, uW, uH, maxz, gxf, gyf, base_z, dc, steps)
+                 : dda_walk<COUNT, false, false>(grid, W, uW, uH, maxz, gxf, gyf, base_z, dc, steps);
 }
 
 }  // namespace sgpu

@@ -1,7 +1,7 @@
 /* Stage 2 on gfx950: the sunlit-hour loop.
  *
  * A pass covers K consecutive sunlit hours (K = 1 is the reference's one hour
- * per launch).  Per pass:
+ * per launch; K <= 128).  Per pass:
  *   k_solar_top  one lane per top cell: direct (SVF-gated) + Perez diffuse for
  *                the K hours -> hourly_top[cell][K] (cell-major: one cell's K
  *                hours are one contiguous row).
@@ -65,23 +65,21 @@ __device__ __forceinline__ void solar_top_cell(const HourArgs& a, int64_t c) {
     float ss, cs, sa, ca;
     sincosf(slope, &ss, &cs);
     sincosf(aspect, &sa, &ca);
-    float out[K];
-#pragma unroll
-    for (int h = 0; h < K; ++h) {
-        out[h] = 0.0f;
-        if (h < a.nh) {
-            const HourConst& hc = a.hours[h];
-            out[h] = hour_irradiance(hc, cs, ss, ca, sa, svf_blocked(a.svf, c, hc.dense, a.n_total));
-        }
-    }
     float* dst = a.hourly_top + c * K;
-    if constexpr (K % 4 == 0) {
+    constexpr int V = (K % 4 == 0) ? 4 : 1;   // hours per store
+    for (int h0 = 0; h0 < K; h0 += V) {
+        float out[V];
 #pragma unroll
-        for (int h = 0; h < K; h += 4)
-            *reinterpret_cast<float4*>(dst + h) = make_float4(out[h], out[h + 1], out[h + 2], out[h + 3]);
-    } else {
-#pragma unroll
-        for (int h = 0; h < K; ++h) dst[h] = out[h];
+        for (int u = 0; u < V; ++u) {
+            const int h = h0 + u;
+            out[u] = 0.0f;
+            if (h < a.nh) {
+                const HourConst& hc = a.hours[h];
+                out[u] = hour_irradiance(hc, cs, ss, ca, sa, svf_blocked(a.svf, c, hc.dense, a.n_total));
+            }
+        }
+        if constexpr (V == 4) *reinterpret_cast<float4*>(dst + h0) = make_float4(out[0], out[1], out[2], out[3]);
+        else dst[h0] = out[0];
     }
 }
 
@@ -170,22 +168,46 @@ __device__ __forceinline__ int64_t uniform64(int64_t x) {
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-/* Lane (g, h) = (lane / K, lane % K) adds w[i] * top[id[i]][h] over the list entries
- * i = g, g + 64/K, ...: each load instruction reads 64/K whole K-hour rows. */
+/* K <= 64: lane (g, h) = (lane / K, lane % K) adds w[i] * top[id[i]][h] over the list
+ * entries i = g, g + 64/K, ...: each load instruction reads 64/K whole K-hour rows.
+ * K = 128: lane l holds hours 2l and 2l + 1 of every entry (one 8-byte load per lane, one
+ * 512-byte row per instruction).  The lane's partial sums over its hours are returned. */
 template <int K>
 __device__ __forceinline__ float apply_list(const int* ids, const float* ws, int cnt, const float* __restrict__ top,
                                             int lane) {
-    constexpr int G = 64 / K;
-    const float* row = top + (lane & (K - 1));
-    float acc0 = 0.0f, acc1 = 0.0f;
-    int i = lane / K;
+    if constexpr (K > 64) {
+        static_assert(K == 128, "two hours per lane");
+        const float* row = top + 2 * lane;
+        float a0 = 0.0f, a1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
+        int i = 0;
 #pragma unroll 4
-    for (; i + G < cnt; i += 2 * G) {
-        acc0 += ws[i] * row[(size_t)(unsigned)ids[i] * K];
-        acc1 += ws[i + G] * row[(size_t)(unsigned)ids[i + G] * K];
+        for (; i + 1 < cnt; i += 2) {
+            const float2 x = *reinterpret_cast<const float2*>(row + (size_t)(unsigned)ids[i] * K);
+            const float2 y = *reinterpret_cast<const float2*>(row + (size_t)(unsigned)ids[i + 1] * K);
+            a0 = fmaf(ws[i], x.x, a0);
+            a1 = fmaf(ws[i], x.y, a1);
+            b0 = fmaf(ws[i + 1], y.x, b0);
+            b1 = fmaf(ws[i + 1], y.y, b1);
+        }
+        if (i < cnt) {
+            const float2 x = *reinterpret_cast<const float2*>(row + (size_t)(unsigned)ids[i] * K);
+            a0 = fmaf(ws[i], x.x, a0);
+            a1 = fmaf(ws[i], x.y, a1);
+        }
+        return (a0 + b0) + (a1 + b1);
+    } else {
+        constexpr int G = 64 / K;
+        const float* row = top + (lane & (K - 1));
+        float acc0 = 0.0f, acc1 = 0.0f;
+        int i = lane / K;
+#pragma unroll 4
+        for (; i + G < cnt; i += 2 * G) {
+            acc0 = fmaf(ws[i], row[(size_t)(unsigned)ids[i] * K], acc0);
+            acc1 = fmaf(ws[i + G], row[(size_t)(unsigned)ids[i + G] * K], acc1);
+        }
+        if (i < cnt) acc0 = fmaf(ws[i], row[(size_t)(unsigned)ids[i] * K], acc0);
+        return acc0 + acc1;
     }
-    if (i < cnt) acc0 += ws[i] * row[(size_t)(unsigned)ids[i] * K];
-    return acc0 + acc1;
 }
 
 /* ba-RLE bytes staged in LDS per workgroup, split evenly over its waves. */
@@ -554,11 +576,20 @@ void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
 }  // namespace
 
 bool supported_hours_per_pass(int K) {
-    return K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64;
+    return K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64 || K == 128;
+}
+
+/* The hourly irradiance rows of a pass (n_top x K floats) must fit the buffer they are written to
+   (k_solar_top) and gathered from (k_reflect). */
+void check_hourly_cap(const HourArgs& a, int K) {
+    if (a.n_top * (int64_t)K > a.hourly_top_cap)
+        fail(SGPU_ERR_STATE, "hourly irradiance buffer holds %lld floats, the pass needs %lld",
+             (long long)a.hourly_top_cap, (long long)(a.n_top * (int64_t)K));
 }
 
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
     if (a.n_top <= 0) return;
+    check_hourly_cap(a, K);
     switch (K) {
         case 1: solar_top_k<1>(a, s); break;
         case 2: solar_top_k<2>(a, s); break;
@@ -567,6 +598,7 @@ void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
         case 16: solar_top_k<16>(a, s); break;
         case 32: solar_top_k<32>(a, s); break;
         case 64: solar_top_k<64>(a, s); break;
+        case 128: solar_top_k<128>(a, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());
@@ -575,6 +607,7 @@ void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
 void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s) {
     if ((a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total)) <= 0) return;
     if (fmt != SGPU_FMT_CSR && fmt != SGPU_FMT_BARLE) fail(SGPU_ERR_ARG, "format must be CSR or BARLE");
+    check_hourly_cap(a, K);
     if (fmt == SGPU_FMT_BARLE && (reinterpret_cast<uintptr_t>(a.nb_bytes) & 15u) != 0)
         fail(SGPU_ERR_ARG, "ba-RLE stream must be 16-byte aligned");
     if (a.dbg_hourly != nullptr) {
@@ -591,6 +624,7 @@ void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s)
         case 16: reflect_k<16>(a, fmt, s); break;
         case 32: reflect_k<32>(a, fmt, s); break;
         case 64: reflect_k<64>(a, fmt, s); break;
+        case 128: reflect_k<128>(a, fmt, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());
@@ -609,6 +643,7 @@ void launch_hour_pass_batch(const BatchHour* d, int ntiles, int64_t top_blocks, 
         case 16: batch_k<16>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
         case 32: batch_k<32>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
         case 64: batch_k<64>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 128: batch_k<128>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());

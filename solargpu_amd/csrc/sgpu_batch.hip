@@ -276,7 +276,7 @@ void viewshed(sgpu_batch* b) {
 void hour_loop(sgpu_batch* b, unsigned fmt, int K, int waves, bool graph) {
     need(b->lists, SGPU_ERR_STATE, "run sgpu_batch_viewshed first");
     need(fmt == SGPU_FMT_CSR || fmt == SGPU_FMT_BARLE, SGPU_ERR_ARG, "format must be CSR or BARLE");
-    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32 or 64");
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64 or 128");
     need(waves == 2 || waves == 4, SGPU_ERR_ARG, "waves must be 2 or 4");
     hipStream_t s = b->stream;
     const int nt = (int)b->tiles.size();
@@ -307,6 +307,8 @@ void hour_loop(sgpu_batch* b, unsigned fmt, int K, int waves, bool graph) {
         a.svf = t.svf;
         a.hours = t.hours_d;
         a.hourly_top = b->hourly_top + t.top_base * K;
+        a.hourly_top_cap = (int64_t)b->hourly_top_elems - t.top_base * K;
+        need((int64_t)t.n_top * K <= a.hourly_top_cap, SGPU_ERR_STATE, "batch hourly buffer too small");
         if (fmt == SGPU_FMT_CSR) {
             a.nb_off = b->csr_off + t.vbase;
             a.nb_ids = b->csr_ids;

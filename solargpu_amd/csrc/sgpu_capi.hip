@@ -105,6 +105,7 @@ HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
     a.aspects = c->aspects;
     a.svf = c->svf;
     a.hourly_top = c->hourly_top;
+    a.hourly_top_cap = (int64_t)c->hourly_top_elems;
     a.sum = c->sum;
     a.comp = c->comp;
     a.waves = c->tune.waves_per_group;
@@ -598,7 +599,7 @@ void do_barle_encode(sgpu_ctx* c) {
 void do_hour_loop(sgpu_ctx* c, unsigned fmt, int K, int stride) {
     check_stage2_ready(c, fmt);
     if (K == 0) K = autotuned_hours_per_pass(c, fmt);
-    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32 or 64");
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64 or 128");
     need(stride >= 1, SGPU_ERR_ARG, "hour_stride must be >= 1");
     ensure_accumulators(c, K);
     std::vector<HourConst> hs = select_hours(c, stride);
@@ -632,7 +633,7 @@ void do_hour_loop_graph(sgpu_ctx* c, unsigned fmt, int K, int replays) {
     check_stage2_ready(c, fmt);
     need(!lists_on_host(c, fmt), SGPU_ERR_STATE, "graph capture needs device-resident lists");
     if (K == 0) K = autotuned_hours_per_pass(c, fmt);
-    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32 or 64");
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64 or 128");
     need(replays >= 1, SGPU_ERR_ARG, "replays must be >= 1");
     ensure_accumulators(c, K);
     ensure_hours_d(c, c->hours);
@@ -751,6 +752,7 @@ void subset_reflected_linear(sgpu_ctx* c, const int32_t* subset, int64_t n, cons
         a.nb_len = L.br_size;
     }
     a.hourly_top = const_cast<float*>(dd);   // K = 1: the top voxels' annual totals
+    a.hourly_top_cap = c->n_top;
     a.hours = c->hours_d;
     a.nh = 1;
     float* own = dalloc<float>(n);
@@ -774,6 +776,7 @@ void annual_reflected_linear(sgpu_ctx* c, unsigned fmt, const float* dd, float* 
     ensure_hours_d(c, c->hours);
     HourArgs a = hour_args(c, fmt);
     a.hourly_top = const_cast<float*>(dd);   // cell-major with K = 1: the top voxels' annual totals
+    a.hourly_top_cap = c->n_top;
     a.hours = c->hours_d;
     a.nh = 1;
     float* own = dalloc<float>(c->n_total);

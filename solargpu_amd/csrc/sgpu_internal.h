@@ -19,7 +19,7 @@ constexpr int kElBins = 18;            // 90 / 5 deg   (sky_bins.cpp:14)
 constexpr int kDirs = kAzBins * kElBins * 2;  // 2592 ray directions (viewshed_stage.cuh:46)
 constexpr float kBinDeg = 5.0f;
 constexpr float kDegToRad = 0.0174532925f;    // pipeline_math_constants.h:15
-constexpr int kMaxHoursPerPass = 64;
+constexpr int kMaxHoursPerPass = 128;
 
 /* ---- errors: thrown inside the library, turned into status codes at the C ABI ---- */
 struct Failure : std::runtime_error {
@@ -169,6 +169,7 @@ struct HourArgs {
     const HourConst* hours;    // this pass's hours (nh entries)
     int nh;
     float* hourly_top;         // n_top x K, cell-major
+    int64_t hourly_top_cap;    // floats available at hourly_top (the launchers check n_top x K)
     const int64_t* nb_off;     // CSR offsets or ba-RLE byte offsets
     const int32_t* nb_ids;
     const uint8_t* nb_bytes;

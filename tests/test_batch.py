@@ -30,7 +30,7 @@ def _tiles():
     return out
 
 
-@pytest.mark.parametrize("fmt,K,waves", [(sg.FMT_BARLE, 64, 4), (sg.FMT_CSR, 16, 2)])
+@pytest.mark.parametrize("fmt,K,waves", [(sg.FMT_BARLE, 64, 4), (sg.FMT_CSR, 16, 2), (sg.FMT_BARLE, 128, 2)])
 def test_batch_matches_single_tile_contexts(fmt, K, waves):
     tiles = _tiles()
     with sg.Batch(0) as b:
