@@ -152,7 +152,7 @@ int sgpu_set_barle(sgpu_ctx* ctx, const int64_t* offsets, const uint8_t* bytes);
 int sgpu_barle_encode(sgpu_ctx* ctx);
 
 /* Stage 2: the annual loop over sunlit hours with format SGPU_FMT_CSR or SGPU_FMT_BARLE.
-   hours_per_pass (1, 2, 4, ..., 128) hours share one pass over the neighbour stream; 0 = autotuned.
+   hours_per_pass (1, 2, 4, ..., 256) hours share one pass over the neighbour stream; 0 = autotuned.
    hour_stride > 1 computes every hour_stride-th sunlit hour only (bounded samples).
    Annual sums are FP32 Kahan-compensated. */
 int sgpu_hour_loop(sgpu_ctx* ctx, unsigned format, int hours_per_pass, int hour_stride);
@@ -178,7 +178,7 @@ int sgpu_sync(sgpu_ctx* ctx);
    are tuned only when the lists of `format` exist.  Every shape gives identical results. */
 typedef struct sgpu_tuning {
     int32_t dirs_per_group;      /* 16, 32 or 48 */
-    int32_t hours_per_pass;      /* 1, 2, 4, ..., 128 */
+    int32_t hours_per_pass;      /* 1, 2, 4, ..., 256 */
     int32_t waves_per_group;     /* 2 or 4 */
     int32_t xcd_band;            /* reflection workgroups per XCD band: 0 (one contiguous eighth
                                     of the voxels per XCD) or 16..4096 (XCDs side by side) */

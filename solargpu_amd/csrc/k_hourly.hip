@@ -1,7 +1,7 @@
 /* Stage 2 on gfx950: the sunlit-hour loop.
  *
  * A pass covers K consecutive sunlit hours (K = 1 is the reference's one hour
- * per launch; K <= 128).  Per pass:
+ * per launch; K <= 256).  Per pass:
  *   k_solar_top  one lane per top cell: direct (SVF-gated) + Perez diffuse for
  *                the K hours -> hourly_top[cell][K] (cell-major: one cell's K
  *                hours are one contiguous row).
@@ -170,31 +170,39 @@ __device__ __forceinline__ int64_t uniform64(int64_t x) {
 
 /* K <= 64: lane (g, h) = (lane / K, lane % K) adds w[i] * top[id[i]][h] over the list
  * entries i = g, g + 64/K, ...: each load instruction reads 64/K whole K-hour rows.
- * K = 128: lane l holds hours 2l and 2l + 1 of every entry (one 8-byte load per lane, one
- * 512-byte row per instruction).  The lane's partial sums over its hours are returned. */
+ * K = 128, 256: lane l holds hours HPL l .. HPL l + HPL - 1 of every entry (HPL = K / 64:
+ * one 8- or 16-byte load per lane, one whole row per instruction).  The lane's partial
+ * sums over its hours are returned. */
 template <int K>
 __device__ __forceinline__ float apply_list(const int* ids, const float* ws, int cnt, const float* __restrict__ top,
                                             int lane) {
     if constexpr (K > 64) {
-        static_assert(K == 128, "two hours per lane");
-        const float* row = top + 2 * lane;
-        float a0 = 0.0f, a1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
+        constexpr int HPL = K / 64;
+        static_assert(HPL == 2 || HPL == 4, "two or four hours per lane");
+        typedef float vec __attribute__((ext_vector_type(HPL)));
+        const float* row = top + HPL * lane;
+        vec a = {}, b = {};
         int i = 0;
-#pragma unroll 4
+#pragma unroll 2
         for (; i + 1 < cnt; i += 2) {
-            const float2 x = *reinterpret_cast<const float2*>(row + (size_t)(unsigned)ids[i] * K);
-            const float2 y = *reinterpret_cast<const float2*>(row + (size_t)(unsigned)ids[i + 1] * K);
-            a0 = fmaf(ws[i], x.x, a0);
-            a1 = fmaf(ws[i], x.y, a1);
-            b0 = fmaf(ws[i + 1], y.x, b0);
-            b1 = fmaf(ws[i + 1], y.y, b1);
+            const vec x = *reinterpret_cast<const vec*>(row + (size_t)(unsigned)ids[i] * K);
+            const vec y = *reinterpret_cast<const vec*>(row + (size_t)(unsigned)ids[i + 1] * K);
+            const float wx = ws[i], wy = ws[i + 1];
+#pragma unroll
+            for (int u = 0; u < HPL; ++u) {
+                a[u] = fmaf(wx, x[u], a[u]);
+                b[u] = fmaf(wy, y[u], b[u]);
+            }
         }
         if (i < cnt) {
-            const float2 x = *reinterpret_cast<const float2*>(row + (size_t)(unsigned)ids[i] * K);
-            a0 = fmaf(ws[i], x.x, a0);
-            a1 = fmaf(ws[i], x.y, a1);
+            const vec x = *reinterpret_cast<const vec*>(row + (size_t)(unsigned)ids[i] * K);
+#pragma unroll
+            for (int u = 0; u < HPL; ++u) a[u] = fmaf(ws[i], x[u], a[u]);
         }
-        return (a0 + b0) + (a1 + b1);
+        float s = 0.0f;
+#pragma unroll
+        for (int u = 0; u < HPL; ++u) s += a[u] + b[u];
+        return s;
     } else {
         constexpr int G = 64 / K;
         const float* row = top + (lane & (K - 1));
@@ -576,7 +584,7 @@ void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
 }  // namespace
 
 bool supported_hours_per_pass(int K) {
-    return K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64 || K == 128;
+    return K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64 || K == 128 || K == 256;
 }
 
 /* The hourly irradiance rows of a pass (n_top x K floats) must fit the buffer they are written to
@@ -599,6 +607,7 @@ void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
         case 32: solar_top_k<32>(a, s); break;
         case 64: solar_top_k<64>(a, s); break;
         case 128: solar_top_k<128>(a, s); break;
+        case 256: solar_top_k<256>(a, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());
@@ -625,6 +634,7 @@ void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s)
         case 32: reflect_k<32>(a, fmt, s); break;
         case 64: reflect_k<64>(a, fmt, s); break;
         case 128: reflect_k<128>(a, fmt, s); break;
+        case 256: reflect_k<256>(a, fmt, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());
@@ -644,6 +654,7 @@ void launch_hour_pass_batch(const BatchHour* d, int ntiles, int64_t top_blocks, 
         case 32: batch_k<32>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
         case 64: batch_k<64>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
         case 128: batch_k<128>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 256: batch_k<256>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());

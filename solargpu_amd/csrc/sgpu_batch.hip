@@ -276,7 +276,7 @@ void viewshed(sgpu_batch* b) {
 void hour_loop(sgpu_batch* b, unsigned fmt, int K, int waves, bool graph) {
     need(b->lists, SGPU_ERR_STATE, "run sgpu_batch_viewshed first");
     need(fmt == SGPU_FMT_CSR || fmt == SGPU_FMT_BARLE, SGPU_ERR_ARG, "format must be CSR or BARLE");
-    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64 or 128");
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64, 128 or 256");
     need(waves == 2 || waves == 4, SGPU_ERR_ARG, "waves must be 2 or 4");
     hipStream_t s = b->stream;
     const int nt = (int)b->tiles.size();

@@ -599,7 +599,7 @@ void do_barle_encode(sgpu_ctx* c) {
 void do_hour_loop(sgpu_ctx* c, unsigned fmt, int K, int stride) {
     check_stage2_ready(c, fmt);
     if (K == 0) K = autotuned_hours_per_pass(c, fmt);
-    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64 or 128");
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64, 128 or 256");
     need(stride >= 1, SGPU_ERR_ARG, "hour_stride must be >= 1");
     ensure_accumulators(c, K);
     std::vector<HourConst> hs = select_hours(c, stride);
@@ -633,7 +633,7 @@ void do_hour_loop_graph(sgpu_ctx* c, unsigned fmt, int K, int replays) {
     check_stage2_ready(c, fmt);
     need(!lists_on_host(c, fmt), SGPU_ERR_STATE, "graph capture needs device-resident lists");
     if (K == 0) K = autotuned_hours_per_pass(c, fmt);
-    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64 or 128");
+    need(supported_hours_per_pass(K), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64, 128 or 256");
     need(replays >= 1, SGPU_ERR_ARG, "replays must be >= 1");
     ensure_accumulators(c, K);
     ensure_hours_d(c, c->hours);

@@ -19,7 +19,7 @@ constexpr int kElBins = 18;            // 90 / 5 deg   (sky_bins.cpp:14)
 constexpr int kDirs = kAzBins * kElBins * 2;  // 2592 ray directions (viewshed_stage.cuh:46)
 constexpr float kBinDeg = 5.0f;
 constexpr float kDegToRad = 0.0174532925f;    // pipeline_math_constants.h:15
-constexpr int kMaxHoursPerPass = 128;
+constexpr int kMaxHoursPerPass = 256;
 
 /* ---- errors: thrown inside the library, turned into status codes at the C ABI ---- */
 struct Failure : std::runtime_error {

@@ -3,7 +3,7 @@
  *
  *   DDA tile       directions per 256-voxel workgroup, G in {16, 32, 48}:
  *                  timed on a 2^17-voxel slice from the middle of the scene;
- *   stage-2 pass   hours per pass K in {32, 64, 128} x waves per reflection
+ *   stage-2 pass   hours per pass K in {64, 128, 256} x waves per reflection
  *                  workgroup in {2, 4} (LDS footprint and occupancy) x XCD
  *                  band in {0, 64, 256} (contiguous eighths or side-by-side
  *                  bands), then receiver groups in {0, 8, 32, 128} x band in
@@ -86,6 +86,7 @@ double time_trace(sgpu_ctx* c, int G, int64_t v0, int64_t n, int32_t* raw) {
 }
 
 double time_passes(sgpu_ctx* c, unsigned fmt, int K, int waves, int band, int group, int passes) {
+    ensure_accumulators(c, K);   // the hourly buffer grows to this trial's n_top x K
     HourArgs a = stage2_args(c, fmt);
     a.waves = waves;
     a.band = band;
@@ -140,7 +141,6 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
     t.rays_per_s = (double)n * kDirs / (best * 1e-3);
     /* stage-2 pass shape */
     if (lists && c->svf_ok && !c->hours.empty()) {
-        ensure_accumulators(c, kMaxHoursPerPass);   // room for every K tried
         ensure_hours_d(c, c->hours);
         double best_rate = 0.0;
         const int passes = 3;
@@ -156,7 +156,7 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
                 t.group_blocks = group;
             }
         };
-        for (int K : {32, 64, 128})
+        for (int K : {64, 128, 256})
             for (int waves : {2, 4})
                 for (int band : {0, 64, 256}) trial(K, waves, band, 0);
         const int K = t.hours_per_pass, waves = t.waves_per_group;
@@ -197,7 +197,7 @@ int sgpu_set_tuning(sgpu_ctx* c, const sgpu_tuning* t) {
         need(c && t, SGPU_ERR_ARG, "null argument");
         need(t->dirs_per_group == 16 || t->dirs_per_group == 32 || t->dirs_per_group == 48, SGPU_ERR_ARG,
              "dirs_per_group must be 16, 32 or 48");
-        need(supported_hours_per_pass(t->hours_per_pass), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64 or 128");
+        need(supported_hours_per_pass(t->hours_per_pass), SGPU_ERR_ARG, "hours_per_pass must be 1, 2, 4, 8, 16, 32, 64, 128 or 256");
         need(t->waves_per_group == 2 || t->waves_per_group == 4, SGPU_ERR_ARG, "waves_per_group must be 2 or 4");
         need(t->xcd_band == 0 || (t->xcd_band >= 16 && t->xcd_band <= 4096), SGPU_ERR_ARG,
              "xcd_band must be 0 or 16..4096");

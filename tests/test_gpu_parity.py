@@ -120,7 +120,7 @@ def test_single_hour_matches_oracle(fmt):
 
 @pytest.mark.parametrize("fmt,K", [(sg.FMT_CSR, 1), (sg.FMT_CSR, 16), (sg.FMT_CSR, 64), (sg.FMT_CSR, 128),
                                    (sg.FMT_BARLE, 1), (sg.FMT_BARLE, 8), (sg.FMT_BARLE, 32), (sg.FMT_BARLE, 64),
-                                   (sg.FMT_BARLE, 128)])
+                                   (sg.FMT_BARLE, 128), (sg.FMT_BARLE, 256)])
 def test_annual_within_1e4_of_fp64(fmt, K):
     sc = synth_scene(12, 64, 64)
     ref = oracle_lists(12, 64, 64)
