@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-csr", action="store_true", help="skip the CSR hour-loop report")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help="G,K,waves,band,group: use this launch shape instead of autotuning (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer drop-in timing")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -179,7 +181,12 @@ def main():
         return a, b
 
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        G, Kt, wv, band, group = (int(x) for x in args.tuning.split(","))
+        ctx.set_tuning(G, Kt, wv, band, group)
+        tuning = ctx.tuning()
+        log(f"tuning: {tuning}")
+    elif not args.no_autotune:
         ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
         tuning = ctx.autotune(fmt)
         log(f"autotune: {tuning}")

@@ -172,50 +172,56 @@ __device__ __forceinline__ int64_t uniform64(int64_t x) {
  * entries i = g, g + 64/K, ...: each load instruction reads 64/K whole K-hour rows.
  * K = 128, 256: lane l holds hours HPL l .. HPL l + HPL - 1 of every entry (HPL = K / 64:
  * one 8- or 16-byte load per lane, one whole row per instruction).  The lane's partial
- * sums over its hours are returned. */
+ * sums over its hours are returned.  OFF32 (the hourly buffer is below 4 GiB): rows are
+ * addressed by a 32-bit byte offset from the buffer base, one shift-add per entry. */
+template <int K, bool OFF32>
+__device__ __forceinline__ float apply_list_at(const int* ids, const float* ws, int cnt,
+                                               const float* __restrict__ top, int lane) {
+    constexpr int HPL = (K > 64) ? K / 64 : 1;
+    constexpr int G = (K > 64) ? 1 : 64 / K;
+    typedef float vec __attribute__((ext_vector_type(HPL)));
+    /* 16 VGPRs of rows in flight per wave: 16, 8 or 4 loads (measured best for K = 64, 128, 256) */
+    constexpr int kUnroll = 8 / HPL;
+    /* byte offset of this lane's hours within a row, and of row `id` */
+    const unsigned lane_off = (K > 64) ? (unsigned)(HPL * lane) * 4u : (unsigned)(lane & (K - 1)) * 4u;
+    auto load = [&](int id) __attribute__((always_inline)) {
+        if constexpr (OFF32) {
+            const unsigned off = (unsigned)id * (unsigned)(K * 4) + lane_off;
+            return *reinterpret_cast<const vec*>(reinterpret_cast<const char*>(top) + off);
+        } else {
+            return *reinterpret_cast<const vec*>(reinterpret_cast<const char*>(top) +
+                                                 (size_t)(unsigned)id * (size_t)(K * 4) + lane_off);
+        }
+    };
+    vec a = {}, b = {};
+    int i = (K > 64) ? 0 : lane / K;
+#pragma unroll kUnroll
+    for (; i + G < cnt; i += 2 * G) {
+        const vec x = load(ids[i]);
+        const vec y = load(ids[i + G]);
+        const float wx = ws[i], wy = ws[i + G];
+#pragma unroll
+        for (int u = 0; u < HPL; ++u) {
+            a[u] = fmaf(wx, x[u], a[u]);
+            b[u] = fmaf(wy, y[u], b[u]);
+        }
+    }
+    if (i < cnt) {
+        const vec x = load(ids[i]);
+#pragma unroll
+        for (int u = 0; u < HPL; ++u) a[u] = fmaf(ws[i], x[u], a[u]);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int u = 0; u < HPL; ++u) s += a[u] + b[u];
+    return s;
+}
+
 template <int K>
 __device__ __forceinline__ float apply_list(const int* ids, const float* ws, int cnt, const float* __restrict__ top,
-                                            int lane) {
-    if constexpr (K > 64) {
-        constexpr int HPL = K / 64;
-        static_assert(HPL == 2 || HPL == 4, "two or four hours per lane");
-        typedef float vec __attribute__((ext_vector_type(HPL)));
-        const float* row = top + HPL * lane;
-        vec a = {}, b = {};
-        int i = 0;
-#pragma unroll 2
-        for (; i + 1 < cnt; i += 2) {
-            const vec x = *reinterpret_cast<const vec*>(row + (size_t)(unsigned)ids[i] * K);
-            const vec y = *reinterpret_cast<const vec*>(row + (size_t)(unsigned)ids[i + 1] * K);
-            const float wx = ws[i], wy = ws[i + 1];
-#pragma unroll
-            for (int u = 0; u < HPL; ++u) {
-                a[u] = fmaf(wx, x[u], a[u]);
-                b[u] = fmaf(wy, y[u], b[u]);
-            }
-        }
-        if (i < cnt) {
-            const vec x = *reinterpret_cast<const vec*>(row + (size_t)(unsigned)ids[i] * K);
-#pragma unroll
-            for (int u = 0; u < HPL; ++u) a[u] = fmaf(ws[i], x[u], a[u]);
-        }
-        float s = 0.0f;
-#pragma unroll
-        for (int u = 0; u < HPL; ++u) s += a[u] + b[u];
-        return s;
-    } else {
-        constexpr int G = 64 / K;
-        const float* row = top + (lane & (K - 1));
-        float acc0 = 0.0f, acc1 = 0.0f;
-        int i = lane / K;
-#pragma unroll 4
-        for (; i + G < cnt; i += 2 * G) {
-            acc0 = fmaf(ws[i], row[(size_t)(unsigned)ids[i] * K], acc0);
-            acc1 = fmaf(ws[i + G], row[(size_t)(unsigned)ids[i + G] * K], acc1);
-        }
-        if (i < cnt) acc0 = fmaf(ws[i], row[(size_t)(unsigned)ids[i] * K], acc0);
-        return acc0 + acc1;
-    }
+                                            int lane, bool off32) {
+    static_assert(K <= 64 || K == 128 || K == 256, "one, two or four hours per lane");
+    return off32 ? apply_list_at<K, true>(ids, ws, cnt, top, lane) : apply_list_at<K, false>(ids, ws, cnt, top, lane);
 }
 
 /* ba-RLE bytes staged in LDS per workgroup, split evenly over its waves. */
@@ -372,6 +378,7 @@ __device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
     }
     const float inv_w = 1.0f / (float)a.W;
     const int cells = a.W * a.H;
+    const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);   // below 4 GiB: 32-bit row offsets
 
     /* phase 1: lane = voxel */
     float own = 0.0f;
@@ -427,7 +434,7 @@ __device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
             cnt += __popcll(m);
             if (cnt > kListCap - 64) {
                 wave_lds_sync();
-                acc += apply_list<K>(list_id[wave], list_w[wave], cnt, a.hourly_top, lane);
+                acc += apply_list<K>(list_id[wave], list_w[wave], cnt, a.hourly_top, lane, off32);
                 wave_lds_sync();
                 cnt = 0;
             }
@@ -466,7 +473,7 @@ __device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
         }
         if (cnt > 0) {
             wave_lds_sync();
-            acc += apply_list<K>(list_id[wave], list_w[wave], cnt, a.hourly_top, lane);
+            acc += apply_list<K>(list_id[wave], list_w[wave], cnt, a.hourly_top, lane, off32);
             wave_lds_sync();
         }
 #pragma unroll
