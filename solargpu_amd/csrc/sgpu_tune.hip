@@ -6,7 +6,7 @@
  *   stage-2 pass   hours per pass K in {64, 128, 256} x waves per reflection
  *                  workgroup in {2, 4} (LDS footprint and occupancy) x XCD
  *                  band in {0, 64, 256} (contiguous eighths or side-by-side
- *                  bands), then receiver groups in {0, 8, 32, 128} x band in
+ *                  bands), then receiver groups in {0, 8, 32, 64, 128} x band in
  *                  {0, 256} for the best of those: three passes each on the
  *                  live lists, scratch accumulators.
  * Every shape computes identical results; only time differs.  The winners are
@@ -160,7 +160,7 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
             for (int waves : {2, 4})
                 for (int band : {0, 64, 256}) trial(K, waves, band, 0);
         const int K = t.hours_per_pass, waves = t.waves_per_group;
-        for (int group : {8, 32, 128})
+        for (int group : {8, 32, 64, 128})
             for (int band : {0, 256}) trial(K, waves, band, group);
         t.voxel_hours_per_s = best_rate;
         c->irr_ok = false;   // the scratch passes overwrote the annual accumulators
