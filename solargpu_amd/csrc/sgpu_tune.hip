@@ -9,7 +9,8 @@
  *                  bands), then receiver groups in {0, 8, 32, 64, 128} x band in
  *                  {0, 256} for the best of those: three passes each on the
  *                  live lists, scratch accumulators.
- * Every shape computes identical results; only time differs.  The winners are
+ * Waves, bands and receiver groups give bit-identical results; K only regroups
+ * the hours of the Kahan-summed annual totals (same sums to FP32 rounding).  The winners are
  * appended to a text cache keyed by device, format and scene size class. */
 #include <cmath>
 #include <cstdio>
