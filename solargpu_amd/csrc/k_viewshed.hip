@@ -10,8 +10,10 @@
  *           with the smallest direction index per cell (LDS atomics), then an
  *           ordered block scan keeps the first occurrence of every cell in
  *           direction order -- the reference's append-if-absent list
- *           (viewshed_kernels.cu:50-55) -- and ORs the SVF bits.  Pass 1
- *           counts, pass 2 writes at scanned offsets.
+ *           (viewshed_kernels.cu:50-55) -- and ORs the SVF bits.  It stores
+ *           the count and leaves the kept ids, in order, at the front of the
+ *           voxel's raw row; after the scan of the counts, k_list_copy moves
+ *           every row's list to its offset (one wave per voxel).
  *
  * The DDA keeps the reference's operation order (viewshed_kernels.cu:27-58);
  * the library is compiled with -ffp-contract=off and IEEE division. */
@@ -85,20 +87,17 @@ struct ListsLds {
     int tmp[kListThreads / 64];
 };
 
-/* One voxel's first-occurrence list from its raw row (workgroup-wide).  Pass 1 stores the
- * count and, when svf is given, the voxel's SVF bytes (svf[byte * n + v]); pass 2 writes
- * the ids at dst. */
-template <bool WRITE>
-__device__ __forceinline__ void voxel_list(ListsLds& s, const int32_t* raw, const DirConst* dirs, uint8_t* svf,
-                                           int svf_bytes, int64_t n, int64_t v, int32_t* count_out,
-                                           int32_t* dst_base) {
+/* One voxel's first-occurrence list from its raw row (workgroup-wide): the count, the
+ * voxel's SVF bytes when svf is given (svf[byte * n + v]), and the kept ids in direction
+ * order written over the front of the row (every read of the row precedes the barriers). */
+__device__ __forceinline__ void voxel_list(ListsLds& s, int32_t* raw, const DirConst* dirs, uint8_t* svf,
+                                           int svf_bytes, int64_t n, int64_t v, int32_t* count_out) {
     int* keys = s.keys;
     int* mins = s.mins;
     unsigned* svf_acc = s.svf_acc;
     const int tid = threadIdx.x;
     for (int i = tid; i < kHashSize; i += kListThreads) { keys[i] = 0; mins[i] = INT_MAX; }
-    if (!WRITE)
-        for (int i = tid; i < svf_bytes; i += kListThreads) svf_acc[i] = 0u;
+    for (int i = tid; i < svf_bytes; i += kListThreads) svf_acc[i] = 0u;
     __syncthreads();
     const int d0 = tid * kDirsPerThread;
     int hv[kDirsPerThread];
@@ -118,7 +117,7 @@ __device__ __forceinline__ void voxel_list(ListsLds& s, const int32_t* raw, cons
             }
             slot[k] = (int)sl;
             atomicMin(&mins[sl], d);
-            if (!WRITE && svf) {
+            if (svf) {
                 const int dense = (dirs[d].info >> 8) - 1;
                 if (dense >= 0) atomicOr(&svf_acc[dense >> 3], 1u << (dense & 7));
             }
@@ -130,20 +129,16 @@ __device__ __forceinline__ void voxel_list(ListsLds& s, const int32_t* raw, cons
     for (int k = 0; k < kDirsPerThread; ++k)
         if (hv[k] && mins[slot[k]] == d0 + k) ++kept;
     int total;
-    const int excl = block_excl_scan(kept, s.tmp, &total);
-    if (!WRITE) {
-        if (tid == 0) *count_out = total;
-        if (svf)
-            for (int i = tid; i < svf_bytes; i += kListThreads) svf[(int64_t)i * n + v] = (uint8_t)svf_acc[i];
-    } else {
-        int32_t* dst = dst_base + excl;
+    const int excl = block_excl_scan(kept, s.tmp, &total);   // barriers: the row is read
+    if (tid == 0) *count_out = total;
+    if (svf)
+        for (int i = tid; i < svf_bytes; i += kListThreads) svf[(int64_t)i * n + v] = (uint8_t)svf_acc[i];
+    int32_t* dst = raw + excl;
 #pragma unroll
-        for (int k = 0; k < kDirsPerThread; ++k)
-            if (hv[k] && mins[slot[k]] == d0 + k) *dst++ = hv[k];
-    }
+    for (int k = 0; k < kDirsPerThread; ++k)
+        if (hv[k] && mins[slot[k]] == d0 + k) *dst++ = hv[k];
 }
 
-template <bool WRITE>
 __global__ __launch_bounds__(kListThreads) void k_lists_batch(BatchListArgs a) {
     __shared__ ListsLds s;
     __shared__ int tile_s;
@@ -160,16 +155,32 @@ __global__ __launch_bounds__(kListThreads) void k_lists_batch(BatchListArgs a) {
     }
     __syncthreads();
     const TileDesc& t = a.tiles[tile_s];
-    voxel_list<WRITE>(s, a.raw + local * kDirs, t.dirs, t.svf, t.svf_bytes, t.n_total, g - t.vbase,
-                      a.counts + local, WRITE ? a.ids + a.offsets[local] : nullptr);
+    voxel_list(s, a.raw + local * kDirs, t.dirs, t.svf, t.svf_bytes, t.n_total, g - t.vbase, a.counts + local);
 }
 
-template <bool WRITE>
 __global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
     __shared__ ListsLds s;
     const int64_t local = blockIdx.x;
-    voxel_list<WRITE>(s, a.raw + local * kDirs, a.dirs, a.svf, a.svf_bytes, a.n_total, a.v0 + local,
-                      a.counts + local, WRITE ? a.ids + a.offsets[local] : nullptr);
+    voxel_list(s, a.raw + local * kDirs, a.dirs, a.svf, a.svf_bytes, a.n_total, a.v0 + local, a.counts + local);
+}
+
+/* Every voxel's kept ids (the front of its raw row) to its scanned offset; one wave per voxel. */
+constexpr int kCopyWaves = 4;
+__global__ __launch_bounds__(64 * kCopyWaves) void k_list_copy(const int32_t* raw, const int32_t* counts,
+                                                               const int64_t* offsets, int32_t* ids, int64_t count) {
+    const int64_t local = (int64_t)blockIdx.x * kCopyWaves + (threadIdx.x >> 6);
+    if (local >= count) return;
+    const int lane = threadIdx.x & 63;
+    const int n = counts[local];
+    const int32_t* src = raw + local * kDirs;
+    int32_t* dst = ids + offsets[local];
+    for (int i = lane; i < n; i += 64) dst[i] = src[i];
+}
+
+void launch_list_copy(const int32_t* raw, const int32_t* counts, const int64_t* offsets, int32_t* ids, int64_t count,
+                      hipStream_t s) {
+    const unsigned g = (unsigned)((count + kCopyWaves - 1) / kCopyWaves);
+    hipLaunchKernelGGL(k_list_copy, dim3(g), dim3(64 * kCopyWaves), 0, s, raw, counts, offsets, ids, count);
 }
 
 /* Persistent multi-tile DDA: every workgroup takes (256-voxel block, 32-direction group)
@@ -312,9 +323,8 @@ void launch_trace(const TraceArgs& a, hipStream_t s) {
 void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s) {
     if (a.count <= 0) return;
     if (a.svf_bytes > kAzBins * kElBins / 8 + 1) fail(SGPU_ERR_ARG, "svf_bytes %d out of range", a.svf_bytes);
-    dim3 grid((unsigned)a.count);
-    if (write_pass) hipLaunchKernelGGL(k_lists<true>, grid, dim3(kListThreads), 0, s, a);
-    else hipLaunchKernelGGL(k_lists<false>, grid, dim3(kListThreads), 0, s, a);
+    if (write_pass) launch_list_copy(a.raw, a.counts, a.offsets, a.ids, a.count, s);
+    else hipLaunchKernelGGL(k_lists, dim3((unsigned)a.count), dim3(kListThreads), 0, s, a);
     SGPU_HIP(hipGetLastError());
 }
 
@@ -326,8 +336,8 @@ void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_
 
 void launch_lists_batch(const BatchListArgs& a, bool write_pass, hipStream_t s) {
     if (a.count <= 0) return;
-    if (write_pass) hipLaunchKernelGGL(k_lists_batch<true>, dim3((unsigned)a.count), dim3(kListThreads), 0, s, a);
-    else hipLaunchKernelGGL(k_lists_batch<false>, dim3((unsigned)a.count), dim3(kListThreads), 0, s, a);
+    if (write_pass) launch_list_copy(a.raw, a.counts, a.offsets, a.ids, a.count, s);
+    else hipLaunchKernelGGL(k_lists_batch, dim3((unsigned)a.count), dim3(kListThreads), 0, s, a);
     SGPU_HIP(hipGetLastError());
 }
 

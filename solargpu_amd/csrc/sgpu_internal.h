@@ -94,7 +94,7 @@ struct TraceArgs {
 void launch_trace(const TraceArgs& a, hipStream_t s);
 
 struct ListArgs {
-    const int32_t* raw;
+    int32_t* raw;           // pass 1 leaves each voxel's kept ids at the front of its row
     int64_t v0, count, n_total;
     const DirConst* dirs;
     int svf_bytes;
@@ -132,7 +132,7 @@ void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_
 struct BatchListArgs {
     const TileDesc* tiles;
     int ntiles;
-    const int32_t* raw;
+    int32_t* raw;
     int64_t v0, count;
     int32_t* counts;
     const int64_t* offsets;
