@@ -14,8 +14,12 @@ namespace sgpu {
 constexpr int kDdaUnroll = 2;
 
 /* The walk for one ray, specialised on the direction's hemisphere (UPPER: the ray
- * leaves through maxz, else through z = 0) and on whether a 32-bit byte offset
- * addresses the whole grid (OFF32).  Exit tests are selects, not branches. */
+ * leaves through maxz, else through z = 0) and on whether 32-bit byte offsets
+ * address the grid (OFF32: W (H + 2) <= 2^30).  Exit tests are selects, not
+ * branches.  OFF32 steps the column and row byte offsets (4 ix, 4 W iy) as
+ * unsigned values: a column or row one or two steps off the grid wraps to a value
+ * >= 4 W, resp. >= 4 W H, so one unsigned compare each is the bounds test and their
+ * sum is the cell's byte offset. */
 template <bool COUNT, bool UPPER, bool OFF32>
 __device__ __forceinline__ int dda_walk(const float* __restrict__ grid, int W, unsigned uW, unsigned uH, float maxz,
                                         float gxf, float gyf, float base_z, const DirConst& dc,
@@ -29,6 +33,43 @@ __device__ __forceinline__ int dda_walk(const float* __restrict__ grid, int W, u
     float tmy = ((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy)) / dc.ay;
     const int sx = (dc.cx >= 0.0f) ? 1 : -1;
     const int sy = (dc.cy >= 0.0f) ? 1 : -1;
+    if constexpr (OFF32) {
+        unsigned sx4 = (unsigned)(4 * sx), sy4 = (unsigned)sy * 4u * uW;   // wave-uniform
+        /* keep the steps in VGPRs: a select between one and a wave-uniform step would
+           otherwise copy the step into a VGPR on every iteration */
+        asm volatile("" : "+v"(sx4), "+v"(sy4));
+        const unsigned lim_x = 4u * uW, lim_y = 4u * uW * uH;
+        unsigned cx4 = 4u * (unsigned)ix, ry4 = 4u * uW * (unsigned)iy;
+        for (;;) {
+            float t[kDdaUnroll], h[kDdaUnroll];
+            unsigned off[kDdaUnroll];
+            bool inside[kDdaUnroll];
+#pragma unroll
+            for (int u = 0; u < kDdaUnroll; ++u) {
+                const bool xs = tmx < tmy;   // a tie steps in y
+                t[u] = xs ? tmx : tmy;
+                cx4 = xs ? cx4 + sx4 : cx4;
+                ry4 = xs ? ry4 : ry4 + sy4;
+                tmx = xs ? tmx + dc.rx : tmx;
+                tmy = xs ? tmy : tmy + dc.ry;
+                inside[u] = cx4 < lim_x && ry4 < lim_y;
+                off[u] = inside[u] ? cx4 + ry4 : 0u;   // off-grid steps read a valid cell, unused
+                h[u] = *(const float*)((const char*)grid + off[u]);
+            }
+            unsigned res = 0;
+            bool done = false;
+#pragma unroll
+            for (int u = 0; u < kDdaUnroll; ++u) {
+                if (COUNT) steps += done ? 0 : 1;
+                const float zr = oz + dc.cz * t[u];
+                const bool zout = UPPER ? (zr > maxz) : (zr < 0.0f);
+                const bool stop = !inside[u] | zout | (h[u] - zr > 0.01f);
+                res = (stop & !done) ? ((inside[u] & !zout) ? (off[u] >> 2) + 1u : 0u) : res;
+                done |= stop;
+            }
+            if (done) return (int)res;
+        }
+    }
     for (;;) {
         float t[kDdaUnroll], h[kDdaUnroll];
         int cell[kDdaUnroll];
@@ -43,8 +84,7 @@ __device__ __forceinline__ int dda_walk(const float* __restrict__ grid, int W, u
             tmy = xs ? tmy : tmy + dc.ry;
             inside[u] = (unsigned)ix < uW && (unsigned)iy < uH;
             cell[u] = inside[u] ? ix + iy * W : 0;   // off-grid steps read a valid cell, unused
-            if (OFF32) h[u] = *(const float*)((const char*)grid + ((unsigned)cell[u] << 2));
-            else h[u] = grid[cell[u]];
+            h[u] = grid[cell[u]];
         }
         int res = 0;
         bool done = false;
@@ -68,7 +108,7 @@ template <bool COUNT>
 __device__ __forceinline__ int dda_first_hit(const float* __restrict__ grid, int W, unsigned uW, unsigned uH,
                                              float maxz, float gxf, float gyf, float base_z, const DirConst& dc,
                                              unsigned long long& steps) {
-    const bool off32 = (unsigned long long)uW * uH <= (1ull << 30);
+    const bool off32 = (unsigned long long)uW * (uH + 2ull) <= (1ull << 30);
     if (dc.info & 1) {
         return off32 ? dda_walk<COUNT, true, true>(grid, W, uW, uH, maxz, gxf, gyf, base_z, dc, steps)
                      : dda_walk<COUNT, true, false>(grid, W, uW, uH, maxz, gxf, gyf, base_z, dc, steps);
