@@ -3,6 +3,8 @@
  * array assembly of main.cpp:112-127.  Built with -ffp-contract=off. */
 #include <algorithm>
 #include <cmath>
+#include <thread>
+#include <vector>
 
 #include "sgpu_internal.h"
 
@@ -65,6 +67,32 @@ void top_slope_aspect(const float* H, int w, int h, double sx, double sy, float*
             aspect[i] = a;
         }
     }
+}
+
+std::vector<int32_t> trace_order(int64_t n_total, const float* grid, const int32_t* pp, const float* wall_z,
+                                 int64_t n_top, int block) {
+    std::vector<int32_t> perm((size_t)n_total);
+    const int64_t nblocks = (n_total + block - 1) / block;
+    const int nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(16, std::min<int64_t>(
+                                                        std::thread::hardware_concurrency(), nblocks / 64)));
+    auto work = [&](int t) {
+        std::vector<std::pair<std::pair<float, float>, int32_t>> key;
+        for (int64_t blk = t; blk < nblocks; blk += nthreads) {
+            const int64_t b = blk * block, e = std::min<int64_t>(n_total, b + block);
+            key.clear();
+            for (int64_t v = b; v < e; ++v) {
+                const float ground = grid[pp[v]];
+                key.push_back({{ground, v < n_top ? ground : wall_z[v - n_top]}, (int32_t)v});
+            }
+            std::stable_sort(key.begin(), key.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (int64_t i = b; i < e; ++i) perm[(size_t)i] = key[(size_t)(i - b)].second;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (std::thread& th : pool) th.join();
+    return perm;
 }
 
 std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32_t* pp, int tile) {

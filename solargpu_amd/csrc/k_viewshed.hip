@@ -33,8 +33,9 @@ constexpr int kHashSize = 4096;              // > 1.5 x the 2592 possible distin
 /* G directions per workgroup (2592 = 162 x 16 = 81 x 32 = 54 x 48): the tile shape the autotuner picks. */
 template <bool COUNT_STEPS, int G>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
-    const int64_t local = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
-    if (local >= a.count) return;
+    const int64_t lane_pos = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    if (lane_pos >= a.count) return;
+    const int64_t local = a.perm ? (int64_t)a.perm[a.v0 + lane_pos] - a.v0 : lane_pos;   // raw row
     const int64_t v = a.subset ? (int64_t)a.subset[a.v0 + local] : a.v0 + local;
     const int top = a.pix_parent[v];
     const int gx = top % a.W, gy = top / a.W;

@@ -52,6 +52,7 @@ int64_t choose_chunk(const sgpu_ctx* c, int64_t n) {
     const double per_voxel = (double)kDirs * 4.0 + 32.0;
     int64_t chunk = (int64_t)(0.4 * (double)free_b / per_voxel);
     chunk = std::min<int64_t>(chunk, (int64_t)1 << 20 << 2);
+    if (chunk > kTraceOrderBlock) chunk -= chunk % kTraceOrderBlock;   // chunks start on lane-order blocks
     chunk = std::max<int64_t>(chunk, 1);
     return std::min<int64_t>(chunk, n);
 }
@@ -335,10 +336,13 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
         counts = dalloc<int32_t>(chunk);
         off = dalloc<int64_t>(chunk + 1);
         scratch = dalloc<int64_t>(scan_scratch_elems(chunk));
+        /* the height-sorted lane order permutes within blocks of kTraceOrderBlock ids from 0:
+           usable when every chunk starts on a block (whole scene, chunk a multiple of it) */
+        const int32_t* perm = (!subset && (chunk >= N || chunk % kTraceOrderBlock == 0)) ? c->trace_perm : nullptr;
         for (int64_t v0 = 0; v0 < N; v0 += chunk) {
             const int64_t cnt = std::min(chunk, N - v0);
             TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, cnt, raw,
-                         steps, subset, c->tune.dirs_per_group};
+                         steps, subset, c->tune.dirs_per_group, perm};
             SGPU_HIP(hipEventRecord(ev_trace.a, c->stream));
             launch_trace(ta, c->stream);
             SGPU_HIP(hipEventRecord(ev_trace.b, c->stream));
@@ -962,6 +966,10 @@ int sgpu_set_scene(sgpu_ctx* c, const float* grid, int w, int h, int64_t n_total
         const std::vector<int32_t> order = processing_order(w, h, n_total, pp, 32);
         c->order = dalloc<int32_t>(n_total);
         SGPU_HIP(hipMemcpyAsync(c->order, order.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        const std::vector<int32_t> tperm = trace_order(n_total, grid, pp, wall_z, n_top, kTraceOrderBlock);
+        c->trace_perm = dalloc<int32_t>(n_total);
+        SGPU_HIP(hipMemcpyAsync(c->trace_perm, tperm.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice,
+                                c->stream));
         SGPU_HIP(hipStreamSynchronize(c->stream));
         c->scene = true;
     });
@@ -1096,7 +1104,7 @@ int sgpu_trace_raw(sgpu_ctx* c, int64_t v0, int64_t count, int32_t* out) {
         int32_t* raw = dalloc<int32_t>((size_t)count * kDirs);
         try {
             TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, count,
-                         raw, nullptr};
+                         raw, nullptr, nullptr, 0, nullptr};
             launch_trace(ta, c->stream);
             SGPU_HIP(hipMemcpyAsync(out, raw, (size_t)count * kDirs * sizeof(int32_t), hipMemcpyDeviceToHost,
                                     c->stream));

@@ -127,6 +127,7 @@ struct sgpu_ctx {
     float* slopes = nullptr;
     float* aspects = nullptr;
     int32_t* order = nullptr;    // reflection-pass processing order (tile-major cells, walls after their cell)
+    int32_t* trace_perm = nullptr;   // DDA lane order (blocks of kTraceOrderBlock ids sorted by height)
     bool scene = false;
 
     bool sky_set = false;
@@ -185,6 +186,7 @@ struct sgpu_ctx {
         sgpu::dfree(slopes);
         sgpu::dfree(aspects);
         sgpu::dfree(order);
+        sgpu::dfree(trace_perm);
         sgpu::dfree(sum);
         sgpu::dfree(comp);
         sgpu::dfree(hourly_top);

@@ -74,6 +74,12 @@ void synth_dsm(uint64_t seed, int w, int h, float cell_m, float density, int tre
 /* Reflection-pass processing order: cells in tiles of tile x tile (tiles and cells in raster
    order), each top voxel followed by the wall voxels it parents. */
 std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32_t* pix_parent, int tile);
+/* DDA processing order: within every block of `block` consecutive voxel ids, the ids sorted
+   by (height of the parent cell, voxel height), so a wave's 64 rays of one direction have
+   similar lengths.  perm[i] is the voxel traced by lane i; blocks start at id 0. */
+std::vector<int32_t> trace_order(int64_t n_total, const float* grid, const int32_t* pix_parent, const float* wall_z,
+                                 int64_t n_top, int block);
+constexpr int kTraceOrderBlock = 512;
 void synth_weather(uint64_t seed, const float* az_alt, float* dni, float* dhi);
 
 /* ---- device kernels (k_viewshed.hip, k_barle.hip, k_hourly.hip) ---- */
@@ -90,6 +96,9 @@ struct TraceArgs {
     unsigned long long* steps;  // optional DDA step counter
     const int32_t* subset;  // optional voxel id list
     int group;              // directions per workgroup: 16, 32 or 48 (0 = 32)
+    /* optional lane order (no subset; v0 and the chunk a multiple of kTraceOrderBlock): lane i
+       traces voxel perm[v0 + i] into raw row perm[v0 + i] - v0 */
+    const int32_t* perm;
 };
 void launch_trace(const TraceArgs& a, hipStream_t s);
 

@@ -76,7 +76,7 @@ void store(const std::string& key, const sgpu_tuning& t) {
 
 double time_trace(sgpu_ctx* c, int G, int64_t v0, int64_t n, int32_t* raw) {
     TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, n, raw,
-                 nullptr, nullptr, G};
+                 nullptr, nullptr, G, c->trace_perm};
     launch_trace(ta, c->stream);   // warm-up
     EventPair e;
     SGPU_HIP(hipEventRecord(e.a, c->stream));
@@ -126,7 +126,7 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
     t.from_cache = 0;
     /* DDA tile shape */
     const int64_t n = std::min<int64_t>(c->n_total, (int64_t)1 << 17);
-    const int64_t v0 = (c->n_total - n) / 2;
+    const int64_t v0 = (c->n_total - n) / 2 / kTraceOrderBlock * kTraceOrderBlock;   // a lane-order block
     int32_t* raw = dalloc<int32_t>((size_t)n * kDirs);
     double best = 1e30;
     try {
