@@ -53,6 +53,125 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     if (COUNT_STEPS) atomicAdd(a.steps, steps);
 }
 
+/* Pooled DDA (grids with 32-bit byte offsets, see dda.h).  Each wave owns the G x 64 rays of
+ * its 64 voxels (the k_trace lane order) over the workgroup's G directions, taken
+ * direction-major (the 64 voxels of one direction, then the next).  A lane whose ray has
+ * ended takes the next ray of the pool once kRefill lanes are idle (or none is busy), so the
+ * wave no longer waits for its longest ray of every direction; a refill sets up the rays of
+ * all idle lanes at once.  The hemisphere and step signs become per-lane values; the
+ * arithmetic of a ray is dda_walk's, so the hits are identical. */
+constexpr int kRefill = 16;
+
+template <bool COUNT_STEPS, int G>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
+    __shared__ DirConst sdir[G];
+    __shared__ float sorg[kTraceBlock / 64][3][64];   // gx, gy, base z of the wave's voxels
+    __shared__ int srow[kTraceBlock / 64][64];        // their raw rows
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = threadIdx.x; i < G; i += kTraceBlock) sdir[i] = a.dirs[blockIdx.y * G + i];
+    const int64_t lane_pos = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    const int64_t wave_pos = lane_pos - lane;
+    const int nvalid = (int)min<int64_t>(64, max<int64_t>(0, a.count - wave_pos));
+    if (lane < nvalid) {
+        const int64_t local = a.perm ? (int64_t)a.perm[a.v0 + lane_pos] - a.v0 : lane_pos;   // raw row
+        const int64_t v = a.subset ? (int64_t)a.subset[a.v0 + local] : a.v0 + local;
+        const int top = a.pix_parent[v];
+        const int gx = top % a.W, gy = top / a.W;
+        sorg[wave][0][lane] = (float)gx;
+        sorg[wave][1][lane] = (float)gy;
+        sorg[wave][2][lane] = (v < a.n_top) ? a.grid[gx + gy * a.W] : a.wall_z[v - a.n_top];
+        srow[wave][lane] = (int)local;
+    }
+    __syncthreads();
+    if (nvalid == 0) return;
+
+    const unsigned uW = (unsigned)a.W, uH = (unsigned)a.H;
+    const unsigned lim_x = 4u * uW, lim_y = 4u * uW * uH;
+    const float* __restrict__ grid = a.grid;
+    const float maxz = a.maxz;
+    unsigned long long steps = 0;
+    /* the lane's ray */
+    bool active = false;
+    unsigned cx4 = 0, ry4 = 0, sx4 = 0, sy4 = 0;
+    float tmx = 0.0f, tmy = 0.0f, oz = 0.0f, cz = 0.0f, rx = 0.0f, ry = 0.0f, zhi = 0.0f, zlo = 0.0f;
+    int32_t* out = nullptr;
+    const int total = G * 64;
+    int next = 0;   // wave-uniform pool cursor
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (next < total && (nidle >= kRefill || nidle == 64)) {
+            if (!active) {
+                const int j = next + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                const int k = j >> 6, vx = j & 63;
+                if (j < total && vx < nvalid) {
+                    /* dda_walk's set-up */
+                    const DirConst dc = sdir[k];
+                    const float gxf = sorg[wave][0][vx], gyf = sorg[wave][1][vx], base_z = sorg[wave][2][vx];
+                    const float ox = gxf + dc.cx * 2.0f;
+                    const float oy = gyf + dc.cy * 2.0f;
+                    oz = base_z + dc.cz * 2.0f;
+                    const int ix = (int)floorf(ox);
+                    const int iy = (int)floorf(oy);
+                    tmx = ((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix)) / dc.ax;
+                    tmy = ((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy)) / dc.ay;
+                    sx4 = (dc.cx >= 0.0f) ? 4u : (unsigned)-4;
+                    sy4 = (dc.cy >= 0.0f) ? 4u * uW : (unsigned)-(int)(4u * uW);
+                    cx4 = 4u * (unsigned)ix;
+                    ry4 = 4u * uW * (unsigned)iy;
+                    cz = dc.cz;
+                    rx = dc.rx;
+                    ry = dc.ry;
+                    const bool upper = dc.info & 1;   // the ray leaves through maxz, else through z = 0
+                    zhi = upper ? maxz : INFINITY;
+                    zlo = upper ? -INFINITY : 0.0f;
+                    out = a.raw + (int64_t)srow[wave][vx] * kDirs + blockIdx.y * G + k;
+                    active = true;
+                }
+            }
+            next += nidle;
+        }
+        if (__ballot(active) == 0) {
+            if (next >= total) break;
+            continue;
+        }
+        if (active) {
+            float t[kDdaUnroll], h[kDdaUnroll];
+            unsigned off[kDdaUnroll];
+            bool inside[kDdaUnroll];
+#pragma unroll
+            for (int u = 0; u < kDdaUnroll; ++u) {
+                const bool xs = tmx < tmy;   // a tie steps in y
+                t[u] = xs ? tmx : tmy;
+                cx4 = xs ? cx4 + sx4 : cx4;
+                ry4 = xs ? ry4 : ry4 + sy4;
+                tmx = xs ? tmx + rx : tmx;
+                tmy = xs ? tmy : tmy + ry;
+                inside[u] = cx4 < lim_x && ry4 < lim_y;
+                off[u] = inside[u] ? cx4 + ry4 : 0u;   // off-grid steps read a valid cell, unused
+                h[u] = *(const float*)((const char*)grid + off[u]);
+            }
+            unsigned res = 0;
+            bool done = false;
+#pragma unroll
+            for (int u = 0; u < kDdaUnroll; ++u) {
+                if (COUNT_STEPS) steps += done ? 0 : 1;
+                const float zr = oz + cz * t[u];
+                const bool zout = (zr > zhi) | (zr < zlo);
+                const bool stop = !inside[u] | zout | (h[u] - zr > 0.01f);
+                res = (stop & !done) ? ((inside[u] & !zout) ? (off[u] >> 2) + 1u : 0u) : res;
+                done |= stop;
+            }
+            if (done) {
+                *out = (int32_t)res;
+                active = false;
+            }
+        }
+    }
+    if (COUNT_STEPS) atomicAdd(a.steps, steps);
+}
+
 /* Inclusive wave64 scan. */
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = threadIdx.x & 63;
@@ -303,6 +422,25 @@ void launch_trace(const TraceArgs& a, hipStream_t s) {
     const int G = a.group ? a.group : 32;
     const unsigned bx = (unsigned)((a.count + kTraceBlock - 1) / kTraceBlock);
     dim3 grid(bx, (unsigned)(kDirs / G));
+    if ((uint64_t)a.W * ((uint64_t)a.H + 2) <= (1ull << 30)) {   // 32-bit byte offsets: the pooled walk
+        switch (G) {
+            case 16:
+                if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 16>), grid, dim3(kTraceBlock), 0, s, a);
+                else hipLaunchKernelGGL((k_trace_pool<false, 16>), grid, dim3(kTraceBlock), 0, s, a);
+                break;
+            case 32:
+                if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 32>), grid, dim3(kTraceBlock), 0, s, a);
+                else hipLaunchKernelGGL((k_trace_pool<false, 32>), grid, dim3(kTraceBlock), 0, s, a);
+                break;
+            case 48:
+                if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 48>), grid, dim3(kTraceBlock), 0, s, a);
+                else hipLaunchKernelGGL((k_trace_pool<false, 48>), grid, dim3(kTraceBlock), 0, s, a);
+                break;
+            default: fail(SGPU_ERR_ARG, "directions per group must be 16, 32 or 48");
+        }
+        SGPU_HIP(hipGetLastError());
+        return;
+    }
     switch (G) {
         case 16:
             if (a.steps) hipLaunchKernelGGL((k_trace<true, 16>), grid, dim3(kTraceBlock), 0, s, a);
