@@ -1,8 +1,8 @@
 /* GPU ba-RLE encoder (format of the reference's barle_compression.h:7-44 and
  * encoder barle_compression.cpp:12-207), one wave per voxel, every step
  * wave-parallel:
- *   - sort: lists of <= 512 ids are bitonic-sorted descending in registers
- *     (E = 1..8 ids per lane, cross-lane stages by xor shuffles, stages of
+ *   - sort: lists of <= 1024 ids are bitonic-sorted descending in registers
+ *     (E = 1..16 ids per lane, cross-lane stages by xor shuffles, stages of
  *     stride >= 64 inside each lane); longer lists in LDS;
  *   - duplicates are dropped with a wave-ordered compaction (ballot);
  *   - delta width classes and run starts (width changes) in parallel;
@@ -17,7 +17,8 @@
  * Pass 1 writes per-voxel byte counts; pass 2 writes the bytes at scanned
  * offsets, so the stream is byte-identical to the reference encoder's.
  * Lists of <= 512 ids (most) take a small-LDS instantiation over all voxels;
- * longer ones are queued by pass 1 and run by a fixed grid of the full-size one. */
+ * pass 1 queues those of 513..1024 ids (registers, 13 KB of LDS) and the longer
+ * ones (LDS sort, 52 KB) for two fixed grids striding over the queues. */
 #include "sgpu_internal.h"
 
 namespace sgpu {
@@ -25,7 +26,9 @@ namespace {
 
 static_assert(kDirs <= 4096, "a voxel has at most one distinct hit per direction");
 constexpr int kShort = 512;
+constexpr int kMedium = 1024;
 constexpr int kLongGrid = 4096;
+constexpr int kHugeGrid = 1024;
 
 __device__ __forceinline__ unsigned width_class(unsigned x) {   // barle_compression.cpp:12-18
     return x <= 15u ? 0u : x <= 255u ? 1u : x <= 65535u ? 2u : x <= 16777215u ? 3u : 4u;
@@ -222,12 +225,14 @@ __device__ __forceinline__ unsigned delta_at(const unsigned* vals, unsigned i) {
 template <bool WRITE, int MAXP>
 __device__ void encode_voxel(const BarleArgs& a, EncLds<MAXP>& s, int64_t v, int64_t b, int n, int lane) {
     int m;
-    if constexpr (MAXP <= 512) {
+    if constexpr (MAXP <= kShort) {
         const int32_t* ids = a.csr_ids + b;
         if (n <= 64) m = sort_unique_regs<1, 6>(ids, n, s.vals, lane);
         else if (n <= 128) m = sort_unique_regs<2, 7>(ids, n, s.vals, lane);
         else if (n <= 256) m = sort_unique_regs<4, 8>(ids, n, s.vals, lane);
         else m = sort_unique_regs<8, 9>(ids, n, s.vals, lane);
+    } else if constexpr (MAXP <= kMedium) {
+        m = sort_unique_regs<16, 10>(a.csr_ids + b, n, s.vals, lane);
     } else {
         int P = 64;
         while (P < n) P <<= 1;
@@ -354,7 +359,8 @@ __device__ void encode_voxel(const BarleArgs& a, EncLds<MAXP>& s, int64_t v, int
     }
 }
 
-/* Lists of <= kShort ids; pass 1 queues the longer ones. */
+/* Lists of <= kShort ids; pass 1 queues the longer ones: up to kMedium ids at the front of
+   long_ids (count long_count[0]), more from its back (count long_count[1]). */
 template <bool WRITE>
 __global__ __launch_bounds__(64) void k_barle_short(BarleArgs a) {
     __shared__ EncLds<kShort> s;
@@ -362,7 +368,10 @@ __global__ __launch_bounds__(64) void k_barle_short(BarleArgs a) {
     const int64_t b = a.csr_off[v];
     const int n = (int)(a.csr_off[v + 1] - b);
     if (n > kShort) {
-        if (!WRITE && threadIdx.x == 0) a.long_ids[atomicAdd(a.long_count, 1u)] = (int32_t)v;
+        if (!WRITE && threadIdx.x == 0) {
+            if (n <= kMedium) a.long_ids[atomicAdd(&a.long_count[0], 1u)] = (int32_t)v;
+            else a.long_ids[a.n - 1 - (int64_t)atomicAdd(&a.long_count[1], 1u)] = (int32_t)v;
+        }
         return;
     }
     if (n <= 0) {
@@ -372,13 +381,26 @@ __global__ __launch_bounds__(64) void k_barle_short(BarleArgs a) {
     encode_voxel<WRITE, kShort>(a, s, v, b, n, threadIdx.x);
 }
 
-/* The queued long lists, a fixed grid striding over the queue. */
+/* The queued lists of kShort + 1 .. kMedium ids, a fixed grid striding over the queue. */
 template <bool WRITE>
 __global__ __launch_bounds__(64) void k_barle_long(BarleArgs a) {
-    __shared__ EncLds<4096> s;
-    const unsigned count = *a.long_count;
+    __shared__ EncLds<kMedium> s;
+    const unsigned count = a.long_count[0];
     for (unsigned q = blockIdx.x; q < count; q += gridDim.x) {
         const int64_t v = a.long_ids[q];
+        const int64_t b = a.csr_off[v];
+        encode_voxel<WRITE, kMedium>(a, s, v, b, (int)(a.csr_off[v + 1] - b), threadIdx.x);
+        __syncthreads();   // LDS reused by the next voxel
+    }
+}
+
+/* The queued lists of more than kMedium ids (back of long_ids). */
+template <bool WRITE>
+__global__ __launch_bounds__(64) void k_barle_huge(BarleArgs a) {
+    __shared__ EncLds<4096> s;
+    const unsigned count = a.long_count[1];
+    for (unsigned q = blockIdx.x; q < count; q += gridDim.x) {
+        const int64_t v = a.long_ids[a.n - 1 - (int64_t)q];
         const int64_t b = a.csr_off[v];
         encode_voxel<WRITE, 4096>(a, s, v, b, (int)(a.csr_off[v + 1] - b), threadIdx.x);
         __syncthreads();   // LDS reused by the next voxel
@@ -394,10 +416,12 @@ void launch_barle(const BarleArgs& a, bool write_pass, hipStream_t s) {
     if (write_pass) {
         hipLaunchKernelGGL(k_barle_short<true>, g, blk, 0, s, a);
         hipLaunchKernelGGL(k_barle_long<true>, dim3(kLongGrid), blk, 0, s, a);
+        hipLaunchKernelGGL(k_barle_huge<true>, dim3(kHugeGrid), blk, 0, s, a);
     } else {
-        SGPU_HIP(hipMemsetAsync(a.long_count, 0, sizeof(unsigned), s));
+        SGPU_HIP(hipMemsetAsync(a.long_count, 0, 2 * sizeof(unsigned), s));
         hipLaunchKernelGGL(k_barle_short<false>, g, blk, 0, s, a);
         hipLaunchKernelGGL(k_barle_long<false>, dim3(kLongGrid), blk, 0, s, a);
+        hipLaunchKernelGGL(k_barle_huge<false>, dim3(kHugeGrid), blk, 0, s, a);
     }
     SGPU_HIP(hipGetLastError());
 }

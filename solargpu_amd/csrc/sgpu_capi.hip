@@ -486,7 +486,7 @@ void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t
         scr = dalloc<int64_t>(scan_scratch_elems(N));
         boff = dalloc<int64_t>(N + 1);
         long_ids = dalloc<int32_t>(N);
-        long_count = dalloc<unsigned>(1);
+        long_count = dalloc<unsigned>(2);
         ba.long_ids = long_ids;
         ba.long_count = long_count;
         ba.sizes = sizes;

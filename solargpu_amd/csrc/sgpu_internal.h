@@ -161,8 +161,9 @@ struct BarleArgs {
     int32_t* sizes;         // size pass output
     const int64_t* out_off; // write pass input
     uint8_t* out;           // write pass output
-    int32_t* long_ids;      // queue of voxels with more than 512 ids (n entries; filled by the size pass)
-    unsigned* long_count;   // its length (zeroed by the size pass)
+    int32_t* long_ids;      // queues of voxels with more than 512 ids (n entries; filled by the size pass):
+                            // up to 1024 ids from the front, more from the back
+    unsigned* long_count;   // their lengths, 2 entries (zeroed by the size pass)
 };
 void launch_barle(const BarleArgs& a, bool write_pass, hipStream_t s);
 

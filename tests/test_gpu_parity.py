@@ -71,6 +71,9 @@ def test_barle_encoder_edge_lists():
     rng = np.random.default_rng(5)
     lists = [[], [1], [15], [16], [255, 256], [70000, 3], list(range(600, 0, -1)),
              [1 << 25, (1 << 25) - 1, 5, 4], list(range(1, 2592 * 3, 3))[:2592]]
+    # the encoder's three list classes: <= 512 ids, 513..1024, longer (and their boundaries)
+    for n in (512, 513, 1000, 1024, 1025, 2047):
+        lists.append(list(rng.choice(1 << 24, size=n, replace=False) + 1))
     for _ in range(sc.num_total - len(lists)):
         n = int(rng.integers(0, 60))
         lists.append(list(rng.choice(sc.num_top, size=n, replace=False) + 1))
