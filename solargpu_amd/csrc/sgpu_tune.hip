@@ -6,8 +6,8 @@
  *   stage-2 pass   hours per pass K in {64, 128, 256} x waves per reflection
  *                  workgroup in {2, 4} (LDS footprint and occupancy) x XCD
  *                  band in {0, 64, 256} (contiguous eighths or side-by-side
- *                  bands), then receiver groups in {0, 8, 32, 64, 128} x band in
- *                  {0, 256} for the best of those: three passes each on the
+ *                  bands), then for the best K: waves {2, 4} x receiver groups
+ *                  {8, 32, 64, 128} x band {0, 256}: three passes each on the
  *                  live lists, scratch accumulators.
  * Waves, bands and receiver groups give bit-identical results; K only regroups
  * the hours of the Kahan-summed annual totals (same sums to FP32 rounding).  The winners are
@@ -160,9 +160,10 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
         for (int K : {64, 128, 256})
             for (int waves : {2, 4})
                 for (int band : {0, 64, 256}) trial(K, waves, band, 0);
-        const int K = t.hours_per_pass, waves = t.waves_per_group;
-        for (int group : {8, 32, 64, 128})
-            for (int band : {0, 256}) trial(K, waves, band, group);
+        const int K = t.hours_per_pass;
+        for (int waves : {2, 4})   // the best waves count with receiver groups can differ
+            for (int group : {8, 32, 64, 128})
+                for (int band : {0, 256}) trial(K, waves, band, group);
         t.voxel_hours_per_s = best_rate;
         c->irr_ok = false;   // the scratch passes overwrote the annual accumulators
     }
