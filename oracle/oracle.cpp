@@ -617,6 +617,29 @@ void orc_top_slope_aspect(const float* H, int w, int h, double sx, double sy, fl
     }
 }
 
+int64_t orc_check_setup_division(const float* cdir, int ndir, int gmax) {
+    int64_t bad = 0;
+    for (int k = 0; k < ndir; ++k) {
+        const DirConst d = make_dir(cdir + 3 * k, 1);
+        for (int axis = 0; axis < 2; ++axis) {
+            const float c = axis ? d.cy : d.cx, a = axis ? d.ay : d.ax, r = axis ? d.ry : d.rx;
+            for (int g = 0; g < gmax; ++g) {
+                const float o = (float)g + c * 2.0f;   // the set-up of trace() above
+                const int i = (int)std::floor(o);
+                const float num = (c >= 0.0f) ? ((float)i + 1.0f - o) : (o - (float)i);
+                const float q_ieee = num / a;
+                const float q0 = num * r;
+                const float q = std::fma(std::fma(-q0, a, num), r, q0);
+                uint32_t x, y;
+                std::memcpy(&x, &q_ieee, 4);
+                std::memcpy(&y, &q, 4);
+                bad += (x != y);
+            }
+        }
+    }
+    return bad;
+}
+
 int orc_trace_ray(const float* grid, int w, int h, float maxz, float gxf, float gyf, float base_z,
                   const float* cdir3, int is_upper, int* steps) {
     DirConst d = make_dir(cdir3, is_upper);

@@ -62,6 +62,12 @@ int64_t orc_detect_walls(const float* heights, int w, int h, float voxel, int* p
 void orc_top_slope_aspect(const float* heights, int w, int h, double sx, double sy, float* slope,
                           float* aspect);
 
+/* Pin of the product's ray set-up: for every direction (cdir, ndir) and axis and every integer
+   origin coordinate g < gmax, the set-up quotient (fraction to the first cell edge) / |c| by
+   IEEE division against q0 = RN(num * rx), q = fma(fma(-q0, a, num), rx, q0) with
+   rx = RN(1 / a) (what the GPU computes).  Returns the number of quotients that differ. */
+int64_t orc_check_setup_division(const float* cdir, int ndir, int gmax);
+
 /* One DDA ray (viewshed_kernels.cu:7-59).  Returns hit cell id + 1, or 0.  *steps gets the step count. */
 int orc_trace_ray(const float* grid, int w, int h, float maxz, float ox_cell, float oy_cell, float base_z,
                   const float* cdir3, int is_upper, int* steps);

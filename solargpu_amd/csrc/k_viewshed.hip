@@ -114,8 +114,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
                     oz = base_z + dc.cz * 2.0f;
                     const int ix = (int)floorf(ox);
                     const int iy = (int)floorf(oy);
-                    tmx = ((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix)) / dc.ax;
-                    tmy = ((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy)) / dc.ay;
+                    tmx = div_by((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix), dc.ax, dc.rx);
+                    tmy = div_by((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy), dc.ay, dc.ry);
                     sx4 = (dc.cx >= 0.0f) ? 4u : (unsigned)-4;
                     sy4 = (dc.cy >= 0.0f) ? 4u * uW : (unsigned)-(int)(4u * uW);
                     cx4 = 4u * (unsigned)ix;

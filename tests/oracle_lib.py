@@ -35,7 +35,8 @@ def load_oracle():
     if not os.path.exists(ORACLE_SO):
         raise ImportError(f"{ORACLE_SO} missing: run make -C oracle")
     lib = ctypes.CDLL(ORACLE_SO)
-    for name in ("orc_viewshed", "orc_detect_walls", "orc_barle_encode_voxel", "orc_barle_encode_all"):
+    for name in ("orc_viewshed", "orc_detect_walls", "orc_barle_encode_voxel", "orc_barle_encode_all",
+                 "orc_check_setup_division"):
         getattr(lib, name).restype = ctypes.c_int64
     return lib
 
@@ -95,6 +96,13 @@ def top_slope_aspect(h, sx, sy):
     a = np.zeros(H * W, np.float32)
     O.orc_top_slope_aspect(_ptr(h), W, H, ctypes.c_double(sx), ctypes.c_double(sy), _ptr(s), _ptr(a))
     return s, a
+
+
+def check_setup_division(cdir, gmax):
+    """Set-up quotients (all directions, both axes, origins < gmax) where the GPU's reciprocal +
+    two-FMA form differs from IEEE division; 0 pins the product's set-up."""
+    c = _c(cdir, np.float32)
+    return O.orc_check_setup_division(_ptr(c), len(c) // 3, int(gmax))
 
 
 def trace_ray(grid, maxz, gx, gy, base_z, cdir3, upper):
