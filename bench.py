@@ -228,6 +228,11 @@ def main():
 
     extra = {}
     exact = ctx.irradiance()
+    # DDA cell steps of the same viewshed (a counting run after the timed steps; counters cost time)
+    ctx.set_step_counting(True)
+    ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
+    dda_steps = ctx.info()["dda_steps"]
+    ctx.set_step_counting(False)
     if not args.no_surrogate and os.path.exists(WEIGHTS) and args.hour_stride == 1:
         extra["surrogate"] = surrogate_leg(sg, sc, az_alt, dni, dhi, cell, ctx.tuning(), exact,
                                            (t_vs + t_hr) / args.steps, H)
@@ -267,6 +272,8 @@ def main():
                    "hours_per_pass": Kp, "nnz": inf["nnz"], "barle_bytes": inf["barle_bytes"]},
         "dda_rays_per_s": rays,
         "dda_rays_per_s_trace_kernel": N * 2592 * args.steps / t_tr,
+        "dda_cell_steps": dda_steps,
+        "dda_cell_steps_per_s_trace_kernel": dda_steps * args.steps / t_tr,
         "stage1_ms": t_vs * 1e3 / args.steps,
         "stage2_ms": t_hr * 1e3 / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
