@@ -112,6 +112,76 @@ __device__ __forceinline__ int dda_walk(const float* __restrict__ grid, int W, u
     }
 }
 
+/* One ray of a pooled walk (lanes of a wave on different rays): dda_walk's OFF32 state
+ * with the direction's step signs and hemisphere as per-lane values. */
+struct PoolRay {
+    unsigned cx4, ry4, sx4, sy4;   // column / row byte offsets and their steps
+    float tmx, tmy, oz, cz, rx, ry;
+    float zhi, zlo;                // leaves above zhi (upper) or below zlo (lower)
+};
+
+/* dda_walk's set-up of the ray from integer cell (gx, gy) at base_z along dc. */
+__device__ __forceinline__ void pool_ray_setup(PoolRay& r, const DirConst& dc, float gxf, float gyf, float base_z,
+                                               unsigned uW, float maxz) {
+    const float ox = gxf + dc.cx * 2.0f;
+    const float oy = gyf + dc.cy * 2.0f;
+    r.oz = base_z + dc.cz * 2.0f;
+    const int ix = (int)floorf(ox);
+    const int iy = (int)floorf(oy);
+    r.tmx = div_by((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix), dc.ax, dc.rx);
+    r.tmy = div_by((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy), dc.ay, dc.ry);
+    r.sx4 = (dc.cx >= 0.0f) ? 4u : (unsigned)-4;
+    r.sy4 = (dc.cy >= 0.0f) ? 4u * uW : (unsigned)-(int)(4u * uW);
+    r.cx4 = 4u * (unsigned)ix;
+    r.ry4 = 4u * uW * (unsigned)iy;
+    r.cz = dc.cz;
+    r.rx = dc.rx;
+    r.ry = dc.ry;
+    const bool upper = dc.info & 1;
+    r.zhi = upper ? maxz : INFINITY;
+    r.zlo = upper ? -INFINITY : 0.0f;
+}
+
+/* kDdaUnroll steps of a pooled ray (dda_walk's OFF32 steps and tests); true when the ray has
+ * ended, with *res the hit cell id + 1 or 0. */
+template <bool COUNT>
+__device__ __forceinline__ bool pool_ray_steps(PoolRay& r, const float* __restrict__ grid, unsigned lim_x,
+                                               unsigned lim_y, unsigned long long& steps, unsigned& res) {
+    float t[kDdaUnroll], h[kDdaUnroll];
+    unsigned off[kDdaUnroll];
+    bool inside[kDdaUnroll];
+#pragma unroll
+    for (int u = 0; u < kDdaUnroll; ++u) {
+        const bool xs = r.tmx < r.tmy;   // a tie steps in y
+        t[u] = xs ? r.tmx : r.tmy;
+        r.cx4 = xs ? r.cx4 + r.sx4 : r.cx4;
+        r.ry4 = xs ? r.ry4 : r.ry4 + r.sy4;
+        r.tmx = xs ? r.tmx + r.rx : r.tmx;
+        r.tmy = xs ? r.tmy : r.tmy + r.ry;
+        inside[u] = r.cx4 < lim_x && r.ry4 < lim_y;
+        off[u] = inside[u] ? r.cx4 + r.ry4 : 0u;   // off-grid steps read a valid cell, unused
+        h[u] = *(const float*)((const char*)grid + off[u]);
+    }
+    unsigned out = 0;
+    bool done = false;
+#pragma unroll
+    for (int u = 0; u < kDdaUnroll; ++u) {
+        if (COUNT) steps += done ? 0 : 1;
+        const float zr = r.oz + r.cz * t[u];
+        const bool zout = (zr > r.zhi) | (zr < r.zlo);
+        const bool stop = !inside[u] | zout | (h[u] - zr > 0.01f);
+        out = (stop & !done) ? ((inside[u] & !zout) ? (off[u] >> 2) + 1u : 0u) : out;
+        done |= stop;
+    }
+    res = out;
+    return done;
+}
+
+/* Grids addressed by 32-bit byte offsets with room for the column / row wrap-around tests. */
+__device__ __host__ __forceinline__ bool grid_off32(unsigned long long w, unsigned long long h) {
+    return w * (h + 2ull) <= (1ull << 30);
+}
+
 /* First hit (cell id + 1) of the ray from integer cell (gx, gy) at height
  * base_z in direction dc, or 0.  `steps` counts DDA steps when COUNT.  The
  * direction is the same for every lane of a wave, so the dispatch is uniform. */
@@ -119,7 +189,7 @@ template <bool COUNT>
 __device__ __forceinline__ int dda_first_hit(const float* __restrict__ grid, int W, unsigned uW, unsigned uH,
                                              float maxz, float gxf, float gyf, float base_z, const DirConst& dc,
                                              unsigned long long& steps) {
-    const bool off32 = (unsigned long long)uW * (uH + 2ull) <= (1ull << 30);
+    const bool off32 = grid_off32(uW, uH);
     if (dc.info & 1) {
         return off32 ? dda_walk<COUNT, true, true>(grid, W, uW, uH, maxz, gxf, gyf, base_z, dc, steps)
                      : dda_walk<COUNT, true, false>(grid, W, uW, uH, maxz, gxf, gyf, base_z, dc, steps);

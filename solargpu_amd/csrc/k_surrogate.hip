@@ -1,6 +1,8 @@
 /* Surrogate path kernels on gfx950 (no reference counterpart):
  *   k_sun_svf    the SVF bitmap from the ~265 sun-bin directions only (same DDA
  *                as the plain tracer, so the bits are identical), lane = voxel;
+ *                k_sun_svf_pool: the same bits, idle lanes refilled with rays as
+ *                in k_trace_pool (grids with 32-bit byte offsets);
  *   k_annual_dd  exact annual direct + diffuse per voxel (FP32 Kahan over all
  *                sunlit hours) and the diffuse part alone -- the solar kernel
  *                of solar_kernel.cu:54-77 without the reflection term;
@@ -51,6 +53,81 @@ __global__ __launch_bounds__(256) void k_sun_svf(SurArgs a) {
         a.svf[(int64_t)byte * a.n_total + v] = (uint8_t)bits;
     }
     a.sun_hits[v] = (uint16_t)hits;
+}
+
+/* Pooled sun-bin SVF (grids with 32-bit byte offsets): each wave owns the n_sun_dirs x 64 rays
+ * of its 64 voxels, direction-major; a lane whose ray has ended takes the next one once
+ * kSunRefill lanes are idle (or none is busy).  A hit sets the voxel's bit in LDS words (bit d
+ * of word d / 32 is SVF byte d / 8, bit d % 8); the bytes and the hit count are written at
+ * the end.  The rays are dda_walk's (pool_ray_*), so the bits equal k_sun_svf's. */
+constexpr int kSunRefill = 16;
+
+__global__ __launch_bounds__(256) void k_sun_svf_pool(SurArgs a) {
+    extern __shared__ unsigned sbits[];   // [4 waves][64 voxels][nw words]
+    __shared__ float sorg[4][3][64];
+    const int nw = (a.n_sun_dirs + 31) >> 5;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (int i = threadIdx.x; i < 4 * 64 * nw; i += 256) sbits[i] = 0u;
+    const int nvalid = (int)min<int64_t>(64, max<int64_t>(0, a.n_total - (v - lane)));
+    if (lane < nvalid) {
+        int top, gx, gy;
+        float z;
+        voxel_origin(a, v, top, gx, gy, z);
+        sorg[wave][0][lane] = (float)gx;
+        sorg[wave][1][lane] = (float)gy;
+        sorg[wave][2][lane] = z;
+    }
+    __syncthreads();
+    unsigned* wbits = sbits + wave * 64 * nw;
+    if (nvalid > 0) {
+        const unsigned uW = (unsigned)a.W;
+        const unsigned lim_x = 4u * uW, lim_y = 4u * uW * (unsigned)a.H;
+        unsigned long long steps = 0;
+        bool active = false;
+        PoolRay ray{};
+        int rd = 0, rvx = 0;   // the lane's ray: sun bin, voxel
+        const int total = a.n_sun_dirs * 64;
+        int next = 0;          // wave-uniform pool cursor
+        for (;;) {
+            const uint64_t idle = __ballot(!active);
+            const int nidle = __popcll(idle);
+            if (next < total && (nidle >= kSunRefill || nidle == 64)) {
+                if (!active) {
+                    const int j = next + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                    const int d = j >> 6, vx = j & 63;
+                    if (j < total && vx < nvalid) {
+                        pool_ray_setup(ray, a.dirs[a.sun_dirs[d]], sorg[wave][0][vx], sorg[wave][1][vx],
+                                       sorg[wave][2][vx], uW, a.maxz);
+                        rd = d;
+                        rvx = vx;
+                        active = true;
+                    }
+                }
+                next += nidle;
+            }
+            if (__ballot(active) == 0) {
+                if (next >= total) break;
+                continue;
+            }
+            if (active) {
+                unsigned res;
+                if (pool_ray_steps<false>(ray, a.grid, lim_x, lim_y, steps, res)) {
+                    if (res) atomicOr(&wbits[rvx * nw + (rd >> 5)], 1u << (rd & 31));
+                    active = false;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < nvalid) {
+        const unsigned* vb = wbits + lane * nw;
+        for (int b = 0; b < a.svf_bytes; ++b) a.svf[(int64_t)b * a.n_total + v] = (uint8_t)(vb[b >> 2] >> (8 * (b & 3)));
+        int hits = 0;
+        for (int w = 0; w < nw; ++w) hits += __popc(vb[w]);
+        a.sun_hits[v] = (uint16_t)hits;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_annual_dd(SurArgs a) {
@@ -241,7 +318,14 @@ __global__ __launch_bounds__(256) void k_compact(const int32_t* flags, const int
 }  // namespace
 
 void launch_sun_svf(const SurArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_sun_svf, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);
+    const dim3 g((unsigned)((a.n_total + 255) / 256));
+    if (grid_off32((unsigned)a.W, (unsigned)a.H)) {
+        if (a.n_sun_dirs > kAzBins * kElBins) fail(SGPU_ERR_ARG, "more sun bins than upper directions");
+        const size_t lds = (size_t)4 * 64 * ((a.n_sun_dirs + 31) / 32) * sizeof(unsigned);
+        hipLaunchKernelGGL(k_sun_svf_pool, g, dim3(256), lds, s, a);
+    } else {
+        hipLaunchKernelGGL(k_sun_svf, g, dim3(256), 0, s, a);
+    }
     SGPU_HIP(hipGetLastError());
 }
 void launch_annual_dd(const SurArgs& a, hipStream_t s) {

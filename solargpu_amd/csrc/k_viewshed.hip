@@ -92,8 +92,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
     unsigned long long steps = 0;
     /* the lane's ray */
     bool active = false;
-    unsigned cx4 = 0, ry4 = 0, sx4 = 0, sy4 = 0;
-    float tmx = 0.0f, tmy = 0.0f, oz = 0.0f, cz = 0.0f, rx = 0.0f, ry = 0.0f, zhi = 0.0f, zlo = 0.0f;
+    PoolRay ray{};
     int32_t* out = nullptr;
     const int total = G * 64;
     int next = 0;   // wave-uniform pool cursor
@@ -106,26 +105,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
                 const int k = j >> 6, vx = j & 63;
                 if (j < total && vx < nvalid) {
-                    /* dda_walk's set-up */
-                    const DirConst dc = sdir[k];
-                    const float gxf = sorg[wave][0][vx], gyf = sorg[wave][1][vx], base_z = sorg[wave][2][vx];
-                    const float ox = gxf + dc.cx * 2.0f;
-                    const float oy = gyf + dc.cy * 2.0f;
-                    oz = base_z + dc.cz * 2.0f;
-                    const int ix = (int)floorf(ox);
-                    const int iy = (int)floorf(oy);
-                    tmx = div_by((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix), dc.ax, dc.rx);
-                    tmy = div_by((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy), dc.ay, dc.ry);
-                    sx4 = (dc.cx >= 0.0f) ? 4u : (unsigned)-4;
-                    sy4 = (dc.cy >= 0.0f) ? 4u * uW : (unsigned)-(int)(4u * uW);
-                    cx4 = 4u * (unsigned)ix;
-                    ry4 = 4u * uW * (unsigned)iy;
-                    cz = dc.cz;
-                    rx = dc.rx;
-                    ry = dc.ry;
-                    const bool upper = dc.info & 1;   // the ray leaves through maxz, else through z = 0
-                    zhi = upper ? maxz : INFINITY;
-                    zlo = upper ? -INFINITY : 0.0f;
+                    pool_ray_setup(ray, sdir[k], sorg[wave][0][vx], sorg[wave][1][vx], sorg[wave][2][vx], uW, maxz);
                     out = a.raw + (int64_t)srow[wave][vx] * kDirs + blockIdx.y * G + k;
                     active = true;
                 }
@@ -137,33 +117,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
             continue;
         }
         if (active) {
-            float t[kDdaUnroll], h[kDdaUnroll];
-            unsigned off[kDdaUnroll];
-            bool inside[kDdaUnroll];
-#pragma unroll
-            for (int u = 0; u < kDdaUnroll; ++u) {
-                const bool xs = tmx < tmy;   // a tie steps in y
-                t[u] = xs ? tmx : tmy;
-                cx4 = xs ? cx4 + sx4 : cx4;
-                ry4 = xs ? ry4 : ry4 + sy4;
-                tmx = xs ? tmx + rx : tmx;
-                tmy = xs ? tmy : tmy + ry;
-                inside[u] = cx4 < lim_x && ry4 < lim_y;
-                off[u] = inside[u] ? cx4 + ry4 : 0u;   // off-grid steps read a valid cell, unused
-                h[u] = *(const float*)((const char*)grid + off[u]);
-            }
-            unsigned res = 0;
-            bool done = false;
-#pragma unroll
-            for (int u = 0; u < kDdaUnroll; ++u) {
-                if (COUNT_STEPS) steps += done ? 0 : 1;
-                const float zr = oz + cz * t[u];
-                const bool zout = (zr > zhi) | (zr < zlo);
-                const bool stop = !inside[u] | zout | (h[u] - zr > 0.01f);
-                res = (stop & !done) ? ((inside[u] & !zout) ? (off[u] >> 2) + 1u : 0u) : res;
-                done |= stop;
-            }
-            if (done) {
+            unsigned res;
+            if (pool_ray_steps<COUNT_STEPS>(ray, grid, lim_x, lim_y, steps, res)) {
                 *out = (int32_t)res;
                 active = false;
             }
@@ -422,7 +377,7 @@ void launch_trace(const TraceArgs& a, hipStream_t s) {
     const int G = a.group ? a.group : 32;
     const unsigned bx = (unsigned)((a.count + kTraceBlock - 1) / kTraceBlock);
     dim3 grid(bx, (unsigned)(kDirs / G));
-    if ((uint64_t)a.W * ((uint64_t)a.H + 2) <= (1ull << 30)) {   // 32-bit byte offsets: the pooled walk
+    if (grid_off32((unsigned)a.W, (unsigned)a.H)) {   // 32-bit byte offsets: the pooled walk
         switch (G) {
             case 16:
                 if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 16>), grid, dim3(kTraceBlock), 0, s, a);
