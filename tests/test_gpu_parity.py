@@ -46,6 +46,26 @@ def test_viewshed_lists_svf_bit_exact(seed, w, h):
         np.testing.assert_array_equal(svf, ref["svf"])
 
 
+@pytest.mark.parametrize("G,chunk", [(32, 1024), (48, 0), (16, 1536), (32, 777)])
+def test_trace_tile_shapes_and_lane_order_are_results_neutral(G, chunk):
+    """Every directions-per-workgroup shape of the pooled trace, with chunks that are whole
+    lane-order blocks (height-sorted lanes) and one that is not (lanes in id order)."""
+    sc = synth_scene(12, 64, 64)
+    ref = oracle_lists(12, 64, 64)
+    az_alt, dni, dhi, _ = sky()
+    with sg.Context(0) as ctx:
+        ctx.set_scene(sc)
+        ctx.set_sky(az_alt, dni, dhi)
+        ctx.set_tuning(G, 64, 4, 0, 0)
+        if chunk:
+            ctx.set_chunk_voxels(chunk)
+        ctx.viewshed(sg.FMT_CSR)
+        off, ids = ctx.csr()
+        np.testing.assert_array_equal(off, ref["off"])
+        np.testing.assert_array_equal(ids, ref["ids"])
+        np.testing.assert_array_equal(ctx.svf(), ref["svf"])
+
+
 def test_trace_raw_matches_oracle():
     sc = synth_scene(12, 64, 64)
     bins = sky()[3]
