@@ -177,11 +177,6 @@ __device__ __forceinline__ bool pool_ray_steps(PoolRay& r, const float* __restri
     return done;
 }
 
-/* Grids addressed by 32-bit byte offsets with room for the column / row wrap-around tests. */
-__device__ __host__ __forceinline__ bool grid_off32(unsigned long long w, unsigned long long h) {
-    return w * (h + 2ull) <= (1ull << 30);
-}
-
 /* First hit (cell id + 1) of the ray from integer cell (gx, gy) at height
  * base_z in direction dc, or 0.  `steps` counts DDA steps when COUNT.  The
  * direction is the same for every lane of a wave, so the dispatch is uniform. */

@@ -296,6 +296,89 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(BatchTraceArgs
     }
 }
 
+/* The persistent multi-tile DDA on the pooled walk (every tile addressed by 32-bit byte
+ * offsets): per work item, each wave runs k_trace_pool's pool over its 64 voxels x 32
+ * directions, every ray with its own tile's heightmap, bounds and height limit. */
+__global__ __launch_bounds__(kTraceBlock) void k_trace_persistent_pool(BatchTraceArgs a) {
+    __shared__ long long item_s;
+    __shared__ int stile[kTraceBlock / 64][64];
+    __shared__ float sorg[kTraceBlock / 64][3][64];
+    constexpr int G = 32;
+    const int64_t groups = kDirs / G;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = (long long)atomicAdd(a.next, 1ull);
+        __syncthreads();   // also: the previous item's LDS reads are done
+        const int64_t item = item_s;
+        if (item >= a.items) break;   // every wave of the workgroup leaves together
+        const int64_t blk = item / groups, grp = item % groups;
+        const int64_t local = blk * kTraceBlock + threadIdx.x;   // raw row
+        const int nvalid = (int)min<int64_t>(64, max<int64_t>(0, a.count - (local - lane)));
+        if (lane < nvalid) {
+            const int64_t g = a.v0 + local;
+            int lo = 0, hi = a.ntiles - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (a.tiles[mid].vbase <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            const TileDesc& t = a.tiles[lo];
+            const int64_t v = g - t.vbase;
+            const int top = t.pix_parent[v];
+            const int gx = top % t.W, gy = top / t.W;
+            stile[wave][lane] = lo;
+            sorg[wave][0][lane] = (float)gx;
+            sorg[wave][1][lane] = (float)gy;
+            sorg[wave][2][lane] = (v < t.n_top) ? t.grid[top] : t.wall_z[v - t.n_top];
+        }
+        __syncthreads();
+        if (nvalid > 0) {
+            unsigned long long steps = 0;
+            bool active = false;
+            PoolRay ray{};
+            const float* grid = nullptr;
+            unsigned lim_x = 0, lim_y = 0;
+            int32_t* out = nullptr;
+            const int total = G * 64;
+            int next = 0;   // wave-uniform pool cursor
+            for (;;) {
+                const uint64_t idle = __ballot(!active);
+                const int nidle = __popcll(idle);
+                if (next < total && (nidle >= kRefill || nidle == 64)) {
+                    if (!active) {
+                        const int j = next + (int)__builtin_amdgcn_mbcnt_hi(
+                                                 (unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                        const int k = j >> 6, vx = j & 63;
+                        if (j < total && vx < nvalid) {
+                            const TileDesc& t = a.tiles[stile[wave][vx]];
+                            const unsigned uW = (unsigned)t.W;
+                            pool_ray_setup(ray, t.dirs[grp * G + k], sorg[wave][0][vx], sorg[wave][1][vx],
+                                           sorg[wave][2][vx], uW, t.maxz);
+                            grid = t.grid;
+                            lim_x = 4u * uW;
+                            lim_y = 4u * uW * (unsigned)t.H;
+                            out = a.raw + (local - lane + vx) * kDirs + grp * G + k;
+                            active = true;
+                        }
+                    }
+                    next += nidle;
+                }
+                if (__ballot(active) == 0) {
+                    if (next >= total) break;
+                    continue;
+                }
+                if (active) {
+                    unsigned res;
+                    if (pool_ray_steps<false>(ray, grid, lim_x, lim_y, steps, res)) {
+                        *out = (int32_t)res;
+                        active = false;
+                    }
+                }
+            }
+        }
+    }
+}
+
 /* ---- int32 counts -> int64 exclusive offsets ---- */
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
@@ -422,9 +505,13 @@ void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s) {
     SGPU_HIP(hipGetLastError());
 }
 
-void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s) {
+void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s, bool pooled) {
     if (a.count <= 0) return;
-    hipLaunchKernelGGL(k_trace_persistent, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s, a);
+    if (pooled)
+        hipLaunchKernelGGL(k_trace_persistent_pool, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s,
+                           a);
+    else
+        hipLaunchKernelGGL(k_trace_persistent, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s, a);
     SGPU_HIP(hipGetLastError());
 }
 

@@ -175,6 +175,8 @@ void viewshed(sgpu_batch* b) {
     }
     const int64_t N = b->n_all;
     const int64_t chunk = batch_chunk(b);
+    bool pooled = true;   // the pooled walk needs every tile within 32-bit byte offsets
+    for (const BatchTile& t : b->tiles) pooled = pooled && grid_off32((unsigned)t.W, (unsigned)t.H);
     TileDesc* tiles_d = nullptr;
     int32_t* raw = nullptr;
     int32_t* counts = nullptr;
@@ -201,7 +203,7 @@ void viewshed(sgpu_batch* b) {
             SGPU_HIP(hipMemsetAsync(next, 0, sizeof(unsigned long long), s));
             BatchTraceArgs ta{tiles_d, nt, v0, cnt, raw, next, items};
             SGPU_HIP(hipEventRecord(et.a, s));
-            launch_trace_persistent(ta, (int)std::min<int64_t>(items, (int64_t)b->cus * 8), s);
+            launch_trace_persistent(ta, (int)std::min<int64_t>(items, (int64_t)b->cus * 8), s, pooled);
             SGPU_HIP(hipEventRecord(et.b, s));
             BatchListArgs la{tiles_d, nt, raw, v0, cnt, counts, off, nullptr};
             SGPU_HIP(hipEventRecord(el.a, s));

@@ -83,6 +83,12 @@ constexpr int kTraceOrderBlock = 512;
 void synth_weather(uint64_t seed, const float* az_alt, float* dni, float* dhi);
 
 /* ---- device kernels (k_viewshed.hip, k_barle.hip, k_hourly.hip) ---- */
+/* Grids the DDA addresses by 32-bit byte offsets, with room for its column / row wrap-around
+   bounds tests (a step or two past the edge must not wrap into range). */
+__device__ __host__ inline bool grid_off32(unsigned long long w, unsigned long long h) {
+    return w * (h + 2ull) <= (1ull << 30);
+}
+
 struct TraceArgs {
     const float* grid;
     int W, H;
@@ -137,7 +143,8 @@ struct BatchTraceArgs {
     unsigned long long* next;   // work-item counter (zeroed before the launch)
     int64_t items;              // ceil(count / 256) blocks x (kDirs / 32) direction groups
 };
-void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s);
+/* pooled: every tile is addressed by 32-bit byte offsets (grid_off32), so the pooled walk applies */
+void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s, bool pooled);
 struct BatchListArgs {
     const TileDesc* tiles;
     int ntiles;
