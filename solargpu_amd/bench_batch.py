@@ -45,7 +45,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--tiles", type=int, default=64)
     ap.add_argument("--fmt", choices=("barle", "csr"), default="barle")
-    ap.add_argument("--hours-per-pass", type=int, default=64)
+    ap.add_argument("--hours-per-pass", type=int, default=128)
     ap.add_argument("--waves", type=int, default=4)
     ap.add_argument("--no-sequential", action="store_true")
     args = ap.parse_args(argv)
