@@ -42,8 +42,10 @@ struct BatchTile {
     DirConst* dirs = nullptr;
     HourConst* hours_d = nullptr;
     uint8_t* svf = nullptr;
+    int32_t* order = nullptr;   // reflection-pass processing order (32-cell tiles, walls after their cell)
 
     void release() {
+        dfree(order);
         dfree(grid);
         dfree(pix_parent);
         dfree(wall_z);
@@ -132,6 +134,7 @@ void add_tile(sgpu_batch* b, const float* grid, int w, int h, int64_t n_total, i
         t.dirs = dalloc<DirConst>(kDirs);
         t.hours_d = dalloc<HourConst>(std::max<size_t>(t.hours.size(), 1));
         t.svf = dalloc<uint8_t>((size_t)std::max(t.sky.svf_bytes, 1) * n_total);
+        t.order = dalloc<int32_t>(n_total);
         std::vector<DirConst> d(kDirs);
         directions(t.sky, d.data());
         hipStream_t s = b->stream;
@@ -144,6 +147,8 @@ void add_tile(sgpu_batch* b, const float* grid, int w, int h, int64_t n_total, i
         if (!t.hours.empty())
             SGPU_HIP(hipMemcpyAsync(t.hours_d, t.hours.data(), t.hours.size() * sizeof(HourConst),
                                     hipMemcpyHostToDevice, s));
+        const std::vector<int32_t> order = processing_order(w, h, n_total, pp, 32);
+        SGPU_HIP(hipMemcpyAsync(t.order, order.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice, s));
         SGPU_HIP(hipStreamSynchronize(s));
     } catch (...) {
         t.release();
@@ -275,6 +280,9 @@ void viewshed(sgpu_batch* b) {
     b->info.barle_bytes = b->br_size;
 }
 
+/* receiver group of the batched reflection pass (workgroups whose waves interleave receivers) */
+constexpr int kBatchGroup = 32;
+
 void hour_loop(sgpu_batch* b, unsigned fmt, int K, int waves, bool graph) {
     need(b->lists, SGPU_ERR_STATE, "run sgpu_batch_viewshed first");
     need(fmt == SGPU_FMT_CSR || fmt == SGPU_FMT_BARLE, SGPU_ERR_ARG, "format must be CSR or BARLE");
@@ -324,11 +332,15 @@ void hour_loop(sgpu_batch* b, unsigned fmt, int K, int waves, bool graph) {
         a.n_sub = t.n_total;   // positions = the tile's voxel ids
         a.v_base = 0;
         a.waves = waves;
+        /* the single-scene receiver assignment within every tile: tile-major order (voxel ids
+           index nb_off, sum and comp) and receiver groups, each tile's blocks whole groups */
+        a.order = t.order;
+        a.group = kBatchGroup;
         d[i].a = a;
         d[i].nsel = (int)t.hours.size();
         d[i].block0 = blocks;
         d[i].top_block0 = top_blocks;
-        blocks += (t.n_total + 64 * waves - 1) / (64 * waves);
+        blocks += ((t.n_total + 64 * waves - 1) / (64 * waves) + kBatchGroup - 1) / kBatchGroup * kBatchGroup;
         top_blocks += (t.n_top + 255) / 256;
         passes = std::max(passes, (d[i].nsel + K - 1) / K);
         vh += t.n_total * (int64_t)t.hours.size();
