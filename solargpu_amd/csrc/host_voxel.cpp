@@ -108,14 +108,40 @@ std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32
     }
     std::vector<int32_t> order;
     order.reserve((size_t)n_total);
-    for (int ty = 0; ty < h; ty += tile)
-        for (int tx = 0; tx < w; tx += tile)
-            for (int y = ty; y < std::min(h, ty + tile); ++y)
-                for (int x = tx; x < std::min(w, tx + tile); ++x) {
-                    const int64_t c = (int64_t)y * w + x;
-                    order.push_back((int32_t)c);
-                    for (int64_t k = start[c]; k < start[c + 1]; ++k) order.push_back(walls[(size_t)k]);
+    /* tiles along a Hilbert curve over the tile grid (side: the next power of two), so that
+       consecutive tiles are always neighbours */
+    const int tw = (w + tile - 1) / tile, th = (h + tile - 1) / tile;
+    int side = 1;
+    while (side < std::max(tw, th)) side <<= 1;
+    std::vector<std::pair<int, int>> tiles;
+    tiles.reserve((size_t)tw * th);
+    for (int64_t d = 0; d < (int64_t)side * side; ++d) {
+        int x = 0, y = 0;
+        int64_t t = d;
+        for (int s = 1; s < side; s <<= 1) {   // Hilbert index -> (x, y)
+            const int rx = (int)(1 & (t / 2)), ry = (int)(1 & (t ^ rx));
+            if (ry == 0) {
+                if (rx == 1) {
+                    x = s - 1 - x;
+                    y = s - 1 - y;
                 }
+                std::swap(x, y);
+            }
+            x += s * rx;
+            y += s * ry;
+            t /= 4;
+        }
+        if (x < tw && y < th) tiles.push_back({x, y});
+    }
+    for (const auto& tt : tiles) {
+        const int tx = tt.first * tile, ty = tt.second * tile;
+        for (int y = ty; y < std::min(h, ty + tile); ++y)
+            for (int x = tx; x < std::min(w, tx + tile); ++x) {
+                const int64_t c = (int64_t)y * w + x;
+                order.push_back((int32_t)c);
+                for (int64_t k = start[c]; k < start[c + 1]; ++k) order.push_back(walls[(size_t)k]);
+            }
+    }
     return order;
 }
 
