@@ -1,8 +1,6 @@
 /* Surrogate path kernels on gfx950 (no reference counterpart):
- *   k_sun_svf    the SVF bitmap from the ~265 sun-bin directions only (same DDA
- *                as the plain tracer, so the bits are identical), lane = voxel;
- *                k_sun_svf_pool: the same bits, idle lanes refilled with rays as
- *                in k_trace_pool (grids with 32-bit byte offsets);
+ *   k_sun_svf_pool  the SVF bitmap from the ~265 sun-bin directions only (the
+ *                plain tracer's pooled DDA, so the bits are identical);
  *   k_annual_dd  exact annual direct + diffuse per voxel (FP32 Kahan over all
  *                sunlit hours) and the diffuse part alone -- the solar kernel
  *                of solar_kernel.cu:54-77 without the reflection term;
@@ -30,36 +28,11 @@ __device__ __forceinline__ void voxel_origin(const SurArgs& a, int64_t v, int& t
     z = (v < a.n_top) ? a.grid[top] : a.wall_z[v - a.n_top];
 }
 
-__global__ __launch_bounds__(256) void k_sun_svf(SurArgs a) {
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= a.n_total) return;
-    int top, gx, gy;
-    float z;
-    voxel_origin(a, v, top, gx, gy, z);
-    unsigned long long steps = 0;
-    int hits = 0;
-    for (int byte = 0; byte < a.svf_bytes; ++byte) {
-        unsigned bits = 0;
-        for (int bit = 0; bit < 8; ++bit) {
-            const int dense = byte * 8 + bit;
-            if (dense >= a.n_sun_dirs) break;
-            const DirConst dc = a.dirs[a.sun_dirs[dense]];   // wave-uniform
-            if (dda_first_hit<false>(a.grid, a.W, (unsigned)a.W, (unsigned)a.H, a.maxz, (float)gx, (float)gy, z, dc,
-                                     steps)) {
-                bits |= 1u << bit;
-                ++hits;
-            }
-        }
-        a.svf[(int64_t)byte * a.n_total + v] = (uint8_t)bits;
-    }
-    a.sun_hits[v] = (uint16_t)hits;
-}
-
-/* Pooled sun-bin SVF (grids with 32-bit byte offsets): each wave owns the n_sun_dirs x 64 rays
+/* Sun-bin SVF on the pooled walk (dda.h): each wave owns the n_sun_dirs x 64 rays
  * of its 64 voxels, direction-major; a lane whose ray has ended takes the next one once
  * kSunRefill lanes are idle (or none is busy).  A hit sets the voxel's bit in LDS words (bit d
  * of word d / 32 is SVF byte d / 8, bit d % 8); the bytes and the hit count are written at
- * the end.  The rays are dda_walk's (pool_ray_*), so the bits equal k_sun_svf's. */
+ * the end.  The rays are the plain viewshed's, so the bits equal its SVF. */
 constexpr int kSunRefill = 16;
 
 __global__ __launch_bounds__(256) void k_sun_svf_pool(SurArgs a) {
@@ -319,13 +292,10 @@ __global__ __launch_bounds__(256) void k_compact(const int32_t* flags, const int
 
 void launch_sun_svf(const SurArgs& a, hipStream_t s) {
     const dim3 g((unsigned)((a.n_total + 255) / 256));
-    if (grid_off32((unsigned)a.W, (unsigned)a.H)) {
-        if (a.n_sun_dirs > kAzBins * kElBins) fail(SGPU_ERR_ARG, "more sun bins than upper directions");
-        const size_t lds = (size_t)4 * 64 * ((a.n_sun_dirs + 31) / 32) * sizeof(unsigned);
-        hipLaunchKernelGGL(k_sun_svf_pool, g, dim3(256), lds, s, a);
-    } else {
-        hipLaunchKernelGGL(k_sun_svf, g, dim3(256), 0, s, a);
-    }
+    if (!grid_off32((unsigned)a.W, (unsigned)a.H)) fail(SGPU_ERR_ARG, "grid beyond 32-bit byte offsets");
+    if (a.n_sun_dirs > kAzBins * kElBins) fail(SGPU_ERR_ARG, "more sun bins than upper directions");
+    const size_t lds = (size_t)4 * 64 * ((a.n_sun_dirs + 31) / 32) * sizeof(unsigned);
+    hipLaunchKernelGGL(k_sun_svf_pool, g, dim3(256), lds, s, a);
     SGPU_HIP(hipGetLastError());
 }
 void launch_annual_dd(const SurArgs& a, hipStream_t s) {

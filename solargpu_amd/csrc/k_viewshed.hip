@@ -1,11 +1,11 @@
 /* Stage 1 on gfx950: hemispherical DDA viewshed, SVF bitmap and CSR
  * neighbour lists.
  *
- * k_trace   one lane per voxel, one workgroup per (256-voxel block,
- *           32-direction group).  Lanes of a wave trace the same direction
- *           from 64 consecutive voxels, so their height-grid reads walk
- *           neighbouring cells together.  First-hit ids go to a per-chunk raw
- *           table raw[voxel][direction] (int32, 0 = no hit).
+ * k_trace_pool  one workgroup per (256-voxel block, G-direction group); each
+ *           wave traces the G x 64 rays of its 64 voxels on the pooled walk of
+ *           dda.h (a lane takes a new ray when its own has ended).  First-hit
+ *           ids go to a per-chunk raw table raw[voxel][direction] (int32,
+ *           0 = no hit).
  * k_lists   one workgroup per voxel: LDS hash set of the voxel's hit cells
  *           with the smallest direction index per cell (LDS atomics), then an
  *           ordered block scan keeps the first occurrence of every cell in
@@ -16,7 +16,7 @@
  *           every row's list to its offset (one wave per voxel).
  *
  * The DDA keeps the reference's operation order (viewshed_kernels.cu:27-58);
- * the library is compiled with -ffp-contract=off and IEEE division. */
+ * the library is compiled with -ffp-contract=off. */
 #include <climits>
 
 #include "dda.h"
@@ -30,36 +30,13 @@ constexpr int kListThreads = 256;
 constexpr int kDirsPerThread = (kDirs + kListThreads - 1) / kListThreads;  // 11
 constexpr int kHashSize = 4096;              // > 1.5 x the 2592 possible distinct hits
 
-/* G directions per workgroup (2592 = 162 x 16 = 81 x 32 = 54 x 48): the tile shape the autotuner picks. */
-template <bool COUNT_STEPS, int G>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
-    const int64_t lane_pos = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
-    if (lane_pos >= a.count) return;
-    const int64_t local = a.perm ? (int64_t)a.perm[a.v0 + lane_pos] - a.v0 : lane_pos;   // raw row
-    const int64_t v = a.subset ? (int64_t)a.subset[a.v0 + local] : a.v0 + local;
-    const int top = a.pix_parent[v];
-    const int gx = top % a.W, gy = top / a.W;
-    const float base_z = (v < a.n_top) ? a.grid[gx + gy * a.W] : a.wall_z[v - a.n_top];
-    const float gxf = (float)gx, gyf = (float)gy;
-    const unsigned W = (unsigned)a.W, H = (unsigned)a.H;
-    const float* __restrict__ grid = a.grid;
-    int32_t* out = a.raw + local * kDirs + blockIdx.y * G;
-    unsigned long long steps = 0;
-
-    for (int k = 0; k < G; ++k) {
-        const DirConst dc = a.dirs[blockIdx.y * G + k];   // wave-uniform
-        out[k] = dda_first_hit<COUNT_STEPS>(grid, a.W, W, H, a.maxz, gxf, gyf, base_z, dc, steps);
-    }
-    if (COUNT_STEPS) atomicAdd(a.steps, steps);
-}
-
-/* Pooled DDA (grids with 32-bit byte offsets, see dda.h).  Each wave owns the G x 64 rays of
- * its 64 voxels (the k_trace lane order) over the workgroup's G directions, taken
+/* G directions per workgroup (2592 = 162 x 16 = 81 x 32 = 54 x 48): the tile shape the autotuner
+ * picks.  Each wave owns the G x 64 rays of its 64 voxels (in the lane order: blocks of ids
+ * sorted by height when a.perm is given) over the workgroup's G directions, taken
  * direction-major (the 64 voxels of one direction, then the next).  A lane whose ray has
  * ended takes the next ray of the pool once kRefill lanes are idle (or none is busy), so the
- * wave no longer waits for its longest ray of every direction; a refill sets up the rays of
- * all idle lanes at once.  The hemisphere and step signs become per-lane values; the
- * arithmetic of a ray is dda_walk's, so the hits are identical. */
+ * wave does not wait for its longest ray of every direction; a refill sets up the rays of
+ * all idle lanes at once. */
 constexpr int kRefill = 16;
 
 template <bool COUNT_STEPS, int G>
@@ -258,47 +235,11 @@ void launch_list_copy(const int32_t* raw, const int32_t* counts, const int64_t* 
     hipLaunchKernelGGL(k_list_copy, dim3(g), dim3(64 * kCopyWaves), 0, s, raw, counts, offsets, ids, count);
 }
 
-/* Persistent multi-tile DDA: every workgroup takes (256-voxel block, 32-direction group)
- * items from a global counter until the chunk is exhausted; each lane finds its voxel's
- * tile (tiles are contiguous in the global voxel order) and traces with that tile's grid. */
-__global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(BatchTraceArgs a) {
-    __shared__ long long item_s;
-    constexpr int G = 32;
-    const int64_t groups = kDirs / G;
-    for (;;) {
-        if (threadIdx.x == 0) item_s = (long long)atomicAdd(a.next, 1ull);
-        __syncthreads();
-        const int64_t item = item_s;
-        __syncthreads();
-        if (item >= a.items) break;   // every wave of the workgroup leaves together
-        const int64_t blk = item / groups, grp = item % groups;
-        const int64_t local = blk * kTraceBlock + threadIdx.x;
-        if (local >= a.count) continue;
-        const int64_t g = a.v0 + local;
-        int lo = 0, hi = a.ntiles - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (a.tiles[mid].vbase <= g) lo = mid;
-            else hi = mid - 1;
-        }
-        const TileDesc& t = a.tiles[lo];
-        const int64_t v = g - t.vbase;
-        const int top = t.pix_parent[v];
-        const int gx = top % t.W, gy = top / t.W;
-        const float base_z = (v < t.n_top) ? t.grid[top] : t.wall_z[v - t.n_top];
-        unsigned long long steps = 0;
-        int32_t* out = a.raw + local * kDirs + grp * G;
-        for (int k = 0; k < G; ++k) {
-            const DirConst dc = t.dirs[grp * G + k];
-            out[k] = dda_first_hit<false>(t.grid, t.W, (unsigned)t.W, (unsigned)t.H, t.maxz, (float)gx, (float)gy,
-                                          base_z, dc, steps);
-        }
-    }
-}
-
-/* The persistent multi-tile DDA on the pooled walk (every tile addressed by 32-bit byte
- * offsets): per work item, each wave runs k_trace_pool's pool over its 64 voxels x 32
- * directions, every ray with its own tile's heightmap, bounds and height limit. */
+/* Persistent multi-tile DDA: every workgroup takes (256-voxel block, 32-direction group) items
+ * from a global counter until the chunk is exhausted; per item each wave runs k_trace_pool's
+ * pool over its 64 voxels x 32 directions, every ray with its own tile's heightmap, bounds and
+ * height limit (tiles are contiguous in the global voxel order; a binary search finds each
+ * voxel's tile). */
 __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent_pool(BatchTraceArgs a) {
     __shared__ long long item_s;
     __shared__ int stile[kTraceBlock / 64][64];
@@ -457,40 +398,22 @@ __global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int32_t* c, in
 
 void launch_trace(const TraceArgs& a, hipStream_t s) {
     if (a.count <= 0) return;
+    if (!grid_off32((unsigned)a.W, (unsigned)a.H)) fail(SGPU_ERR_ARG, "grid beyond 32-bit byte offsets");
     const int G = a.group ? a.group : 32;
     const unsigned bx = (unsigned)((a.count + kTraceBlock - 1) / kTraceBlock);
     dim3 grid(bx, (unsigned)(kDirs / G));
-    if (grid_off32((unsigned)a.W, (unsigned)a.H)) {   // 32-bit byte offsets: the pooled walk
-        switch (G) {
-            case 16:
-                if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 16>), grid, dim3(kTraceBlock), 0, s, a);
-                else hipLaunchKernelGGL((k_trace_pool<false, 16>), grid, dim3(kTraceBlock), 0, s, a);
-                break;
-            case 32:
-                if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 32>), grid, dim3(kTraceBlock), 0, s, a);
-                else hipLaunchKernelGGL((k_trace_pool<false, 32>), grid, dim3(kTraceBlock), 0, s, a);
-                break;
-            case 48:
-                if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 48>), grid, dim3(kTraceBlock), 0, s, a);
-                else hipLaunchKernelGGL((k_trace_pool<false, 48>), grid, dim3(kTraceBlock), 0, s, a);
-                break;
-            default: fail(SGPU_ERR_ARG, "directions per group must be 16, 32 or 48");
-        }
-        SGPU_HIP(hipGetLastError());
-        return;
-    }
     switch (G) {
         case 16:
-            if (a.steps) hipLaunchKernelGGL((k_trace<true, 16>), grid, dim3(kTraceBlock), 0, s, a);
-            else hipLaunchKernelGGL((k_trace<false, 16>), grid, dim3(kTraceBlock), 0, s, a);
+            if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 16>), grid, dim3(kTraceBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_trace_pool<false, 16>), grid, dim3(kTraceBlock), 0, s, a);
             break;
         case 32:
-            if (a.steps) hipLaunchKernelGGL((k_trace<true, 32>), grid, dim3(kTraceBlock), 0, s, a);
-            else hipLaunchKernelGGL((k_trace<false, 32>), grid, dim3(kTraceBlock), 0, s, a);
+            if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 32>), grid, dim3(kTraceBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_trace_pool<false, 32>), grid, dim3(kTraceBlock), 0, s, a);
             break;
         case 48:
-            if (a.steps) hipLaunchKernelGGL((k_trace<true, 48>), grid, dim3(kTraceBlock), 0, s, a);
-            else hipLaunchKernelGGL((k_trace<false, 48>), grid, dim3(kTraceBlock), 0, s, a);
+            if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 48>), grid, dim3(kTraceBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_trace_pool<false, 48>), grid, dim3(kTraceBlock), 0, s, a);
             break;
         default: fail(SGPU_ERR_ARG, "directions per group must be 16, 32 or 48");
     }
@@ -505,13 +428,9 @@ void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s) {
     SGPU_HIP(hipGetLastError());
 }
 
-void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s, bool pooled) {
+void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s) {
     if (a.count <= 0) return;
-    if (pooled)
-        hipLaunchKernelGGL(k_trace_persistent_pool, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s,
-                           a);
-    else
-        hipLaunchKernelGGL(k_trace_persistent, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s, a);
+    hipLaunchKernelGGL(k_trace_persistent_pool, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s, a);
     SGPU_HIP(hipGetLastError());
 }
 

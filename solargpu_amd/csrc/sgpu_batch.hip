@@ -105,7 +105,8 @@ void add_tile(sgpu_batch* b, const float* grid, int w, int h, int64_t n_total, i
               const float* wall_z, float maxz, const float* slopes, const float* aspects, const float* az_alt,
               const float* dni, const float* dhi, int32_t* index) {
     need(grid && pp && slopes && aspects && az_alt && dni && dhi, SGPU_ERR_ARG, "null argument");
-    need(w > 0 && h > 0 && (int64_t)w * h < (int64_t)1 << 31, SGPU_ERR_ARG, "grid size out of range");
+    need(w > 0 && h > 0 && grid_off32((unsigned)w, (unsigned)h), SGPU_ERR_ARG,
+         "grid size out of range (width x (height + 2) must not exceed 2^30 cells)");
     need(n_top == (int64_t)w * h, SGPU_ERR_ARG, "num_top_voxels must equal grid_width * grid_height");
     need(n_total >= n_top && n_total < (int64_t)1 << 31, SGPU_ERR_ARG, "num_total_voxels out of range");
     need(n_total == n_top || wall_z != nullptr, SGPU_ERR_ARG, "wall_z required when there are wall voxels");
@@ -180,8 +181,6 @@ void viewshed(sgpu_batch* b) {
     }
     const int64_t N = b->n_all;
     const int64_t chunk = batch_chunk(b);
-    bool pooled = true;   // the pooled walk needs every tile within 32-bit byte offsets
-    for (const BatchTile& t : b->tiles) pooled = pooled && grid_off32((unsigned)t.W, (unsigned)t.H);
     TileDesc* tiles_d = nullptr;
     int32_t* raw = nullptr;
     int32_t* counts = nullptr;
@@ -208,7 +207,7 @@ void viewshed(sgpu_batch* b) {
             SGPU_HIP(hipMemsetAsync(next, 0, sizeof(unsigned long long), s));
             BatchTraceArgs ta{tiles_d, nt, v0, cnt, raw, next, items};
             SGPU_HIP(hipEventRecord(et.a, s));
-            launch_trace_persistent(ta, (int)std::min<int64_t>(items, (int64_t)b->cus * 8), s, pooled);
+            launch_trace_persistent(ta, (int)std::min<int64_t>(items, (int64_t)b->cus * 8), s);
             SGPU_HIP(hipEventRecord(et.b, s));
             BatchListArgs la{tiles_d, nt, raw, v0, cnt, counts, off, nullptr};
             SGPU_HIP(hipEventRecord(el.a, s));

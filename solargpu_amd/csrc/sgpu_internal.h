@@ -84,7 +84,8 @@ void synth_weather(uint64_t seed, const float* az_alt, float* dni, float* dhi);
 
 /* ---- device kernels (k_viewshed.hip, k_barle.hip, k_hourly.hip) ---- */
 /* Grids the DDA addresses by 32-bit byte offsets, with room for its column / row wrap-around
-   bounds tests (a step or two past the edge must not wrap into range). */
+   bounds tests (a step or two past the edge must not wrap into range): the supported grids
+   (up to 32766 x 32766 cells; voxel counts are below 2^31 anyway). */
 __device__ __host__ inline bool grid_off32(unsigned long long w, unsigned long long h) {
     return w * (h + 2ull) <= (1ull << 30);
 }
@@ -143,8 +144,7 @@ struct BatchTraceArgs {
     unsigned long long* next;   // work-item counter (zeroed before the launch)
     int64_t items;              // ceil(count / 256) blocks x (kDirs / 32) direction groups
 };
-/* pooled: every tile is addressed by 32-bit byte offsets (grid_off32), so the pooled walk applies */
-void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s, bool pooled);
+void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s);
 struct BatchListArgs {
     const TileDesc* tiles;
     int ntiles;

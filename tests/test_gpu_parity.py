@@ -90,6 +90,18 @@ def test_c1_configuration_end_to_end_vs_oracle():
     assert _rel(irr, exact).max() <= ANNUAL_RTOL
 
 
+def test_grid_beyond_32bit_byte_offsets_is_rejected():
+    """Grids with W (H + 2) > 2^30 are refused with a status code before any array is read."""
+    import ctypes
+    g = np.zeros(4, np.float32)
+    pp = np.zeros(4, np.int32)
+    with sg.Context(0) as ctx:
+        for w, h in ((32768, 32767), (1 << 20, 1023)):
+            rc = sg._lib.sgpu_set_scene(ctx._h, sg._ptr(g), w, h, ctypes.c_int64(w * h), ctypes.c_int64(w * h),
+                                        sg._ptr(pp), None, ctypes.c_float(1.0), sg._ptr(g), sg._ptr(g))
+            assert rc != 0 and b"2^30" in sg._lib.sgpu_last_error()
+
+
 def test_trace_raw_matches_oracle():
     sc = synth_scene(12, 64, 64)
     bins = sky()[3]
