@@ -102,6 +102,25 @@ def test_grid_beyond_32bit_byte_offsets_is_rejected():
             assert rc != 0 and b"2^30" in sg._lib.sgpu_last_error()
 
 
+def test_hourly_buffer_beyond_4gib_matches_32bit_rows():
+    """2048 x 2100 cells at K = 256 hours per pass need 4.4 GB of hourly rows, addressed with
+    64-bit offsets; K = 128 (2.2 GB) uses 32-bit row offsets.  The annual sums agree to FP32
+    regrouping of the hours (well inside the 1e-4 bar)."""
+    sc = synth_scene(20260203, 2048, 2100, 0.5, 0.45, 40)
+    assert sc.num_top * 256 * 4 > (1 << 32) > sc.num_top * 128 * 4
+    az_alt, dni, dhi, _ = sky()
+    with sg.Context(0) as ctx:
+        ctx.set_scene(sc)
+        ctx.set_sky(az_alt, dni, dhi)
+        ctx.viewshed(sg.FMT_BARLE)
+        ctx.set_tuning(32, 128, 4, 256, 32)
+        ctx.hour_loop(sg.FMT_BARLE, 128, 8)
+        narrow = ctx.irradiance()
+        ctx.hour_loop(sg.FMT_BARLE, 256, 8)
+        wide = ctx.irradiance()
+    assert _rel(wide, narrow).max() <= 1e-5
+
+
 def test_trace_raw_matches_oracle():
     sc = synth_scene(12, 64, 64)
     bins = sky()[3]
