@@ -120,6 +120,16 @@ int sgpu_create(int device, sgpu_ctx** out);
 int sgpu_destroy(sgpu_ctx* ctx);
 int sgpu_get_info(sgpu_ctx* ctx, sgpu_info* out);
 
+/* Set the scene from a heightmap, voxelized on the GPU: the wall voxels and slopes/aspects of
+   src/voxelization.cpp:10-97 (detect_wall_voxels, compute_top_slope_aspect) and the assembly of
+   src/main.cpp:112-127 (tops in raster order, then walls; voxel size = scale_x; max_height = the
+   heightmap's maximum), in one call with the arrays left on the device.  Parents and wall heights
+   equal sgpu_assemble_voxels' bit for bit; slopes and aspects agree to device atan2f rounding. */
+int sgpu_set_scene_heights(sgpu_ctx* ctx, const float* heights, int grid_width, int grid_height,
+                           double scale_x, double scale_y);
+/* The context's voxel arrays (num_total entries; wall_z num_total - num_top); any may be NULL. */
+int sgpu_get_voxels(sgpu_ctx* ctx, int32_t* pixel_parent, float* slopes, float* aspects, float* wall_z);
+
 /* Upload the scene.  wall_z may be NULL when num_total == num_top. */
 int sgpu_set_scene(sgpu_ctx* ctx, const float* grid, int grid_width, int grid_height, int64_t num_total,
                    int64_t num_top, const int32_t* pixel_parent, const float* wall_z, float max_height,

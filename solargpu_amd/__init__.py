@@ -295,6 +295,24 @@ class Context:
                                    _I64(sc.num_top), _ptr(sc.pixel_parent), _ptr(sc.wall_z) if sc.wall_z.size else None,
                                    _F(sc.max_height), _ptr(sc.slopes), _ptr(sc.aspects)))
 
+    def set_scene_heights(self, heights: np.ndarray, cell: float, cell_y: float | None = None) -> None:
+        """Set the scene from a heightmap voxelized on the GPU (the arrays stay on the device)."""
+        h = _c(heights, np.float32)
+        self._scene = None
+        _check(_lib.sgpu_set_scene_heights(self._h, _ptr(h), h.shape[1], h.shape[0], ctypes.c_double(cell),
+                                           ctypes.c_double(cell if cell_y is None else cell_y)))
+
+    def voxels(self, heights: np.ndarray, cell: float) -> Scene:
+        """The context's voxel arrays as a Scene (heights and cell as given to set_scene_heights)."""
+        h = _c(heights, np.float32)
+        n = self.info()["num_total"]
+        pp = np.zeros(n, np.int32)
+        sl = np.zeros(n, np.float32)
+        asp = np.zeros(n, np.float32)
+        wz = np.zeros(max(n - h.size, 0), np.float32)
+        _check(_lib.sgpu_get_voxels(self._h, _ptr(pp), _ptr(sl), _ptr(asp), _ptr(wz) if wz.size else None))
+        return Scene(h, pp, sl, asp, wz, float(h.max()), float(cell))
+
     def set_sky(self, az_alt: np.ndarray, dni: np.ndarray, dhi: np.ndarray) -> None:
         self._az_alt = _c(az_alt, np.float32)
         _check(_lib.sgpu_set_sky(self._h, _ptr(self._az_alt), _ptr(_c(dni, np.float32)), _ptr(_c(dhi, np.float32))))

@@ -933,6 +933,23 @@ int sgpu_get_info(sgpu_ctx* c, sgpu_info* out) {
     });
 }
 
+namespace {
+
+/* The reflection-pass processing order and the DDA lane order of the context's scene, built on
+   the host from its heightmap, parents and wall heights, then uploaded. */
+void upload_orders(sgpu_ctx* c, const float* grid, const int32_t* pp, const float* wall_z) {
+    const std::vector<int32_t> order = processing_order(c->W, c->H, c->n_total, pp, 32);
+    c->order = dalloc<int32_t>(c->n_total);
+    SGPU_HIP(hipMemcpyAsync(c->order, order.data(), c->n_total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    const std::vector<int32_t> tperm = trace_order(c->n_total, grid, pp, wall_z, c->n_top, kTraceOrderBlock);
+    c->trace_perm = dalloc<int32_t>(c->n_total);
+    SGPU_HIP(hipMemcpyAsync(c->trace_perm, tperm.data(), c->n_total * sizeof(int32_t), hipMemcpyHostToDevice,
+                            c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
 int sgpu_set_scene(sgpu_ctx* c, const float* grid, int w, int h, int64_t n_total, int64_t n_top, const int32_t* pp,
                    const float* wall_z, float maxz, const float* slopes, const float* aspects) {
     return guarded([&] {
@@ -964,15 +981,70 @@ int sgpu_set_scene(sgpu_ctx* c, const float* grid, int w, int h, int64_t n_total
         SGPU_HIP(hipMemcpyAsync(c->aspects, aspects, n_total * sizeof(float), hipMemcpyHostToDevice, c->stream));
         if (nw > 0)
             SGPU_HIP(hipMemcpyAsync(c->wall_z, wall_z, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        const std::vector<int32_t> order = processing_order(w, h, n_total, pp, 32);
-        c->order = dalloc<int32_t>(n_total);
-        SGPU_HIP(hipMemcpyAsync(c->order, order.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-        const std::vector<int32_t> tperm = trace_order(n_total, grid, pp, wall_z, n_top, kTraceOrderBlock);
-        c->trace_perm = dalloc<int32_t>(n_total);
-        SGPU_HIP(hipMemcpyAsync(c->trace_perm, tperm.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice,
-                                c->stream));
-        SGPU_HIP(hipStreamSynchronize(c->stream));
+        upload_orders(c, grid, pp, wall_z);
         c->scene = true;
+    });
+}
+
+int sgpu_set_scene_heights(sgpu_ctx* c, const float* heights, int w, int h, double scale_x, double scale_y) {
+    return guarded([&] {
+        need(c && heights, SGPU_ERR_ARG, "null argument");
+        need(w > 0 && h > 0 && grid_off32((unsigned)w, (unsigned)h), SGPU_ERR_ARG,
+             "grid size out of range (width x (height + 2) must not exceed 2^30 cells)");
+        need(scale_x > 0.0 && scale_y > 0.0, SGPU_ERR_ARG, "pixel scales must be positive");
+        c->set_device();
+        c->drop_products();
+        c->drop_scene();
+        const int64_t n_top = (int64_t)w * h;
+        float maxz = heights[0];   // the reference's max_height: the heightmap's maximum
+        for (int64_t i = 1; i < n_top; ++i) maxz = std::max(maxz, heights[i]);
+        c->grid = dalloc<float>(n_top);
+        SGPU_HIP(hipMemcpyAsync(c->grid, heights, n_top * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        DeviceVoxels dv;
+        int64_t n_total = 0;
+        try {
+            n_total = voxelize_device(c->stream, c->grid, w, h, scale_x, scale_y, &dv);
+        } catch (...) {
+            dfree(dv.pix_parent);
+            dfree(dv.slopes);
+            dfree(dv.aspects);
+            dfree(dv.wall_z);
+            throw;
+        }
+        c->pix_parent = dv.pix_parent;
+        c->slopes = dv.slopes;
+        c->aspects = dv.aspects;
+        c->wall_z = dv.wall_z;
+        c->W = w;
+        c->H = h;
+        c->n_total = n_total;
+        c->n_top = n_top;
+        c->maxz = maxz;
+        /* the processing orders are built on the host from the parents and wall heights */
+        std::vector<int32_t> pp((size_t)n_total);
+        std::vector<float> wz((size_t)std::max<int64_t>(n_total - n_top, 1));
+        SGPU_HIP(hipMemcpyAsync(pp.data(), c->pix_parent, n_total * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                c->stream));
+        SGPU_HIP(hipMemcpyAsync(wz.data(), c->wall_z, wz.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        upload_orders(c, heights, pp.data(), wz.data());
+        c->scene = true;
+    });
+}
+
+int sgpu_get_voxels(sgpu_ctx* c, int32_t* pixel_parent, float* slopes, float* aspects, float* wall_z) {
+    return guarded([&] {
+        need(c, SGPU_ERR_ARG, "null context");
+        need(c->scene, SGPU_ERR_STATE, "scene not set");
+        c->set_device();
+        const size_t n = (size_t)c->n_total, nw = (size_t)(c->n_total - c->n_top);
+        if (pixel_parent)
+            SGPU_HIP(hipMemcpyAsync(pixel_parent, c->pix_parent, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        if (slopes) SGPU_HIP(hipMemcpyAsync(slopes, c->slopes, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        if (aspects) SGPU_HIP(hipMemcpyAsync(aspects, c->aspects, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        if (wall_z && nw)
+            SGPU_HIP(hipMemcpyAsync(wall_z, c->wall_z, nw * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
     });
 }
 

@@ -156,6 +156,19 @@ struct BatchListArgs {
 };
 void launch_lists_batch(const BatchListArgs& a, bool write_pass, hipStream_t s);
 
+/* Voxel arrays built on the device from a heightmap (k_voxelize.hip): num_total entries each,
+   wall_z num_total - num_top (at least one).  The caller owns them (hipFree). */
+struct DeviceVoxels {
+    int32_t* pix_parent = nullptr;
+    float* slopes = nullptr;
+    float* aspects = nullptr;
+    float* wall_z = nullptr;
+};
+/* The voxels of the w x h heightmap heights_dev (device) with pixel scales sx, sy (the voxel size
+   is sx); returns num_total. */
+int64_t voxelize_device(hipStream_t s, const float* heights_dev, int w, int h, double sx, double sy,
+                        DeviceVoxels* out);
+
 /* Exclusive scan of n int32 counts into n+1 int64 offsets (offsets[n] = total). */
 void launch_scan_counts(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* scratch_blocks,
                         hipStream_t s);
