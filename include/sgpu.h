@@ -123,10 +123,16 @@ int sgpu_get_info(sgpu_ctx* ctx, sgpu_info* out);
 /* Set the scene from a heightmap, voxelized on the GPU: the wall voxels and slopes/aspects of
    src/voxelization.cpp:10-97 (detect_wall_voxels, compute_top_slope_aspect) and the assembly of
    src/main.cpp:112-127 (tops in raster order, then walls; voxel size = scale_x; max_height = the
-   heightmap's maximum), in one call with the arrays left on the device.  Parents and wall heights
-   equal sgpu_assemble_voxels' bit for bit; slopes and aspects agree to device atan2f rounding. */
+   heightmap's maximum), in one call with the arrays left on the device; the processing orders are
+   built there too.  Parents and wall heights equal sgpu_assemble_voxels' bit for bit; slopes and
+   aspects agree to the device atan2f's rounding (3 ulp). */
 int sgpu_set_scene_heights(sgpu_ctx* ctx, const float* heights, int grid_width, int grid_height,
                            double scale_x, double scale_y);
+/* The context's processing orders (num_total entries each; either may be NULL): the reflection
+   pass's (Hilbert-ordered 32-cell tiles, each top voxel followed by its walls) and the DDA's lane
+   order (blocks of 512 ids sorted by parent-cell height, voxel height).  No reference counterpart;
+   for tests and tooling. */
+int sgpu_get_orders(sgpu_ctx* ctx, int32_t* order, int32_t* lane_order);
 /* The context's voxel arrays (num_total entries; wall_z num_total - num_top); any may be NULL. */
 int sgpu_get_voxels(sgpu_ctx* ctx, int32_t* pixel_parent, float* slopes, float* aspects, float* wall_z);
 

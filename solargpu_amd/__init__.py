@@ -302,6 +302,14 @@ class Context:
         _check(_lib.sgpu_set_scene_heights(self._h, _ptr(h), h.shape[1], h.shape[0], ctypes.c_double(cell),
                                            ctypes.c_double(cell if cell_y is None else cell_y)))
 
+    def orders(self) -> tuple[np.ndarray, np.ndarray]:
+        """(reflection-pass processing order, DDA lane order) of the context's scene."""
+        n = self.info()["num_total"]
+        order = np.zeros(n, np.int32)
+        lanes = np.zeros(n, np.int32)
+        _check(_lib.sgpu_get_orders(self._h, _ptr(order), _ptr(lanes)))
+        return order, lanes
+
     def voxels(self, heights: np.ndarray, cell: float) -> Scene:
         """The context's voxel arrays as a Scene (heights and cell as given to set_scene_heights)."""
         h = _c(heights, np.float32)

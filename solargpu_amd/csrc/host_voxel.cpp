@@ -95,22 +95,7 @@ std::vector<int32_t> trace_order(int64_t n_total, const float* grid, const int32
     return perm;
 }
 
-std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32_t* pp, int tile) {
-    const int64_t n_top = (int64_t)w * h;
-    /* walls grouped by parent cell, stable (counting sort) */
-    std::vector<int64_t> start(n_top + 1, 0);
-    for (int64_t v = n_top; v < n_total; ++v) ++start[(int64_t)pp[v] + 1];
-    for (int64_t c = 0; c < n_top; ++c) start[c + 1] += start[c];
-    std::vector<int32_t> walls((size_t)(n_total - n_top));
-    {
-        std::vector<int64_t> fill(start.begin(), start.end() - 1);
-        for (int64_t v = n_top; v < n_total; ++v) walls[(size_t)fill[pp[v]]++] = (int32_t)v;
-    }
-    std::vector<int32_t> order;
-    order.reserve((size_t)n_total);
-    /* tiles along a Hilbert curve over the tile grid (side: the next power of two), so that
-       consecutive tiles are always neighbours */
-    const int tw = (w + tile - 1) / tile, th = (h + tile - 1) / tile;
+std::vector<std::pair<int, int>> hilbert_tiles(int tw, int th) {
     int side = 1;
     while (side < std::max(tw, th)) side <<= 1;
     std::vector<std::pair<int, int>> tiles;
@@ -133,6 +118,23 @@ std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32
         }
         if (x < tw && y < th) tiles.push_back({x, y});
     }
+    return tiles;
+}
+
+std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32_t* pp, int tile) {
+    const int64_t n_top = (int64_t)w * h;
+    /* walls grouped by parent cell, stable (counting sort) */
+    std::vector<int64_t> start(n_top + 1, 0);
+    for (int64_t v = n_top; v < n_total; ++v) ++start[(int64_t)pp[v] + 1];
+    for (int64_t c = 0; c < n_top; ++c) start[c + 1] += start[c];
+    std::vector<int32_t> walls((size_t)(n_total - n_top));
+    {
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t v = n_top; v < n_total; ++v) walls[(size_t)fill[pp[v]]++] = (int32_t)v;
+    }
+    std::vector<int32_t> order;
+    order.reserve((size_t)n_total);
+    const std::vector<std::pair<int, int>> tiles = hilbert_tiles((w + tile - 1) / tile, (h + tile - 1) / tile);
     for (const auto& tt : tiles) {
         const int tx = tt.first * tile, ty = tt.second * tile;
         for (int y = ty; y < std::min(h, ty + tile); ++y)

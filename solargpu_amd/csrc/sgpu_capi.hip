@@ -1009,6 +1009,7 @@ int sgpu_set_scene_heights(sgpu_ctx* c, const float* heights, int w, int h, doub
             dfree(dv.slopes);
             dfree(dv.aspects);
             dfree(dv.wall_z);
+            dfree(dv.wall_off);
             throw;
         }
         c->pix_parent = dv.pix_parent;
@@ -1020,15 +1021,30 @@ int sgpu_set_scene_heights(sgpu_ctx* c, const float* heights, int w, int h, doub
         c->n_total = n_total;
         c->n_top = n_top;
         c->maxz = maxz;
-        /* the processing orders are built on the host from the parents and wall heights */
-        std::vector<int32_t> pp((size_t)n_total);
-        std::vector<float> wz((size_t)std::max<int64_t>(n_total - n_top, 1));
-        SGPU_HIP(hipMemcpyAsync(pp.data(), c->pix_parent, n_total * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                c->stream));
-        SGPU_HIP(hipMemcpyAsync(wz.data(), c->wall_z, wz.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-        SGPU_HIP(hipStreamSynchronize(c->stream));
-        upload_orders(c, heights, pp.data(), wz.data());
+        /* the processing orders, on the device too */
+        try {
+            c->order = dalloc<int32_t>(n_total);
+            c->trace_perm = dalloc<int32_t>(n_total);
+            orders_device(c->stream, c->grid, w, h, dv, n_total, 32, kTraceOrderBlock, c->order, c->trace_perm);
+        } catch (...) {
+            dfree(dv.wall_off);
+            throw;
+        }
+        dfree(dv.wall_off);
         c->scene = true;
+    });
+}
+
+int sgpu_get_orders(sgpu_ctx* c, int32_t* order, int32_t* lane_order) {
+    return guarded([&] {
+        need(c, SGPU_ERR_ARG, "null context");
+        need(c->scene, SGPU_ERR_STATE, "scene not set");
+        c->set_device();
+        const size_t n = (size_t)c->n_total;
+        if (order) SGPU_HIP(hipMemcpyAsync(order, c->order, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        if (lane_order)
+            SGPU_HIP(hipMemcpyAsync(lane_order, c->trace_perm, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
     });
 }
 

@@ -74,6 +74,9 @@ void synth_dsm(uint64_t seed, int w, int h, float cell_m, float density, int tre
 /* Reflection-pass processing order: cells in tiles of tile x tile (tiles and cells in raster
    order), each top voxel followed by the wall voxels it parents. */
 std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32_t* pix_parent, int tile);
+/* The tiles (x, y) of a tw x th tile grid along a Hilbert curve (side: the next power of two),
+   so that consecutive tiles are neighbours: the order of processing_order's tiles. */
+std::vector<std::pair<int, int>> hilbert_tiles(int tw, int th);
 /* DDA processing order: within every block of `block` consecutive voxel ids, the ids sorted
    by (height of the parent cell, voxel height), so a wave's 64 rays of one direction have
    similar lengths.  perm[i] is the voxel traced by lane i; blocks start at id 0. */
@@ -163,11 +166,16 @@ struct DeviceVoxels {
     float* slopes = nullptr;
     float* aspects = nullptr;
     float* wall_z = nullptr;
+    int64_t* wall_off = nullptr;   // num_top + 1: cell c's walls are ids num_top + wall_off[c] ..
 };
 /* The voxels of the w x h heightmap heights_dev (device) with pixel scales sx, sy (the voxel size
    is sx); returns num_total. */
 int64_t voxelize_device(hipStream_t s, const float* heights_dev, int w, int h, double sx, double sy,
                         DeviceVoxels* out);
+/* processing_order and trace_order of a scene voxelized by voxelize_device, built on the device
+   (identical to the host functions): order and perm get num_total entries (caller-allocated). */
+void orders_device(hipStream_t s, const float* heights_dev, int w, int h, const DeviceVoxels& v, int64_t n_total,
+                   int tile, int block, int32_t* order, int32_t* perm);
 
 /* Exclusive scan of n int32 counts into n+1 int64 offsets (offsets[n] = total). */
 void launch_scan_counts(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* scratch_blocks,

@@ -53,7 +53,14 @@ def test_device_voxelization_matches_reference_golden(name):
 def test_device_voxelization_matches_host(seed, w, h, cell):
     heights = sg.synth_dsm(seed, w, h, cell, 0.45, 8)
     host = sg.assemble_scene(heights, cell)
-    dev = _device_scene(heights, cell, cell)
+    with sg.Context(0) as ctx:
+        ctx.set_scene_heights(heights, cell, cell)
+        dev = ctx.voxels(heights, cell)
+        order_d, lanes_d = ctx.orders()
+        ctx.set_scene(host)   # the same scene through the host path
+        order_h, lanes_h = ctx.orders()
+    np.testing.assert_array_equal(order_d, order_h)
+    np.testing.assert_array_equal(lanes_d, lanes_h)
     assert dev.num_total == host.num_total and dev.max_height == host.max_height
     np.testing.assert_array_equal(dev.pixel_parent, host.pixel_parent)
     np.testing.assert_array_equal(dev.wall_z.view(np.uint32), host.wall_z.view(np.uint32))
