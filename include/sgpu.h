@@ -136,6 +136,15 @@ int sgpu_get_orders(sgpu_ctx* ctx, int32_t* order, int32_t* lane_order);
 /* The context's voxel arrays (num_total entries; wall_z num_total - num_top); any may be NULL. */
 int sgpu_get_voxels(sgpu_ctx* ctx, int32_t* pixel_parent, float* slopes, float* aspects, float* wall_z);
 
+/* Persisted neighbour lists (no reference counterpart: the reference recomputes the viewshed on
+   every run).  sgpu_save_lists writes the SVF bitmap and the device-resident lists of `formats`
+   (CSR and/or ba-RLE, int64 offsets) to `path`; sgpu_load_lists reads them into a context whose
+   scene (dimensions and a checksum of heights, parents, wall heights) and sky (sun bins) match
+   the file's, in place of sgpu_viewshed.  Mismatches return SGPU_ERR_ARG, damaged files
+   SGPU_ERR_FORMAT. */
+int sgpu_save_lists(sgpu_ctx* ctx, const char* path, unsigned formats);
+int sgpu_load_lists(sgpu_ctx* ctx, const char* path);
+
 /* Upload the scene.  wall_z may be NULL when num_total == num_top. */
 int sgpu_set_scene(sgpu_ctx* ctx, const float* grid, int grid_width, int grid_height, int64_t num_total,
                    int64_t num_top, const int32_t* pixel_parent, const float* wall_z, float max_height,

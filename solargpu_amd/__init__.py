@@ -302,6 +302,14 @@ class Context:
         _check(_lib.sgpu_set_scene_heights(self._h, _ptr(h), h.shape[1], h.shape[0], ctypes.c_double(cell),
                                            ctypes.c_double(cell if cell_y is None else cell_y)))
 
+    def save_lists(self, path: str, fmt: int = FMT_CSR | FMT_BARLE) -> None:
+        """Write the SVF bitmap and the lists of `fmt` to `path` (reruns skip the viewshed)."""
+        _check(_lib.sgpu_save_lists(self._h, path.encode(), ctypes.c_uint(fmt)))
+
+    def load_lists(self, path: str) -> None:
+        """Read lists saved for this scene and sky, in place of viewshed()."""
+        _check(_lib.sgpu_load_lists(self._h, path.encode()))
+
     def orders(self) -> tuple[np.ndarray, np.ndarray]:
         """(reflection-pass processing order, DDA lane order) of the context's scene."""
         n = self.info()["num_total"]
