@@ -6,7 +6,7 @@
  *           dda.h (a lane takes a new ray when its own has ended).  First-hit
  *           ids go to a per-chunk raw table raw[voxel][direction] (int32,
  *           0 = no hit).
- * k_lists   one workgroup per voxel: LDS hash set of the voxel's hit cells
+ * k_lists   one workgroup per 8 voxels, one at a time: LDS hash set of the voxel's hit cells
  *           with the smallest direction index per cell (LDS atomics), then an
  *           ordered block scan keeps the first occurrence of every cell in
  *           direction order -- the reference's append-if-absent list
@@ -133,53 +133,74 @@ __device__ __forceinline__ int block_excl_scan(int x, int* tmp, int* total) {
 }
 
 struct ListsLds {
-    int keys[kHashSize];
-    int mins[kHashSize];
+    __align__(16) int keys[kHashSize];
+    __align__(16) int mins[kHashSize];
     unsigned svf_acc[kAzBins * kElBins / 8 + 1];
     int tmp[kListThreads / 64];
 };
 
-/* One voxel's first-occurrence list from its raw row (workgroup-wide): the count, the
- * voxel's SVF bytes when svf is given (svf[byte * n + v]), and the kept ids in direction
- * order written over the front of the row (every read of the row precedes the barriers). */
-__device__ __forceinline__ void voxel_list(ListsLds& s, int32_t* raw, const DirConst* dirs, uint8_t* svf,
+/* Thread t's directions 11 t .. 11 t + 10 of a raw row (0 past the last direction). */
+__device__ __forceinline__ void load_row(const int32_t* row, int (&hv)[kDirsPerThread]) {
+    const int d0 = threadIdx.x * kDirsPerThread;
+#pragma unroll
+    for (int k = 0; k < kDirsPerThread; ++k) hv[k] = (d0 + k < kDirs) ? row[d0 + k] : 0;
+}
+
+/* Dense SVF bit of thread t's directions (-1: none). */
+__device__ __forceinline__ void load_dense(const DirConst* dirs, int (&dn)[kDirsPerThread]) {
+    const int d0 = threadIdx.x * kDirsPerThread;
+#pragma unroll
+    for (int k = 0; k < kDirsPerThread; ++k) dn[k] = (d0 + k < kDirs) ? (dirs[d0 + k].info >> 8) - 1 : -1;
+}
+
+/* One voxel's first-occurrence list from its raw row hv (workgroup-wide, thread t holding
+ * directions 11 t .. 11 t + 10): the count, the voxel's SVF bytes when svf is given
+ * (svf[byte * n + v]), and the kept ids in direction order written over the front of the row
+ * (every read of the row precedes the barriers).  A hit equal to the thread's previous
+ * direction's hit cannot be a first occurrence and skips the hash set (adjacent elevations of
+ * one azimuth mostly hit the same cell: about three hits in four); the others are inserted
+ * with all of a thread's compare-and-swaps in flight at once, probing on only after a slot
+ * held another cell.  The LDS is free again when the call returns. */
+__device__ __forceinline__ void voxel_list(ListsLds& s, const int (&hv)[kDirsPerThread],
+                                           const int (&dn)[kDirsPerThread], int32_t* raw, uint8_t* svf,
                                            int svf_bytes, int64_t n, int64_t v, int32_t* count_out) {
     int* keys = s.keys;
     int* mins = s.mins;
     unsigned* svf_acc = s.svf_acc;
     const int tid = threadIdx.x;
-    for (int i = tid; i < kHashSize; i += kListThreads) { keys[i] = 0; mins[i] = INT_MAX; }
+    for (int i = tid; i < kHashSize / 4; i += kListThreads) {
+        reinterpret_cast<int4*>(keys)[i] = make_int4(0, 0, 0, 0);
+        reinterpret_cast<int4*>(mins)[i] = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+    }
     for (int i = tid; i < svf_bytes; i += kListThreads) svf_acc[i] = 0u;
     __syncthreads();
     const int d0 = tid * kDirsPerThread;
-    int hv[kDirsPerThread];
-    int slot[kDirsPerThread];
+    bool cand[kDirsPerThread];
+    unsigned sl[kDirsPerThread];
+    int got[kDirsPerThread];
 #pragma unroll
     for (int k = 0; k < kDirsPerThread; ++k) {
-        const int d = d0 + k;
-        const int h = (d < kDirs) ? raw[d] : 0;
-        hv[k] = h;
-        slot[k] = 0;
-        if (h) {
-            unsigned sl = ((unsigned)h * 2654435761u) >> (32 - 12);
-            for (;;) {
-                const int prev = atomicCAS(&keys[sl], 0, h);
-                if (prev == 0 || prev == h) break;
-                sl = (sl + 1) & (kHashSize - 1);
+        cand[k] = hv[k] != 0 && (k == 0 || hv[k] != hv[k - 1]);
+        sl[k] = ((unsigned)hv[k] * 2654435761u) >> (32 - 12);
+    }
+#pragma unroll
+    for (int k = 0; k < kDirsPerThread; ++k) got[k] = cand[k] ? atomicCAS(&keys[sl[k]], 0, hv[k]) : 0;
+#pragma unroll
+    for (int k = 0; k < kDirsPerThread; ++k) {
+        if (cand[k]) {
+            while (got[k] != 0 && got[k] != hv[k]) {
+                sl[k] = (sl[k] + 1) & (kHashSize - 1);
+                got[k] = atomicCAS(&keys[sl[k]], 0, hv[k]);
             }
-            slot[k] = (int)sl;
-            atomicMin(&mins[sl], d);
-            if (svf) {
-                const int dense = (dirs[d].info >> 8) - 1;
-                if (dense >= 0) atomicOr(&svf_acc[dense >> 3], 1u << (dense & 7));
-            }
+            atomicMin(&mins[sl[k]], d0 + k);
         }
+        if (svf && hv[k] != 0 && dn[k] >= 0) atomicOr(&svf_acc[dn[k] >> 3], 1u << (dn[k] & 7));
     }
     __syncthreads();
     int kept = 0;
 #pragma unroll
     for (int k = 0; k < kDirsPerThread; ++k)
-        if (hv[k] && mins[slot[k]] == d0 + k) ++kept;
+        if (cand[k] && mins[sl[k]] == d0 + k) ++kept;
     int total;
     const int excl = block_excl_scan(kept, s.tmp, &total);   // barriers: the row is read
     if (tid == 0) *count_out = total;
@@ -188,7 +209,8 @@ __device__ __forceinline__ void voxel_list(ListsLds& s, int32_t* raw, const DirC
     int32_t* dst = raw + excl;
 #pragma unroll
     for (int k = 0; k < kDirsPerThread; ++k)
-        if (hv[k] && mins[slot[k]] == d0 + k) *dst++ = hv[k];
+        if (cand[k] && mins[sl[k]] == d0 + k) *dst++ = hv[k];
+    __syncthreads();   // every LDS read of this voxel is done
 }
 
 __global__ __launch_bounds__(kListThreads) void k_lists_batch(BatchListArgs a) {
@@ -207,13 +229,31 @@ __global__ __launch_bounds__(kListThreads) void k_lists_batch(BatchListArgs a) {
     }
     __syncthreads();
     const TileDesc& t = a.tiles[tile_s];
-    voxel_list(s, a.raw + local * kDirs, t.dirs, t.svf, t.svf_bytes, t.n_total, g - t.vbase, a.counts + local);
+    int32_t* row = a.raw + local * kDirs;
+    int hv[kDirsPerThread], dn[kDirsPerThread];
+    load_row(row, hv);
+    load_dense(t.dirs, dn);
+    voxel_list(s, hv, dn, row, t.svf, t.svf_bytes, t.n_total, g - t.vbase, a.counts + local);
 }
 
+/* kListVoxels consecutive voxels per workgroup; the next voxel's raw row is loaded into
+ * registers while the current one goes through the hash set. */
+constexpr int kListVoxels = 8;
 __global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
     __shared__ ListsLds s;
-    const int64_t local = blockIdx.x;
-    voxel_list(s, a.raw + local * kDirs, a.dirs, a.svf, a.svf_bytes, a.n_total, a.v0 + local, a.counts + local);
+    const int64_t first = (int64_t)blockIdx.x * kListVoxels;
+    const int nvox = (int)min<int64_t>(kListVoxels, a.count - first);
+    int hv[kDirsPerThread], nx[kDirsPerThread], dn[kDirsPerThread];
+    load_dense(a.dirs, dn);
+    load_row(a.raw + first * kDirs, hv);
+    for (int i = 0; i < nvox; ++i) {
+        const int64_t local = first + i;
+        if (i + 1 < nvox) load_row(a.raw + (local + 1) * kDirs, nx);
+        voxel_list(s, hv, dn, a.raw + local * kDirs, a.svf, a.svf_bytes, a.n_total, a.v0 + local,
+                   a.counts + local);
+#pragma unroll
+        for (int k = 0; k < kDirsPerThread; ++k) hv[k] = nx[k];
+    }
 }
 
 /* Every voxel's kept ids (the front of its raw row) to its scanned offset; one wave per voxel. */
@@ -424,7 +464,8 @@ void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s) {
     if (a.count <= 0) return;
     if (a.svf_bytes > kAzBins * kElBins / 8 + 1) fail(SGPU_ERR_ARG, "svf_bytes %d out of range", a.svf_bytes);
     if (write_pass) launch_list_copy(a.raw, a.counts, a.offsets, a.ids, a.count, s);
-    else hipLaunchKernelGGL(k_lists, dim3((unsigned)a.count), dim3(kListThreads), 0, s, a);
+    else hipLaunchKernelGGL(k_lists, dim3((unsigned)((a.count + kListVoxels - 1) / kListVoxels)), dim3(kListThreads), 0,
+                            s, a);
     SGPU_HIP(hipGetLastError());
 }
 

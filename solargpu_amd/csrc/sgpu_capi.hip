@@ -49,6 +49,7 @@ int64_t choose_chunk(const sgpu_ctx* c, int64_t n) {
        table entries per chunk. */
     size_t free_b = 0, total_b = 0;
     SGPU_HIP(hipMemGetInfo(&free_b, &total_b));
+    free_b += c->ws_bytes;   // the kept workspace is reused or replaced
     const double per_voxel = (double)kDirs * 4.0 + 32.0;
     int64_t chunk = (int64_t)(0.4 * (double)free_b / per_voxel);
     chunk = std::min<int64_t>(chunk, (int64_t)1 << 20 << 2);
@@ -332,10 +333,20 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
     std::vector<std::pair<int32_t*, int64_t>> pieces;
     int64_t base = 0;
     try {
-        raw = dalloc<int32_t>((size_t)chunk * kDirs);
-        counts = dalloc<int32_t>(chunk);
-        off = dalloc<int64_t>(chunk + 1);
-        scratch = dalloc<int64_t>(scan_scratch_elems(chunk));
+        if (c->ws_chunk < chunk) {
+            c->drop_workspace();
+            c->ws_raw = dalloc<int32_t>((size_t)chunk * kDirs);
+            c->ws_counts = dalloc<int32_t>(chunk);
+            c->ws_off = dalloc<int64_t>(chunk + 1);
+            c->ws_scratch = dalloc<int64_t>(scan_scratch_elems(chunk));
+            c->ws_chunk = chunk;
+            c->ws_bytes = (size_t)chunk * kDirs * 4 + (size_t)chunk * 4 + (size_t)(chunk + 1) * 8 +
+                          (size_t)scan_scratch_elems(chunk) * 8;
+        }
+        raw = c->ws_raw;
+        counts = c->ws_counts;
+        off = c->ws_off;
+        scratch = c->ws_scratch;
         /* the height-sorted lane order permutes within blocks of kTraceOrderBlock ids from 0:
            usable when every chunk starts on a block (whole scene, chunk a multiple of it) */
         const int32_t* perm = (!subset && (chunk >= N || chunk % kTraceOrderBlock == 0)) ? c->trace_perm : nullptr;
@@ -390,10 +401,7 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
         }
         if (barle_only) {
             out.nnz = base;
-            dfree(raw);
-            dfree(counts);
-            dfree(off);
-            dfree(scratch);
+            c->drop_workspace();   // this mode is for scenes that stretch the memory
             SGPU_HIP(hipMemcpyAsync(out.br_off + N, &bbase, sizeof bbase, hipMemcpyHostToDevice, c->stream));
             if (bpieces.size() == 1) {
                 out.br_bytes = bpieces[0].first;
@@ -417,10 +425,7 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
         }
         if (to_host) {
             out.nnz = base;
-            dfree(raw);
-            dfree(counts);
-            dfree(off);
-            dfree(scratch);
+            c->drop_workspace();   // this mode is for scenes that stretch the memory
             return out;
         }
         SGPU_HIP(hipMemcpyAsync(out.off + N, &base, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
@@ -444,20 +449,13 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
     } catch (...) {
         for (auto& p : pieces) dfree(p.first);
         for (auto& p : bpieces) dfree(p.first);
-        dfree(raw);
-        dfree(counts);
-        dfree(off);
-        dfree(scratch);
+        c->drop_workspace();
         dfree(out.off);
         dfree(out.ids);
         dfree(out.br_off);
         dfree(out.br_bytes);
         throw;
     }
-    dfree(raw);
-    dfree(counts);
-    dfree(off);
-    dfree(scratch);
     out.nnz = base;
     return out;
 }

@@ -154,6 +154,16 @@ struct sgpu_ctx {
     size_t hours_d_cap = 0;
     bool irr_ok = false;
 
+    /* Viewshed workspace for chunks of up to ws_chunk voxels (raw first-hit table, per-voxel
+       counts and offsets, scan scratch), kept between viewsheds of the scene: a 2^22-voxel
+       chunk's table is 43.5 GB, whose allocation is not repeated per call. */
+    int32_t* ws_raw = nullptr;
+    int32_t* ws_counts = nullptr;
+    int64_t* ws_off = nullptr;
+    int64_t* ws_scratch = nullptr;
+    int64_t ws_chunk = 0;
+    size_t ws_bytes = 0;
+
     int64_t chunk_override = 0;
     bool kernel_timing = false;
     bool count_steps = false;
@@ -179,7 +189,16 @@ struct sgpu_ctx {
         sur.release();
         host.release();
     }
+    void drop_workspace() {
+        sgpu::dfree(ws_raw);
+        sgpu::dfree(ws_counts);
+        sgpu::dfree(ws_off);
+        sgpu::dfree(ws_scratch);
+        ws_chunk = 0;
+        ws_bytes = 0;
+    }
     void drop_scene() {
+        drop_workspace();
         sgpu::dfree(grid);
         sgpu::dfree(pix_parent);
         sgpu::dfree(wall_z);
