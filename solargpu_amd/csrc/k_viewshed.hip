@@ -39,6 +39,11 @@ constexpr int kHashSize = 4096;              // > 1.5 x the 2592 possible distin
  * all idle lanes at once. */
 constexpr int kRefill = 16;
 
+/* Workgroup x of a row of the grid runs on XCD x % 8 (gridDim.x a multiple of 8); it takes voxel
+ * block (x % 8) * (gridDim.x / 8) + x / 8, so each XCD traces one contiguous eighth of the
+ * blocks and its rays read a compact band of the heightmap through its own L2. */
+__device__ __forceinline__ unsigned trace_block(unsigned x, unsigned nx) { return (x % 8u) * (nx / 8u) + x / 8u; }
+
 template <bool COUNT_STEPS, int G>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
     __shared__ DirConst sdir[G];
@@ -46,7 +51,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
     __shared__ int srow[kTraceBlock / 64][64];        // their raw rows
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int i = threadIdx.x; i < G; i += kTraceBlock) sdir[i] = a.dirs[blockIdx.y * G + i];
-    const int64_t lane_pos = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    const int64_t lane_pos = (int64_t)trace_block(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x;
     const int64_t wave_pos = lane_pos - lane;
     const int nvalid = (int)min<int64_t>(64, max<int64_t>(0, a.count - wave_pos));
     if (lane < nvalid) {
@@ -440,7 +445,7 @@ void launch_trace(const TraceArgs& a, hipStream_t s) {
     if (a.count <= 0) return;
     if (!grid_off32((unsigned)a.W, (unsigned)a.H)) fail(SGPU_ERR_ARG, "grid beyond 32-bit byte offsets");
     const int G = a.group ? a.group : 32;
-    const unsigned bx = (unsigned)((a.count + kTraceBlock - 1) / kTraceBlock);
+    const unsigned bx = (unsigned)((a.count + kTraceBlock - 1) / kTraceBlock + 7) / 8u * 8u;   // whole XCD sets
     dim3 grid(bx, (unsigned)(kDirs / G));
     switch (G) {
         case 16:
