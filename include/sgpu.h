@@ -139,9 +139,9 @@ int sgpu_get_voxels(sgpu_ctx* ctx, int32_t* pixel_parent, float* slopes, float* 
 /* Persisted neighbour lists (no reference counterpart: the reference recomputes the viewshed on
    every run).  sgpu_save_lists writes the SVF bitmap and the device-resident lists of `formats`
    (CSR and/or ba-RLE, int64 offsets) to `path`; sgpu_load_lists reads them into a context whose
-   scene (dimensions and a checksum of heights, parents, wall heights) and sky (sun bins) match
-   the file's, in place of sgpu_viewshed.  Mismatches return SGPU_ERR_ARG, damaged files
-   SGPU_ERR_FORMAT. */
+   scene (dimensions and a checksum of heights, parents, wall heights and height limit) and sky
+   (sun bins) match the file's, in place of sgpu_viewshed.  Mismatches return SGPU_ERR_ARG,
+   damaged files SGPU_ERR_FORMAT. */
 int sgpu_save_lists(sgpu_ctx* ctx, const char* path, unsigned formats);
 int sgpu_load_lists(sgpu_ctx* ctx, const char* path);
 

@@ -7,7 +7,8 @@
  * CSR offsets (num_total + 1 int64) and ids (nnz int32) when present, ba-RLE offsets (num_total
  * + 1 int64) and bytes when present.  The header carries the scene's and the sky's checksums
  * (a position-weighted 64-bit sum of the words of the heightmap, parents and wall heights; of
- * the dense sun-bin map), so lists are only loaded into the scene and sky they were made for.
+ * the dense sun-bin map; the scene's also takes the height limit), so lists are only loaded
+ * into the scene and sky they were made for.
  * Sections stream through a pinned host buffer in 64 MB pieces. */
 #include <cstdio>
 #include <cstring>
@@ -67,6 +68,9 @@ uint64_t scene_checksum(sgpu_ctx* c) {
         sum = device_checksum(c->stream, c->grid, c->n_top, 1, tmp);
         sum = sum * 0x9E3779B97F4A7C15ull + device_checksum(c->stream, c->pix_parent, c->n_total, 2, tmp);
         sum = sum * 0x9E3779B97F4A7C15ull + device_checksum(c->stream, c->wall_z, c->n_total - c->n_top, 3, tmp);
+        uint32_t zbits = 0;   // the height limit rays leave through
+        std::memcpy(&zbits, &c->maxz, sizeof zbits);
+        sum = sum * 0x9E3779B97F4A7C15ull + zbits;
     } catch (...) {
         dfree(tmp);
         throw;

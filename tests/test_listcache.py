@@ -1,6 +1,8 @@
 """Persisted neighbour lists (sgpu_save_lists / sgpu_load_lists): a reloaded context reproduces
 the viewshed's lists and SVF and the hour loop's annual sums bit for bit; files only load into
 the scene and sky they were made for; damaged files are refused."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -80,6 +82,9 @@ def test_files_load_only_into_their_scene_and_sky(tmp_path):
         a.save_lists(path, sg.FMT_CSR)
     other = synth_scene(34, 64, 48)   # same dimensions, other heights
     with _ctx(other) as b, pytest.raises(sg.SolarGPUError, match="another scene"):
+        b.load_lists(path)
+    higher = dataclasses.replace(sc, max_height=sc.max_height + 5.0)   # rays leave at another height
+    with _ctx(higher) as b, pytest.raises(sg.SolarGPUError, match="another scene"):
         b.load_lists(path)
     az_alt = sg.sun_positions(-70.0, -33.0, -4)   # another site: other sun bins
     dni, dhi = sg.synth_weather(5, az_alt)
