@@ -3,6 +3,7 @@
  * sunlit-hour loop.  All work runs on the context's own HIP stream; every
  * failure becomes a status code and a thread-local message. */
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -995,7 +996,10 @@ int sgpu_set_scene_heights(sgpu_ctx* c, const float* heights, int w, int h, doub
         c->drop_scene();
         const int64_t n_top = (int64_t)w * h;
         float maxz = heights[0];   // the reference's max_height: the heightmap's maximum
-        for (int64_t i = 1; i < n_top; ++i) maxz = std::max(maxz, heights[i]);
+        for (int64_t i = 0; i < n_top; ++i) {
+            need(std::isfinite(heights[i]), SGPU_ERR_ARG, "heights must be finite");
+            maxz = std::max(maxz, heights[i]);
+        }
         c->grid = dalloc<float>(n_top);
         SGPU_HIP(hipMemcpyAsync(c->grid, heights, n_top * sizeof(float), hipMemcpyHostToDevice, c->stream));
         DeviceVoxels dv;

@@ -87,3 +87,15 @@ def test_lists_from_device_scene_equal_oracle():
         irr = ctx.irradiance()
     rel = np.abs(irr.astype(np.float64) - exact) / np.maximum(np.abs(exact), 1.0)
     assert rel.max() <= 1e-4
+
+
+@pytest.mark.gpu
+def test_non_finite_heights_are_refused():
+    heights = sg.synth_dsm(5, 32, 24, 1.0, 0.3, 2)
+    with sg.Context(0) as ctx:
+        for bad in (np.nan, np.inf, -np.inf):
+            h = heights.copy()
+            h[7, 11] = bad
+            with pytest.raises(sg.SolarGPUError, match="finite"):
+                ctx.set_scene_heights(h, 1.0)
+        ctx.set_scene_heights(heights, 1.0)   # the context stays usable
