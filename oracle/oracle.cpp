@@ -207,11 +207,8 @@ float perez_f32(float cos_z, float cos_aoi, float dif, float dir, float slope, i
 
 /* The FP32 epsilon bin of an hour: a discrete decision shared with the FP64 oracle. */
 int hour_eps_bin(float alt, float dif, float dir) {
-    float cos_z = cosf((90.0f - alt) * kDegToRadF);
-    float za = acosf(cos_z);
-    float eps = ((dif + dir / cosf(za)) / dif + 1.041f * (za * za * za)) / (1.0f + 1.041f * za * za * za);
-    eps = fmaxf(1.0f, eps);
-    return eps_bin(eps);
+    float eps;
+    return orc_hour_eps(alt, dif, dir, &eps);
 }
 
 inline bool svf_bit(const uint8_t* svf, int64_t v, int di, int64_t n_total) {
@@ -766,6 +763,15 @@ int orc_barle_decode(const uint8_t* bytes, int64_t size, uint32_t* vals, int cap
         ++n;
     });
     return ok ? n : -1;
+}
+
+int orc_hour_eps(float alt, float dif, float dir, float* eps_out) {
+    const float cos_z = cosf((90.0f - alt) * kDegToRadF);
+    const float za = acosf(cos_z);
+    float eps = ((dif + dir / cosf(za)) / dif + 1.041f * (za * za * za)) / (1.0f + 1.041f * za * za * za);
+    eps = fmaxf(1.0f, eps);
+    *eps_out = eps;
+    return eps_bin(eps);
 }
 
 int orc_sunlit_hours(const float* naa, int* hours_out) {

@@ -101,6 +101,11 @@ int orc_barle_decode(const uint8_t* bytes, int64_t size, uint32_t* vals, int cap
    receives their indices day*24+hour. */
 int orc_sunlit_hours(const float* nudged_az_alt, int* hours_out);
 
+/* Perez sky clearness epsilon of one hour in FP32 (solar_kernel.cu:23-40: zenith from the
+   nudged altitude, max(1, eps)), into *eps; returns its bin 0..7 (the discrete decision the
+   FP64 oracle shares with the FP32 path). */
+int orc_hour_eps(float alt, float dif, float dir, float* eps);
+
 /* Solar kernel for one hour (solar_kernel.cu:54-77) on voxels [0, n). */
 void orc_solar_hour(const float* nudged_az_alt, const int* sun_dense_idx, const float* dni,
                     const float* dhi, int hour_index, int n, const float* slopes, const float* aspects,

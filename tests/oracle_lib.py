@@ -184,6 +184,13 @@ def sunlit_hours(nudged):
     return out[:n].copy()
 
 
+def hour_eps(alt, dif, dir_):
+    """FP32 Perez clearness epsilon of one hour and its bin."""
+    e = ctypes.c_float()
+    b = O.orc_hour_eps(_F(alt), _F(dif), _F(dir_), ctypes.byref(e))
+    return e.value, b
+
+
 def solar_hour(sc, bins, dni, dhi, svf, hour_index, nthreads=0):
     out = np.zeros(sc.num_total, np.float32)
     O.orc_solar_hour(_ptr(bins["nudged"]), _ptr(bins["sun_dense_idx"]), _ptr(_c(dni, np.float32)),

@@ -280,3 +280,28 @@ def test_generators_are_deterministic_and_in_spec():
     assert (dhi[lit] > 0).all() and (dni[~lit] == 0).all() and (dhi[~lit] == 0).all()
     d2, h2 = sg.synth_weather(7, az_alt)
     assert np.array_equal(dni, d2) and np.array_equal(dhi, h2)
+
+
+PEREZ_EDGES = np.array([1.065, 1.230, 1.500, 1.950, 2.800, 4.500, 6.200], np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("site,seed", [(SITE, 20260101), ((15.65, 46.56, 1), 20260102), ((-3.7, 40.4, 0), 7)])
+def test_perez_epsilon_bins_agree_with_fp64_away_from_edges(site, seed):
+    """Every sunlit hour's Perez clearness epsilon in FP64 (zenith in radians from the nudged
+    altitude, no float rounding): hours within 1e-5 relative of a bin edge are reported, and at
+    every other hour the FP32 bin (the reference's arithmetic, which the library's hour constants
+    and the FP64 oracle share) is the bin of the FP64 epsilon."""
+    az_alt, dni, dhi, bins = sky(site, seed)
+    hours = orc.sunlit_hours(bins["nudged"])
+    assert hours.size > 4000
+    alt = bins["nudged"][hours, 1].astype(np.float64)
+    dif, dr = dhi[hours].astype(np.float64), dni[hours].astype(np.float64)
+    assert (dif > 0).all()   # the generator keeps DHI > 0 in sunlit hours (no Perez 0/0)
+    za = np.radians(90.0 - alt)
+    eps64 = np.fmax(1.0, ((dif + dr / np.cos(za)) / dif + 1.041 * za ** 3) / (1.0 + 1.041 * za ** 3))
+    bin64 = np.searchsorted(PEREZ_EDGES, eps64, side="right")
+    bin32 = np.array([orc.hour_eps(a, f, d)[1] for a, f, d in zip(alt, dif, dr)])
+    near = (np.abs(eps64[:, None] / PEREZ_EDGES[None, :] - 1.0) <= 1e-5).any(axis=1)
+    print(f"site {site}: {hours.size} sunlit hours, {int(near.sum())} within 1e-5 of a Perez bin edge "
+          f"(hours {hours[near].tolist()})")
+    np.testing.assert_array_equal(bin32[~near], bin64[~near])
