@@ -80,7 +80,7 @@ def main(argv=None):
             with sg.Context(0) as ctx:
                 ctx.set_scene(sc)
                 ctx.set_sky(az, dni, dhi)
-                ctx.set_tuning(16, K, waves, 0, 0)
+                ctx.set_tuning(32, K, waves, 256, 32)   # the batched launches' shapes
                 ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
                 tv += ctx.info()["t_viewshed_ms"]
                 ctx.hour_loop(fmt, K)

@@ -218,29 +218,6 @@ __device__ __forceinline__ void voxel_list(ListsLds& s, const int (&hv)[kDirsPer
     __syncthreads();   // every LDS read of this voxel is done
 }
 
-__global__ __launch_bounds__(kListThreads) void k_lists_batch(BatchListArgs a) {
-    __shared__ ListsLds s;
-    __shared__ int tile_s;
-    const int64_t local = blockIdx.x;
-    const int64_t g = a.v0 + local;
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = a.ntiles - 1;   // last tile with vbase <= g
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (a.tiles[mid].vbase <= g) lo = mid;
-            else hi = mid - 1;
-        }
-        tile_s = lo;
-    }
-    __syncthreads();
-    const TileDesc& t = a.tiles[tile_s];
-    int32_t* row = a.raw + local * kDirs;
-    int hv[kDirsPerThread], dn[kDirsPerThread];
-    load_row(row, hv);
-    load_dense(t.dirs, dn);
-    voxel_list(s, hv, dn, row, t.svf, t.svf_bytes, t.n_total, g - t.vbase, a.counts + local);
-}
-
 /* kListVoxels consecutive voxels per workgroup; the next voxel's raw row is loaded into
  * registers while the current one goes through the hash set. */
 constexpr int kListVoxels = 8;
@@ -256,6 +233,42 @@ __global__ __launch_bounds__(kListThreads) void k_lists(ListArgs a) {
         if (i + 1 < nvox) load_row(a.raw + (local + 1) * kDirs, nx);
         voxel_list(s, hv, dn, a.raw + local * kDirs, a.svf, a.svf_bytes, a.n_total, a.v0 + local,
                    a.counts + local);
+#pragma unroll
+        for (int k = 0; k < kDirsPerThread; ++k) hv[k] = nx[k];
+    }
+}
+
+/* The same over the concatenated voxels of a batch: each voxel's tile (binary search on the
+ * tile bases) gives its SVF bitmap and direction table. */
+__global__ __launch_bounds__(kListThreads) void k_lists_batch(BatchListArgs a) {
+    __shared__ ListsLds s;
+    __shared__ int tile_s;
+    const int64_t first = (int64_t)blockIdx.x * kListVoxels;
+    const int nvox = (int)min<int64_t>(kListVoxels, a.count - first);
+    int hv[kDirsPerThread], nx[kDirsPerThread], dn[kDirsPerThread];
+    int cur = -1;   // the tile dn belongs to
+    load_row(a.raw + first * kDirs, hv);
+    for (int i = 0; i < nvox; ++i) {
+        const int64_t local = first + i;
+        const int64_t g = a.v0 + local;
+        if (threadIdx.x == 0) {
+            int lo = 0, hi = a.ntiles - 1;   // last tile with vbase <= g
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (a.tiles[mid].vbase <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            tile_s = lo;   // the previous voxel's reads of tile_s precede voxel_list's barriers
+        }
+        __syncthreads();
+        const int ti = tile_s;
+        const TileDesc& t = a.tiles[ti];
+        if (ti != cur) {
+            load_dense(t.dirs, dn);
+            cur = ti;
+        }
+        if (i + 1 < nvox) load_row(a.raw + (local + 1) * kDirs, nx);
+        voxel_list(s, hv, dn, a.raw + local * kDirs, t.svf, t.svf_bytes, t.n_total, g - t.vbase, a.counts + local);
 #pragma unroll
         for (int k = 0; k < kDirsPerThread; ++k) hv[k] = nx[k];
     }
@@ -483,7 +496,8 @@ void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_
 void launch_lists_batch(const BatchListArgs& a, bool write_pass, hipStream_t s) {
     if (a.count <= 0) return;
     if (write_pass) launch_list_copy(a.raw, a.counts, a.offsets, a.ids, a.count, s);
-    else hipLaunchKernelGGL(k_lists_batch, dim3((unsigned)a.count), dim3(kListThreads), 0, s, a);
+    else hipLaunchKernelGGL(k_lists_batch, dim3((unsigned)((a.count + kListVoxels - 1) / kListVoxels)),
+                            dim3(kListThreads), 0, s, a);
     SGPU_HIP(hipGetLastError());
 }
 
