@@ -1,7 +1,10 @@
 /* Stage 1 on gfx950: hemispherical DDA viewshed, SVF bitmap and CSR
  * neighbour lists.
  *
- * k_trace_pool  one workgroup per (256-voxel block, G-direction group); each
+ * k_trace_pool  one workgroup per (G-direction group, 256-voxel block), the
+ *           direction groups of a block consecutive in the grid, so the resident
+ *           workgroups trace a few consecutive blocks (a strip of the scene whose
+ *           heightmap neighbourhood every XCD's L2 holds); each
  *           wave traces the G x 64 rays of its 64 voxels on the pooled walk of
  *           dda.h (a lane takes a new ray when its own has ended).  First-hit
  *           ids go to a per-chunk raw table raw[voxel][direction] (int32,
@@ -39,19 +42,14 @@ constexpr int kHashSize = 4096;              // > 1.5 x the 2592 possible distin
  * all idle lanes at once. */
 constexpr int kRefill = 16;
 
-/* Workgroup x of a row of the grid runs on XCD x % 8 (gridDim.x a multiple of 8); it takes voxel
- * block (x % 8) * (gridDim.x / 8) + x / 8, so each XCD traces one contiguous eighth of the
- * blocks and its rays read a compact band of the heightmap through its own L2. */
-__device__ __forceinline__ unsigned trace_block(unsigned x, unsigned nx) { return (x % 8u) * (nx / 8u) + x / 8u; }
-
 template <bool COUNT_STEPS, int G>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
     __shared__ DirConst sdir[G];
     __shared__ float sorg[kTraceBlock / 64][3][64];   // gx, gy, base z of the wave's voxels
     __shared__ int srow[kTraceBlock / 64][64];        // their raw rows
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int i = threadIdx.x; i < G; i += kTraceBlock) sdir[i] = a.dirs[blockIdx.y * G + i];
-    const int64_t lane_pos = (int64_t)trace_block(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x;
+    for (int i = threadIdx.x; i < G; i += kTraceBlock) sdir[i] = a.dirs[blockIdx.x * G + i];
+    const int64_t lane_pos = (int64_t)blockIdx.y * kTraceBlock + threadIdx.x;
     const int64_t wave_pos = lane_pos - lane;
     const int nvalid = (int)min<int64_t>(64, max<int64_t>(0, a.count - wave_pos));
     if (lane < nvalid) {
@@ -88,7 +86,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
                 const int k = j >> 6, vx = j & 63;
                 if (j < total && vx < nvalid) {
                     pool_ray_setup(ray, sdir[k], sorg[wave][0][vx], sorg[wave][1][vx], sorg[wave][2][vx], uW, maxz);
-                    out = a.raw + (int64_t)srow[wave][vx] * kDirs + blockIdx.y * G + k;
+                    out = a.raw + (int64_t)srow[wave][vx] * kDirs + blockIdx.x * G + k;
                     active = true;
                 }
             }
@@ -458,8 +456,8 @@ void launch_trace(const TraceArgs& a, hipStream_t s) {
     if (a.count <= 0) return;
     if (!grid_off32((unsigned)a.W, (unsigned)a.H)) fail(SGPU_ERR_ARG, "grid beyond 32-bit byte offsets");
     const int G = a.group ? a.group : 32;
-    const unsigned bx = (unsigned)((a.count + kTraceBlock - 1) / kTraceBlock + 7) / 8u * 8u;   // whole XCD sets
-    dim3 grid(bx, (unsigned)(kDirs / G));
+    const unsigned by = (unsigned)((a.count + kTraceBlock - 1) / kTraceBlock);
+    dim3 grid((unsigned)(kDirs / G), by);
     switch (G) {
         case 16:
             if (a.steps) hipLaunchKernelGGL((k_trace_pool<true, 16>), grid, dim3(kTraceBlock), 0, s, a);
