@@ -3,8 +3,8 @@
  *
  * k_trace_pool  one workgroup per (G-direction group, 256-voxel block), the
  *           direction groups of a block consecutive in the grid, so the resident
- *           workgroups trace a few consecutive blocks (a strip of the scene whose
- *           heightmap neighbourhood every XCD's L2 holds); each
+ *           workgroups trace a few consecutive blocks and their first-hit writes
+ *           fall in a few tens of MB of the raw table; each
  *           wave traces the G x 64 rays of its 64 voxels on the pooled walk of
  *           dda.h (a lane takes a new ray when its own has ended).  First-hit
  *           ids go to a per-chunk raw table raw[voxel][direction] (int32,
