@@ -184,17 +184,38 @@ __device__ __forceinline__ float apply_list_at(const int* ids, const float* ws, 
     constexpr int kUnroll = 8 / HPL;
     /* byte offset of this lane's hours within a row, and of row `id` */
     const unsigned lane_off = (K > 64) ? (unsigned)(HPL * lane) * 4u : (unsigned)(lane & (K - 1)) * 4u;
+    const char* lane_top = reinterpret_cast<const char*>(top) + lane_off;   // 64-bit rows: one shift-add each
+    constexpr int kRowShift = __builtin_ctz(K * 4);
     auto load = [&](int id) __attribute__((always_inline)) {
         if constexpr (OFF32) {
             const unsigned off = (unsigned)id * (unsigned)(K * 4) + lane_off;
             return *reinterpret_cast<const vec*>(reinterpret_cast<const char*>(top) + off);
         } else {
-            return *reinterpret_cast<const vec*>(reinterpret_cast<const char*>(top) +
-                                                 (size_t)(unsigned)id * (size_t)(K * 4) + lane_off);
+            return *reinterpret_cast<const vec*>(lane_top + ((uint64_t)(unsigned)id << kRowShift));
         }
     };
     vec a = {}, b = {};
     int i = (K > 64) ? 0 : lane / K;
+    if constexpr (!OFF32) {
+        /* 64-bit rows: the batch's loads issued before its FMAs (the scheduler otherwise pairs
+           them), entries still alternating between a and b in list order */
+        constexpr int B = 2 * kUnroll;
+        for (; i + (B - 1) * G < cnt; i += B * G) {
+            vec x[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) x[u] = load(ids[i + u * G]);
+            __builtin_amdgcn_sched_group_barrier(0x0020, B, 0);   // the VMEM reads first
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const float w = ws[i + u * G];
+#pragma unroll
+                for (int h = 0; h < HPL; ++h) {
+                    if (u & 1) b[h] = fmaf(w, x[u][h], b[h]);
+                    else a[h] = fmaf(w, x[u][h], a[h]);
+                }
+            }
+        }
+    }
 #pragma unroll kUnroll
     for (; i + G < cnt; i += 2 * G) {
         const vec x = load(ids[i]);
