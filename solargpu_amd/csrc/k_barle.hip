@@ -273,7 +273,7 @@ __device__ void encode_voxel(const BarleArgs& a, EncLds<MAXP>& s, int64_t v, int
         __syncthreads();
     }
     nr = optimize_runs_wave(s.keys, nr, m, lane);
-    if (!WRITE) {
+    if (!WRITE || a.slots) {
         /* per split one length byte and its payload, per block of 8 splits 3 code bytes; a run
            of len splits into ceil(len / 255) pieces of 255 and the rest */
         int bytes = 0, pieces = 0;
@@ -287,7 +287,7 @@ __device__ void encode_voxel(const BarleArgs& a, EncLds<MAXP>& s, int64_t v, int
         bytes = wave_sum(bytes);
         pieces = wave_sum(pieces);
         if (lane == 0) a.sizes[v] = bytes + 3 * ((pieces + 7) / 8);
-        return;
+        if (!WRITE) return;
     }
     /* split at 255 (barle_compression.cpp:29-44): run r's pieces at the prefix of piece counts */
     int ns = 0;
@@ -306,7 +306,7 @@ __device__ void encode_voxel(const BarleArgs& a, EncLds<MAXP>& s, int64_t v, int
     __syncthreads();
     /* headers and payload positions: block b (splits 8b .. 8b+7) starts at 11 b + the payload
        bytes of the splits before it; its 3 code bytes, then cnt length bytes, then payloads */
-    uint8_t* out = a.out + a.out_off[v];
+    uint8_t* out = a.slots ? a.out + 6 * b + 8 * v : a.out + a.out_off[v];
     int pay = 0;   // payload bytes before this chunk of splits
     for (int base = 0; base < ns; base += 64) {
         const int t = base + lane;
@@ -368,14 +368,14 @@ __global__ __launch_bounds__(64) void k_barle_short(BarleArgs a) {
     const int64_t b = a.csr_off[v];
     const int n = (int)(a.csr_off[v + 1] - b);
     if (n > kShort) {
-        if (!WRITE && threadIdx.x == 0) {
+        if ((!WRITE || a.slots) && threadIdx.x == 0) {
             if (n <= kMedium) a.long_ids[atomicAdd(&a.long_count[0], 1u)] = (int32_t)v;
             else a.long_ids[a.n - 1 - (int64_t)atomicAdd(&a.long_count[1], 1u)] = (int32_t)v;
         }
         return;
     }
     if (n <= 0) {
-        if (!WRITE && threadIdx.x == 0) a.sizes[v] = 0;
+        if ((!WRITE || a.slots) && threadIdx.x == 0) a.sizes[v] = 0;
         return;
     }
     encode_voxel<WRITE, kShort>(a, s, v, b, n, threadIdx.x);
@@ -413,6 +413,7 @@ void launch_barle(const BarleArgs& a, bool write_pass, hipStream_t s) {
     if (a.n <= 0) return;
     if (!a.long_ids || !a.long_count) fail(SGPU_ERR_ARG, "ba-RLE encoder needs its long-list queue");
     const dim3 g((unsigned)a.n), blk(64);
+    if (write_pass && a.slots) SGPU_HIP(hipMemsetAsync(a.long_count, 0, 2 * sizeof(unsigned), s));
     if (write_pass) {
         hipLaunchKernelGGL(k_barle_short<true>, g, blk, 0, s, a);
         hipLaunchKernelGGL(k_barle_long<true>, dim3(kLongGrid), blk, 0, s, a);
@@ -423,6 +424,27 @@ void launch_barle(const BarleArgs& a, bool write_pass, hipStream_t s) {
         hipLaunchKernelGGL(k_barle_long<false>, dim3(kLongGrid), blk, 0, s, a);
         hipLaunchKernelGGL(k_barle_huge<false>, dim3(kHugeGrid), blk, 0, s, a);
     }
+    SGPU_HIP(hipGetLastError());
+}
+
+/* One wave per voxel: its slot's sizes[v] bytes to out + out_off[v]. */
+__global__ __launch_bounds__(256) void k_barle_compact(const uint8_t* slots, const int64_t* csr_off,
+                                                       const int32_t* sizes, const int64_t* out_off, int64_t n,
+                                                       uint8_t* out) {
+    const int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= n) return;
+    const int lane = threadIdx.x & 63;
+    const uint8_t* src = slots + 6 * csr_off[v] + 8 * v;
+    uint8_t* dst = out + out_off[v];
+    const int len = sizes[v];
+    for (int i = lane; i < len; i += 64) dst[i] = src[i];
+}
+
+void launch_barle_compact(const uint8_t* slots, const int64_t* csr_off, const int32_t* sizes, const int64_t* out_off,
+                          int64_t n, uint8_t* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_barle_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, slots, csr_off, sizes, out_off,
+                       n, out);
     SGPU_HIP(hipGetLastError());
 }
 

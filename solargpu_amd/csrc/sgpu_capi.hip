@@ -479,6 +479,7 @@ void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t
     int64_t* scr = nullptr;
     int64_t* boff = nullptr;
     uint8_t* bytes = nullptr;
+    uint8_t* slots = nullptr;
     int32_t* long_ids = nullptr;
     unsigned* long_count = nullptr;
     try {
@@ -489,15 +490,34 @@ void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t
         ba.long_ids = long_ids;
         ba.long_count = long_count;
         ba.sizes = sizes;
-        launch_barle(ba, false, stream);
+        /* one encoding pass into per-voxel slots (6 B per id + 8 per voxel) when they fit a
+           quarter of the free memory, else a size pass and a write pass */
+        int64_t nnz = 0;
+        SGPU_HIP(hipMemcpyAsync(&nnz, off + N, sizeof nnz, hipMemcpyDeviceToHost, stream));
+        SGPU_HIP(hipStreamSynchronize(stream));
+        const size_t slot_bytes = (size_t)(6 * nnz + 8 * N + 16);
+        size_t free_b = 0, total_b = 0;
+        SGPU_HIP(hipMemGetInfo(&free_b, &total_b));
+        if (slot_bytes <= free_b / 4) {
+            slots = dalloc<uint8_t>(slot_bytes);
+            ba.out = slots;
+            ba.slots = true;
+            launch_barle(ba, true, stream);
+        } else {
+            launch_barle(ba, false, stream);
+        }
         launch_scan_counts(sizes, N, boff, scr, stream);
         int64_t total = 0;
         SGPU_HIP(hipMemcpyAsync(&total, boff + N, sizeof total, hipMemcpyDeviceToHost, stream));
         SGPU_HIP(hipStreamSynchronize(stream));
         bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
-        ba.out_off = boff;
-        ba.out = bytes;
-        launch_barle(ba, true, stream);
+        if (slots) {
+            launch_barle_compact(slots, off, sizes, boff, N, bytes, stream);
+        } else {
+            ba.out_off = boff;
+            ba.out = bytes;
+            launch_barle(ba, true, stream);
+        }
         SGPU_HIP(hipStreamSynchronize(stream));
         *size = total;
     } catch (...) {
@@ -505,12 +525,14 @@ void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t
         dfree(scr);
         dfree(boff);
         dfree(bytes);
+        dfree(slots);
         dfree(long_ids);
         dfree(long_count);
         throw;
     }
     dfree(sizes);
     dfree(scr);
+    dfree(slots);
     dfree(long_ids);
     dfree(long_count);
     *br_off = boff;

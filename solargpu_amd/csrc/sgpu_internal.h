@@ -192,7 +192,14 @@ struct BarleArgs {
     int32_t* long_ids;      // queues of voxels with more than 512 ids (n entries; filled by the size pass):
                             // up to 1024 ids from the front, more from the back
     unsigned* long_count;   // their lengths, 2 entries (zeroed by the size pass)
+    /* One pass instead of size + write: voxel v's stream goes to its slot at byte
+       6 csr_off[v] + 8 v of `out` (at most 5.375 n + 3 bytes for n ids), the write pass stores
+       `sizes` and queues the long lists itself; launch_barle_compact then packs the slots. */
+    bool slots;
 };
+/* Packs one-pass slots (BarleArgs::slots) into the stream at the scanned offsets. */
+void launch_barle_compact(const uint8_t* slots, const int64_t* csr_off, const int32_t* sizes, const int64_t* out_off,
+                          int64_t n, uint8_t* out, hipStream_t s);
 void launch_barle(const BarleArgs& a, bool write_pass, hipStream_t s);
 
 struct HourArgs {
