@@ -491,14 +491,16 @@ void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t
         ba.long_count = long_count;
         ba.sizes = sizes;
         /* one encoding pass into per-voxel slots (6 B per id + 8 per voxel) when they fit a
-           quarter of the free memory, else a size pass and a write pass */
+           quarter of the free memory, else a size pass and a write pass (SGPU_BARLE_PASSES=2
+           forces the latter) */
         int64_t nnz = 0;
         SGPU_HIP(hipMemcpyAsync(&nnz, off + N, sizeof nnz, hipMemcpyDeviceToHost, stream));
         SGPU_HIP(hipStreamSynchronize(stream));
         const size_t slot_bytes = (size_t)(6 * nnz + 8 * N + 16);
         size_t free_b = 0, total_b = 0;
         SGPU_HIP(hipMemGetInfo(&free_b, &total_b));
-        if (slot_bytes <= free_b / 4) {
+        const char* passes = getenv("SGPU_BARLE_PASSES");
+        if (!(passes && passes[0] == '2') && slot_bytes <= free_b / 4) {
             slots = dalloc<uint8_t>(slot_bytes);
             ba.out = slots;
             ba.slots = true;

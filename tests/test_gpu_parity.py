@@ -140,8 +140,12 @@ def test_barle_gpu_encoder_bit_exact():
         np.testing.assert_array_equal(data, edata)
 
 
-def test_barle_encoder_edge_lists():
-    """Crafted lists through the GPU encoder (set_csr -> barle_encode) vs the oracle encoder."""
+@pytest.mark.parametrize("passes", ["1", "2"])
+def test_barle_encoder_edge_lists(passes, monkeypatch):
+    """Crafted lists through the GPU encoder (set_csr -> barle_encode) vs the oracle encoder, in
+    one pass into per-voxel slots (the default) and in a size pass + write pass (the path taken
+    when the slots do not fit)."""
+    monkeypatch.setenv("SGPU_BARLE_PASSES", passes)
     sc = synth_scene(11, 48, 40)
     rng = np.random.default_rng(5)
     lists = [[], [1], [15], [16], [255, 256], [70000, 3], list(range(600, 0, -1)),
