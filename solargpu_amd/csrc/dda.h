@@ -32,7 +32,7 @@ __device__ __forceinline__ float div_by(float a, float b, float rb) {
 /* Steps whose cells and heights are fetched before any of them is tested: the
  * traversal does not depend on the heights, so kDdaUnroll loads are in flight at
  * once; the tests then run in step order, so the first exit wins. */
-constexpr int kDdaUnroll = 2;
+constexpr int kDdaUnroll = 3;
 
 /* One ray of a pooled walk. */
 struct PoolRay {

@@ -378,7 +378,8 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
                 int64_t* boff = nullptr;
                 uint8_t* bytes = nullptr;
                 int64_t size = 0;
-                encode_barle_on(c->stream, off, ids, cnt, &boff, &bytes, &size);
+                encode_barle_on(c->stream, off, ids, cnt, &boff, &bytes, &size, reinterpret_cast<uint8_t*>(raw),
+                                (size_t)chunk * kDirs * sizeof(int32_t));   // the chunk's lists are out of raw
                 bpieces.push_back({bytes, size});
                 const unsigned g = (unsigned)((cnt + 255) / 256);
                 hipLaunchKernelGGL(k_add_base, dim3(g), dim3(256), 0, c->stream, boff, cnt, bbase, out.br_off + v0);
@@ -463,14 +464,16 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
 
 void encode_barle(sgpu_ctx* c, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,
                   uint8_t** br_bytes, int64_t* size) {
-    encode_barle_on(c->stream, off, ids, N, br_off, br_bytes, size);
+    /* the viewshed workspace's raw table is idle once the lists are built */
+    encode_barle_on(c->stream, off, ids, N, br_off, br_bytes, size, reinterpret_cast<uint8_t*>(c->ws_raw),
+                    c->ws_raw ? (size_t)c->ws_chunk * kDirs * sizeof(int32_t) : 0);
 }
 
 }  // namespace
 
 /* ba-RLE streams of N CSR lists: size pass -> int64 scan -> write pass. */
 void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,
-                           uint8_t** br_bytes, int64_t* size) {
+                           uint8_t** br_bytes, int64_t* size, uint8_t* scratch, size_t scratch_bytes) {
     BarleArgs ba{};
     ba.csr_off = off;
     ba.csr_ids = ids;
@@ -490,31 +493,35 @@ void sgpu::encode_barle_on(hipStream_t stream, const int64_t* off, const int32_t
         ba.long_ids = long_ids;
         ba.long_count = long_count;
         ba.sizes = sizes;
-        /* one encoding pass into per-voxel slots (6 B per id + 8 per voxel) when they fit a
-           quarter of the free memory, else a size pass and a write pass (SGPU_BARLE_PASSES=2
-           forces the latter) */
+        /* one encoding pass into per-voxel slots (6 B per id + 8 per voxel) in the caller's
+           scratch, or allocated when they fit a quarter of the free memory; else a size pass and a
+           write pass (SGPU_BARLE_PASSES=2 forces the latter) */
         int64_t nnz = 0;
         SGPU_HIP(hipMemcpyAsync(&nnz, off + N, sizeof nnz, hipMemcpyDeviceToHost, stream));
         SGPU_HIP(hipStreamSynchronize(stream));
         const size_t slot_bytes = (size_t)(6 * nnz + 8 * N + 16);
-        size_t free_b = 0, total_b = 0;
-        SGPU_HIP(hipMemGetInfo(&free_b, &total_b));
         const char* passes = getenv("SGPU_BARLE_PASSES");
-        if (!(passes && passes[0] == '2') && slot_bytes <= free_b / 4) {
-            slots = dalloc<uint8_t>(slot_bytes);
-            ba.out = slots;
+        const bool two_pass = passes && passes[0] == '2';
+        if (!two_pass && scratch && slot_bytes <= scratch_bytes) {
+            ba.out = scratch;
             ba.slots = true;
-            launch_barle(ba, true, stream);
-        } else {
-            launch_barle(ba, false, stream);
+        } else if (!two_pass) {
+            size_t free_b = 0, total_b = 0;
+            SGPU_HIP(hipMemGetInfo(&free_b, &total_b));
+            if (slot_bytes <= free_b / 4) {
+                slots = dalloc<uint8_t>(slot_bytes);
+                ba.out = slots;
+                ba.slots = true;
+            }
         }
+        launch_barle(ba, ba.slots, stream);
         launch_scan_counts(sizes, N, boff, scr, stream);
         int64_t total = 0;
         SGPU_HIP(hipMemcpyAsync(&total, boff + N, sizeof total, hipMemcpyDeviceToHost, stream));
         SGPU_HIP(hipStreamSynchronize(stream));
         bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(total, 1));
-        if (slots) {
-            launch_barle_compact(slots, off, sizes, boff, N, bytes, stream);
+        if (ba.slots) {
+            launch_barle_compact(ba.out, off, sizes, boff, N, bytes, stream);
         } else {
             ba.out_off = boff;
             ba.out = bytes;

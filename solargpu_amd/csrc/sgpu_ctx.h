@@ -256,10 +256,13 @@ void subset_hour_loop(sgpu_ctx* c, const int32_t* subset, int64_t n, const Subse
 void annual_reflected_linear(sgpu_ctx* c, unsigned fmt, const float* dd, float* out_dev);
 /* Apply the autotuner's cached launch shapes for this scene class, if any (no measuring). */
 bool apply_cached_tuning(sgpu_ctx* c, unsigned fmt);
-/* ba-RLE streams of N CSR lists on a stream (size pass, int64 scan, write pass); the caller
-   owns *br_off (N + 1) and *br_bytes (*size bytes). */
+/* ba-RLE streams of N CSR lists on a stream (one encoding pass into per-voxel slots, or a size
+   pass and a write pass; int64 scan); the caller owns *br_off (N + 1) and *br_bytes (*size
+   bytes).  scratch: idle device memory the slots may use (the viewshed's raw table once its
+   lists are out), else they are allocated when they fit. */
 void encode_barle_on(hipStream_t s, const int64_t* off, const int32_t* ids, int64_t N, int64_t** br_off,
-                     uint8_t** br_bytes, int64_t* size);
+                     uint8_t** br_bytes, int64_t* size,
+                     uint8_t* scratch = nullptr, size_t scratch_bytes = 0);
 /* Same walk over a subset's own lists (positions of `subset`); out: n reflected sums. */
 void subset_reflected_linear(sgpu_ctx* c, const int32_t* subset, int64_t n, const SubsetLists& L, unsigned fmt,
                              const float* dd, float* out);
