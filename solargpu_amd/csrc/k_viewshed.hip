@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent_pool(BatchTrac
     __shared__ long long item_s;
     __shared__ int stile[kTraceBlock / 64][64];
     __shared__ float sorg[kTraceBlock / 64][3][64];
-    constexpr int G = 32;
+    constexpr int G = kPersistentDirs;
     const int64_t groups = kDirs / G;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (;;) {

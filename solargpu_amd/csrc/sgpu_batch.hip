@@ -203,7 +203,7 @@ void viewshed(sgpu_batch* b) {
         int64_t base = 0;
         for (int64_t v0 = 0; v0 < N; v0 += chunk) {
             const int64_t cnt = std::min(chunk, N - v0);
-            const int64_t items = (cnt + 255) / 256 * (kDirs / 32);
+            const int64_t items = (cnt + 255) / 256 * (kDirs / kPersistentDirs);
             SGPU_HIP(hipMemsetAsync(next, 0, sizeof(unsigned long long), s));
             BatchTraceArgs ta{tiles_d, nt, v0, cnt, raw, next, items};
             SGPU_HIP(hipEventRecord(et.a, s));

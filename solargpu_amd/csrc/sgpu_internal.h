@@ -83,6 +83,8 @@ std::vector<std::pair<int, int>> hilbert_tiles(int tw, int th);
 std::vector<int32_t> trace_order(int64_t n_total, const float* grid, const int32_t* pix_parent, const float* wall_z,
                                  int64_t n_top, int block);
 constexpr int kTraceOrderBlock = 512;
+/* Directions per item of the persistent batched trace (a divisor of the 2592). */
+constexpr int kPersistentDirs = 32;
 void synth_weather(uint64_t seed, const float* az_alt, float* dni, float* dhi);
 
 /* ---- device kernels (k_viewshed.hip, k_barle.hip, k_hourly.hip) ---- */

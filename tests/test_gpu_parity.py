@@ -169,6 +169,25 @@ def test_barle_encoder_edge_lists(passes, monkeypatch):
     np.testing.assert_array_equal(data, edata)
 
 
+def test_repeated_viewsheds_reuse_the_workspace():
+    """Viewsheds on one context keep their workspace (raw table, counts, offsets) between calls
+    and the encoder borrows its raw table for its slots; a smaller chunk reuses it, a larger one
+    replaces it: the lists and streams stay identical."""
+    sc = synth_scene(13, 37, 53)
+    with _ctx(sc, chunk=512) as ctx:
+        off0, ids0 = ctx.csr()
+        boff0, data0 = ctx.barle()
+        for chunk in (256, 2048, 512):
+            ctx.set_chunk_voxels(chunk)
+            ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
+            off, ids = ctx.csr()
+            boff, data = ctx.barle()
+            np.testing.assert_array_equal(off, off0)
+            np.testing.assert_array_equal(ids, ids0)
+            np.testing.assert_array_equal(boff, boff0)
+            np.testing.assert_array_equal(data, data0)
+
+
 def test_chunking_is_results_neutral():
     sc = synth_scene(13, 37, 53)
     ref = oracle_lists(13, 37, 53)
