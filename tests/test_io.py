@@ -90,3 +90,41 @@ def test_voxel_dump_layout(tmp_path):
     np.testing.assert_array_equal(rec[:20, 2], h.ravel())
     np.testing.assert_array_equal(rec[20:, 2], sc.wall_z)
     np.testing.assert_array_equal(rec[:, 3], irr)
+
+
+@pytest.mark.gpu
+def test_cli_matches_drop_in_entry_point(tmp_path):
+    """The drop-in CLI with the reference's arguments (DSM, lon, lat, tz, direct and diffuse
+    series) on a seeded DSM written as a GeoTIFF: its solar-potential raster and voxel dump hold
+    the ba-RLE pipeline's annual sums, those of compressed_gpu_irradiance on the same scene (to
+    FP32 regrouping of the hours, should the processes' cached launch shapes differ)."""
+    import subprocess
+    from PIL import Image
+    from PIL.TiffImagePlugin import ImageFileDirectory_v2
+    cli = os.path.join(os.path.dirname(sg.__file__), "bin", "solargpu")
+    assert os.path.exists(cli), "the CLI is built with the library (make)"
+    heights = sg.synth_dsm(41, 48, 40, 1.0, 0.35, 3)
+    ifd = ImageFileDirectory_v2()
+    ifd[33550] = (1.0, 1.0, 0.0)
+    ifd.tagtype[33550] = 12
+    dsm = str(tmp_path / "scene.tif")
+    Image.fromarray(heights, mode="F").save(dsm, tiffinfo=ifd)
+    lon, lat, tz = 15.65, 46.56, 1
+    az_alt = sg.sun_positions(lon, lat, tz)
+    dni, dhi = sg.synth_weather(41, az_alt)
+    np.savetxt(tmp_path / "direct.txt", dni, fmt="%.9g")
+    np.savetxt(tmp_path / "diffuse.txt", dhi, fmt="%.9g")
+    r = subprocess.run([cli, dsm, str(lon), str(lat), str(tz), "direct.txt", "diffuse.txt"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sc = sg.assemble_scene(heights, 1.0)
+    want = sg.compressed_gpu_irradiance(sc, az_alt, dni, dhi)
+    H, W = heights.shape
+    raster = sg.GeoTiff(str(tmp_path / "scene.tif_solar_potential.tiff")).read()
+    assert raster.shape == (H, W)
+    np.testing.assert_allclose(raster.ravel(), want[: W * H], rtol=1e-5, atol=1e-3)
+    raw = open(tmp_path / "scene.tif_voxels.bin", "rb").read()
+    n_total, n_top = np.frombuffer(raw[:8], np.int32)
+    assert (n_total, n_top) == (sc.num_total, W * H)
+    rec = np.frombuffer(raw[8:], np.float32).reshape(-1, 4)
+    np.testing.assert_allclose(rec[:, 3], want, rtol=1e-5, atol=1e-3)
