@@ -53,3 +53,33 @@ def test_context_without_gpu_reports_error():
     assert st in (1, 2)
     lib.sgpu_last_error.restype = ctypes.c_char_p
     assert lib.sgpu_last_error()
+
+
+def test_headers_compile_link_and_run(tmp_path):
+    """What a reference build does with the headers: include/sgpu.h compiled as C and
+    include/solar_gpu_api.h + include/sun_position.h as C++, linked against libsolargpu.so, and
+    the host-side entry points run (sun positions through both)."""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.join(ROOT, "solargpu_amd", "lib")
+    link = ["-L" + libdir, "-Wl,-rpath," + libdir, "-lsolargpu", "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+    (tmp_path / "t.c").write_text(
+        '#include "sgpu.h"\n'
+        "static float az_alt[2 * SGPU_HOURS_PER_YEAR];\n"
+        "int main(void) { return sgpu_sun_positions(15.65f, 46.56f, 1, az_alt) != 0 || az_alt[2 * 12 + 1] <= 0.0f; }\n")
+    (tmp_path / "t.cpp").write_text(
+        '#include <vector>\n#include "solar_gpu_api.h"\n#include "sun_position.h"\n'
+        "int main() {\n"
+        "    std::vector<solar_gpu::SunSample> s;\n"
+        "    solar_gpu::SunPositionCalculator(15.65f, 46.56f, 1).calc(s);\n"
+        "    auto f = &solar_gpu::compressed_gpu_irradiance;\n"
+        "    auto g = &solar_gpu::uncompressed_gpu_irradiance;\n"
+        "    return (s.size() != 8760 || !f || !g) ? 1 : 0;\n"
+        "}\n")
+    for cc, src, std in (("gcc", "t.c", "-std=c11"), ("g++", "t.cpp", "-std=c++17")):
+        exe = str(tmp_path / (src + ".bin"))
+        r = subprocess.run([cc, std, "-Wall", "-Werror", "-I" + inc, str(tmp_path / src), "-o", exe] + link,
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, (src, r.stdout, r.stderr)
