@@ -249,12 +249,18 @@ __global__ __launch_bounds__(kListThreads) void k_lists_batch(BatchListArgs a) {
     for (int i = 0; i < nvox; ++i) {
         const int64_t local = first + i;
         const int64_t g = a.v0 + local;
-        if (threadIdx.x == 0) {
-            int lo = 0, hi = a.ntiles - 1;   // last tile with vbase <= g
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (a.tiles[mid].vbase <= g) lo = mid;
-                else hi = mid - 1;
+        if (threadIdx.x == 0) {   // last tile with vbase <= g
+            int lo = 0;
+            if (i == 0) {
+                int hi = a.ntiles - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (a.tiles[mid].vbase <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+            } else {   // the voxels are consecutive: from the previous voxel's tile on
+                lo = tile_s;
+                while (lo + 1 < a.ntiles && a.tiles[lo + 1].vbase <= g) ++lo;
             }
             tile_s = lo;   // the previous voxel's reads of tile_s precede voxel_list's barriers
         }
