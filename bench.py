@@ -279,7 +279,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": f"k_reflect<K={Kp}, ba-RLE>", "avg_launch_ms": avg_launch * 1e3,
-                     "algorithmic_bytes_per_launch": bytes_all / passes},
+                     "algorithmic_bytes_per_launch": bytes_all / passes,
+                     # what binds the batched gather: its L2-miss traffic (the PMC figure above) per
+                     # launch time, against the copy bandwidth measured in this run
+                     "traffic_gbps": (traffic / avg_launch / 1e9) if traffic else None,
+                     "traffic_frac_of_copy": (traffic / avg_launch / 1e9 / copy_gbps) if traffic else None},
         "measured_copy_gbps": copy_gbps,
     }
     res.update(extra)
