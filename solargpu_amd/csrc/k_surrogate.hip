@@ -4,7 +4,8 @@
  *   k_annual_dd  exact annual direct + diffuse per voxel (FP32 Kahan over all
  *                sunlit hours) and the diffuse part alone -- the solar kernel
  *                of solar_kernel.cu:54-77 without the reflection term;
- *   k_features   32-sector horizon march + the 53 features of the MLP, bf16;
+ *   k_horizon_pool  32-sector horizon march per voxel, pooled over voxels and sectors;
+ *   k_features   the 53 features of the MLP from the march results, bf16;
  *   k_gate       surrogate estimate, predicted error, fallback flags;
  *   k_combine    scatter of the exact fallback totals. */
 #include <cmath>
@@ -133,38 +134,111 @@ __global__ __launch_bounds__(256) void k_annual_dd(SurArgs a) {
     a.diff[v] = sd;
 }
 
-/* Horizon of one azimuth sector: march cell by cell from the cell centre along
- * (sin az, -cos az) (azimuth clockwise from north = -y, as the ray directions),
- * tracking the steepest tan(elevation) = (h - z0) / distance (distance in cells,
- * heights in metres, the geometry of the viewshed rays).  Stops at the grid edge,
- * at 1024 cells, or when even the highest cell could no longer raise it. */
-__device__ __forceinline__ void horizon(const SurArgs& a, int gx, int gy, float z0, float sx, float sy,
-                                        float& best, int& best_cell) {
-    const float ox = gx + 0.5f, oy = gy + 0.5f;
-    int ix = gx, iy = gy;
-    const float ax = fmaxf(fabsf(sx), 1e-8f), ay = fmaxf(fabsf(sy), 1e-8f);
-    float tmx = ((sx >= 0.0f) ? (ix + 1.0f - ox) : (ox - ix)) / ax;
-    float tmy = ((sy >= 0.0f) ? (iy + 1.0f - oy) : (oy - iy)) / ay;
-    const float rx = 1.0f / ax, ry = 1.0f / ay;
-    const int stx = (sx >= 0.0f) ? 1 : -1, sty = (sy >= 0.0f) ? 1 : -1;
-    best = -1e30f;
-    best_cell = -1;
-    const float top = a.maxz - z0;
-    for (int k = 0; k < 2048; ++k) {
-        float t;
-        if (tmx < tmy) { t = tmx; ix += stx; tmx += rx; }
-        else { t = tmy; iy += sty; tmy += ry; }
-        if ((unsigned)ix >= (unsigned)a.W || (unsigned)iy >= (unsigned)a.H || t > 1024.0f) break;
-        const int cell = ix + iy * a.W;
-        const float tn = (a.grid[cell] - z0) / fmaxf(t, 0.5f);
-        if (tn > best) { best = tn; best_cell = cell; }
-        if (top <= best * t) break;
+/* Horizon of every (voxel, azimuth sector) of a chunk: march cell by cell from the cell centre
+ * along (sin az, -cos az) (azimuth clockwise from north = -y, as the ray directions), tracking
+ * the steepest tan(elevation) = (h - z0) / distance (distance in cells, heights in metres, the
+ * geometry of the viewshed rays); stop at the grid edge, at 1024 cells or 2048 steps, or when
+ * even the highest cell could no longer raise it.  hz gets (best tangent bits, its cell; -1 when
+ * no cell was passed).  The marches are pooled like the DDA: each wave owns the 32 x 64 marches
+ * of its 64 voxels, sector-major, and a lane whose march has ended takes the next once
+ * kSunRefill lanes are idle (or none is busy), so the wave does not wait for its longest march
+ * of every sector. */
+__global__ __launch_bounds__(256) void k_horizon_pool(SurArgs a) {
+    __shared__ int sgx[4][64], sgy[4][64];
+    __shared__ float sz[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t vwave = a.hz_v0 + (int64_t)blockIdx.x * 256 + wave * 64;   // the wave's first voxel
+    const int nvalid = (int)min<int64_t>(64, max<int64_t>(0, a.hz_v0 + a.hz_count - vwave));
+    if (lane < nvalid) {
+        int top, gx, gy;
+        float z;
+        voxel_origin(a, vwave + lane, top, gx, gy, z);
+        sgx[wave][lane] = gx;
+        sgy[wave][lane] = gy;
+        sz[wave][lane] = z;
+    }
+    __syncthreads();
+    if (nvalid <= 0) return;
+    bool active = false;
+    int ix = 0, iy = 0, stx = 0, sty = 0, k = 0, bcell = -1, rvx = 0, rs = 0;
+    float tmx = 0.0f, tmy = 0.0f, rx = 0.0f, ry = 0.0f, best = 0.0f, z0 = 0.0f, top = 0.0f;
+    const int total = kSectors * 64;
+    int next = 0;   // wave-uniform pool cursor
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (next < total && (nidle >= kSunRefill || nidle == 64)) {
+            if (!active) {
+                const int j = next + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                const int sec = j >> 6, vx = j & 63;
+                if (j < total && vx < nvalid) {
+                    const float az = (sec + 0.5f) * (6.28318531f / kSectors);
+                    float sx, cy;
+                    sincosf(az, &sx, &cy);
+                    const float sy = -cy;
+                    const int gx = sgx[wave][vx], gy = sgy[wave][vx];
+                    z0 = sz[wave][vx];
+                    const float ox = gx + 0.5f, oy = gy + 0.5f;
+                    ix = gx;
+                    iy = gy;
+                    const float ax = fmaxf(fabsf(sx), 1e-8f), ay = fmaxf(fabsf(sy), 1e-8f);
+                    tmx = ((sx >= 0.0f) ? (ix + 1.0f - ox) : (ox - ix)) / ax;
+                    tmy = ((sy >= 0.0f) ? (iy + 1.0f - oy) : (oy - iy)) / ay;
+                    rx = 1.0f / ax;
+                    ry = 1.0f / ay;
+                    stx = (sx >= 0.0f) ? 1 : -1;
+                    sty = (sy >= 0.0f) ? 1 : -1;
+                    best = -1e30f;
+                    bcell = -1;
+                    top = a.maxz - z0;
+                    k = 0;
+                    rvx = vx;
+                    rs = sec;
+                    active = true;
+                }
+            }
+            next += nidle;
+        }
+        if (__ballot(active) == 0) {
+            if (next >= total) break;
+            continue;
+        }
+        if (active) {
+            bool end;
+            float t;
+            if (tmx < tmy) {
+                t = tmx;
+                ix += stx;
+                tmx += rx;
+            } else {
+                t = tmy;
+                iy += sty;
+                tmy += ry;
+            }
+            if ((unsigned)ix >= (unsigned)a.W || (unsigned)iy >= (unsigned)a.H || t > 1024.0f) {
+                end = true;
+            } else {
+                const int cell = ix + iy * a.W;
+                const float tn = (a.grid[cell] - z0) / fmaxf(t, 0.5f);
+                if (tn > best) {
+                    best = tn;
+                    bcell = cell;
+                }
+                end = top <= best * t;
+            }
+            if (++k >= 2048) end = true;
+            if (end) {
+                a.hz[(vwave - a.hz_v0 + rvx) * kSectors + rs] = make_int2(__float_as_int(best), bcell);
+                active = false;
+            }
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_features(SurArgs a) {
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= a.n_total) return;
+    const int64_t v = a.hz_v0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= a.hz_v0 + a.hz_count) return;
     int top, gx, gy;
     float z;
     voxel_origin(a, v, top, gx, gy, z);
@@ -181,9 +255,9 @@ __global__ __launch_bounds__(256) void k_features(SurArgs a) {
         const float az = (s + 0.5f) * (6.28318531f / kSectors);
         float sx, cy;
         sincosf(az, &sx, &cy);
-        float best;
-        int cell;
-        horizon(a, gx, gy, z, sx, -cy, best, cell);
+        const int2 hit = a.hz[(v - a.hz_v0) * kSectors + s];   // the sector's march (k_horizon_pool)
+        const float best = __int_as_float(hit.x);
+        const int cell = hit.y;
         const float ang = (cell >= 0) ? atanf(best) : 0.0f;
         row[s] = f2bf(ang * 0.63661977f);
         const float se = sinf(fmaxf(ang, 0.0f));
@@ -302,9 +376,18 @@ void launch_annual_dd(const SurArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_annual_dd, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);
     SGPU_HIP(hipGetLastError());
 }
-void launch_features(const SurArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_features, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);
-    SGPU_HIP(hipGetLastError());
+void launch_features(const SurArgs& a, int2* hz, int64_t hz_cap, hipStream_t s) {
+    if (hz_cap <= 0) fail(SGPU_ERR_ARG, "horizon buffer missing");
+    for (int64_t v0 = 0; v0 < a.n_total; v0 += hz_cap) {
+        SurArgs c = a;
+        c.hz = hz;
+        c.hz_v0 = v0;
+        c.hz_count = std::min<int64_t>(hz_cap, a.n_total - v0);
+        const dim3 g((unsigned)((c.hz_count + 255) / 256));
+        hipLaunchKernelGGL(k_horizon_pool, g, dim3(256), 0, s, c);
+        hipLaunchKernelGGL(k_features, g, dim3(256), 0, s, c);
+        SGPU_HIP(hipGetLastError());
+    }
 }
 void launch_gate(const SurArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_gate, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);

@@ -101,8 +101,12 @@ struct SurrogateState {
     float* diff = nullptr;       // annual diffuse
     uint16_t* sun_hits = nullptr;
     int16_t* sun_dirs = nullptr;
+    int2* hz = nullptr;          // horizon march results of a chunk of voxels
+    int64_t hz_cap = 0;          // voxels it holds
     bool ready = false;
     void release() {
+        dfree(hz);
+        hz_cap = 0;
         dfree(X);
         dfree(dd);
         dfree(diff);

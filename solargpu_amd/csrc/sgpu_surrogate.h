@@ -36,11 +36,17 @@ struct SurArgs {
     float* out;                // surrogate annual total
     int32_t* fallback;         // 1 = take the exact plain result
     float tau, refl_rel_err, diff_miss_weight, share_abs_err;
+    /* horizon march results of voxels [hz_v0, hz_v0 + hz_count): (best tangent bits, cell) per
+       sector, set by launch_features per chunk */
+    int2* hz;
+    int64_t hz_v0, hz_count;
 };
 
 void launch_sun_svf(const SurArgs& a, hipStream_t s);
 void launch_annual_dd(const SurArgs& a, hipStream_t s);
-void launch_features(const SurArgs& a, hipStream_t s);
+/* The 32-sector horizon march (pooled over voxels and sectors) and the feature rows, over chunks
+   of at most hz_cap voxels whose march results go to hz (hz_cap x 32 entries). */
+void launch_features(const SurArgs& a, int2* hz, int64_t hz_cap, hipStream_t s);
 void launch_gate(const SurArgs& a, hipStream_t s);
 void launch_scatter(const int32_t* ids, const float* vals, int64_t n, float* out, hipStream_t s);
 /* ids[off[v]] = v for every flagged v (off = exclusive scan of the flags). */

@@ -78,7 +78,11 @@ void prepare(sgpu_ctx* c, float lat, float cell_m, sgpu_surrogate_stats* st) {
     launch_annual_dd(a, c->stream);
     SGPU_HIP(hipEventRecord(e2.b, c->stream));
     SGPU_HIP(hipEventRecord(e3.a, c->stream));
-    launch_features(a, c->stream);
+    if (!s.hz) {
+        s.hz_cap = std::min<int64_t>(N, (int64_t)1 << 24);   // 4 GB of march results at most
+        s.hz = dalloc<int2>((size_t)s.hz_cap * kSectors);
+    }
+    launch_features(a, s.hz, s.hz_cap, c->stream);
     SGPU_HIP(hipEventRecord(e3.b, c->stream));
     SGPU_HIP(hipStreamSynchronize(c->stream));
     c->svf_ok = true;
