@@ -143,6 +143,8 @@ __global__ __launch_bounds__(256) void k_annual_dd(SurArgs a) {
  * of its 64 voxels, sector-major, and a lane whose march has ended takes the next once
  * kSunRefill lanes are idle (or none is busy), so the wave does not wait for its longest march
  * of every sector. */
+constexpr int kHorizonUnroll = 3;
+
 __global__ __launch_bounds__(256) void k_horizon_pool(SurArgs a) {
     __shared__ int sgx[4][64], sgy[4][64];
     __shared__ float sz[4][64];
@@ -205,29 +207,46 @@ __global__ __launch_bounds__(256) void k_horizon_pool(SurArgs a) {
             continue;
         }
         if (active) {
-            bool end;
-            float t;
-            if (tmx < tmy) {
-                t = tmx;
-                ix += stx;
-                tmx += rx;
-            } else {
-                t = tmy;
-                iy += sty;
-                tmy += ry;
-            }
-            if ((unsigned)ix >= (unsigned)a.W || (unsigned)iy >= (unsigned)a.H || t > 1024.0f) {
-                end = true;
-            } else {
-                const int cell = ix + iy * a.W;
-                const float tn = (a.grid[cell] - z0) / fmaxf(t, 0.5f);
-                if (tn > best) {
-                    best = tn;
-                    bcell = cell;
+            // kHorizonUnroll steps per pool turn: the cells depend only on the traversal, so the
+            // heights of all of them are fetched before the first is tested (an off-grid or
+            // too-far step fetches cell 0, unused); the tests then run in step order, so the
+            // result equals that of one step per turn.
+            int cells[kHorizonUnroll];
+            float ts[kHorizonUnroll], hs[kHorizonUnroll];
+            bool off[kHorizonUnroll];
+#pragma unroll
+            for (int u = 0; u < kHorizonUnroll; ++u) {
+                if (tmx < tmy) {
+                    ts[u] = tmx;
+                    ix += stx;
+                    tmx += rx;
+                } else {
+                    ts[u] = tmy;
+                    iy += sty;
+                    tmy += ry;
                 }
-                end = top <= best * t;
+                off[u] = (unsigned)ix >= (unsigned)a.W || (unsigned)iy >= (unsigned)a.H || ts[u] > 1024.0f;
+                cells[u] = off[u] ? 0 : ix + iy * a.W;
             }
-            if (++k >= 2048) end = true;
+#pragma unroll
+            for (int u = 0; u < kHorizonUnroll; ++u) hs[u] = a.grid[cells[u]];
+            bool end = false;
+#pragma unroll
+            for (int u = 0; u < kHorizonUnroll; ++u) {
+                if (!end) {
+                    if (off[u]) {
+                        end = true;
+                    } else {
+                        const float tn = (hs[u] - z0) / fmaxf(ts[u], 0.5f);
+                        if (tn > best) {
+                            best = tn;
+                            bcell = cells[u];
+                        }
+                        end = top <= best * ts[u];
+                    }
+                    if (++k >= 2048) end = true;
+                }
+            }
             if (end) {
                 a.hz[(vwave - a.hz_v0 + rvx) * kSectors + rs] = make_int2(__float_as_int(best), bcell);
                 active = false;
