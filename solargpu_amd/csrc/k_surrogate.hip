@@ -104,34 +104,55 @@ __global__ __launch_bounds__(256) void k_sun_svf_pool(SurArgs a) {
     }
 }
 
+constexpr int kDdVoxels = 2;   // voxels per lane in k_annual_dd
+
 __global__ __launch_bounds__(256) void k_annual_dd(SurArgs a) {
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= a.n_total) return;
-    float ss, cs, sa, ca;
-    sincosf(a.slopes[v], &ss, &cs);
-    sincosf(a.aspects[v], &sa, &ca);
-    float s = 0.0f, c = 0.0f, sd = 0.0f, cd = 0.0f;
+    // Each lane sums kDdVoxels voxels 256 apart, so every hour's wave-uniform constants serve
+    // several independent Kahan chains; each voxel's sums keep hour order.
+    int64_t v[kDdVoxels];
+    bool ok[kDdVoxels];
+    float ss[kDdVoxels], cs[kDdVoxels], sa[kDdVoxels], ca[kDdVoxels];
+    float s[kDdVoxels], c[kDdVoxels], sd[kDdVoxels], cd[kDdVoxels];
+#pragma unroll
+    for (int i = 0; i < kDdVoxels; ++i) {
+        v[i] = (int64_t)blockIdx.x * (256 * kDdVoxels) + i * 256 + threadIdx.x;
+        ok[i] = v[i] < a.n_total;
+        const int64_t vl = ok[i] ? v[i] : 0;   // a voxel past the end computes voxel 0, unstored
+        sincosf(a.slopes[vl], &ss[i], &cs[i]);
+        sincosf(a.aspects[vl], &sa[i], &ca[i]);
+        v[i] = vl;
+        s[i] = c[i] = sd[i] = cd[i] = 0.0f;
+    }
     for (int h = 0; h < a.n_hours; ++h) {
         const HourConst& hc = a.hours[h];   // wave-uniform
-        const float cdiff = hc.cos_az * ca + hc.sin_az * sa;
-        const float cos_aoi = hc.cos_z * cs + hc.sin_z * ss * cdiff;
-        bool shaded = false;
-        if (hc.dense >= 0) shaded = (a.svf[(int64_t)(hc.dense >> 3) * a.n_total + v] >> (hc.dense & 7)) & 1;
-        float dir = 0.0f;
-        if (cos_aoi > 0.0f && hc.dense >= 0 && !shaded) dir = fmaxf(0.0f, hc.dni_eff * cos_aoi);
-        const float aa = fmaxf(0.0f, cos_aoi);
-        const float dif = fmaxf(0.0f, hc.dhi * ((1 - hc.f1) * ((1 + cs) / 2) + hc.f1 * (aa * hc.inv_b) + hc.f2 * ss));
-        const float x = dir + dif;
-        float y = x - c, t = s + y;
-        c = (t - s) - y;
-        s = t;
-        y = dif - cd;
-        t = sd + y;
-        cd = (t - sd) - y;
-        sd = t;
+#pragma unroll
+        for (int i = 0; i < kDdVoxels; ++i) {
+            const float cdiff = hc.cos_az * ca[i] + hc.sin_az * sa[i];
+            const float cos_aoi = hc.cos_z * cs[i] + hc.sin_z * ss[i] * cdiff;
+            bool shaded = false;
+            if (hc.dense >= 0) shaded = (a.svf[(int64_t)(hc.dense >> 3) * a.n_total + v[i]] >> (hc.dense & 7)) & 1;
+            float dir = 0.0f;
+            if (cos_aoi > 0.0f && hc.dense >= 0 && !shaded) dir = fmaxf(0.0f, hc.dni_eff * cos_aoi);
+            const float aa = fmaxf(0.0f, cos_aoi);
+            const float dif =
+                fmaxf(0.0f, hc.dhi * ((1 - hc.f1) * ((1 + cs[i]) / 2) + hc.f1 * (aa * hc.inv_b) + hc.f2 * ss[i]));
+            const float x = dir + dif;
+            float y = x - c[i], t = s[i] + y;
+            c[i] = (t - s[i]) - y;
+            s[i] = t;
+            y = dif - cd[i];
+            t = sd[i] + y;
+            cd[i] = (t - sd[i]) - y;
+            sd[i] = t;
+        }
     }
-    a.dd[v] = s;
-    a.diff[v] = sd;
+#pragma unroll
+    for (int i = 0; i < kDdVoxels; ++i) {
+        if (ok[i]) {
+            a.dd[v[i]] = s[i];
+            a.diff[v[i]] = sd[i];
+        }
+    }
 }
 
 /* Horizon of every (voxel, azimuth sector) of a chunk: march cell by cell from the cell centre
@@ -392,7 +413,8 @@ void launch_sun_svf(const SurArgs& a, hipStream_t s) {
     SGPU_HIP(hipGetLastError());
 }
 void launch_annual_dd(const SurArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_annual_dd, dim3((unsigned)((a.n_total + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_annual_dd, dim3((unsigned)((a.n_total + 256 * kDdVoxels - 1) / (256 * kDdVoxels))), dim3(256), 0,
+                       s, a);
     SGPU_HIP(hipGetLastError());
 }
 void launch_features(const SurArgs& a, int2* hz, int64_t hz_cap, hipStream_t s) {
