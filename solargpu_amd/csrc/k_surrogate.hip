@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_annual_dd(SurArgs a) {
  * of its 64 voxels, sector-major, and a lane whose march has ended takes the next once
  * kSunRefill lanes are idle (or none is busy), so the wave does not wait for its longest march
  * of every sector. */
-constexpr int kHorizonUnroll = 3;
+constexpr int kHorizonUnroll = 8;
 
 __global__ __launch_bounds__(256) void k_horizon_pool(SurArgs a) {
     __shared__ int sgx[4][64], sgy[4][64];
