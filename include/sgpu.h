@@ -189,6 +189,10 @@ int sgpu_hour_single(sgpu_ctx* ctx, unsigned format, int hour_index, float* hour
                      float* reflected_out);
 /* Time each reflection / solar launch with HIP events on the context stream (new). */
 int sgpu_set_kernel_timing(sgpu_ctx* ctx, int enable);
+/* Diagnostic (new; tests): the hour passes may use at most `floats` of the hourly irradiance
+   buffer (0 = all of it).  A pass whose n_top x hours_per_pass rows do not fit is refused with
+   SGPU_ERR_STATE before its kernels launch. */
+int sgpu_set_hourly_limit(sgpu_ctx* ctx, int64_t floats);
 /* Count DDA steps in the trace kernel (slower; new). */
 int sgpu_set_step_counting(sgpu_ctx* ctx, int enable);
 /* Synchronise the context stream. */

@@ -28,11 +28,22 @@
  *   orc_barle_encode_*  golden streams from the reference's barle_compression.cpp
  *   orc_barle_decode    round trip on the golden streams and on random lists
  *   orc_solar_hour      closed forms (flat unshaded voxel, shaded voxel = diffuse
- *                       only) against an independent float64 Perez
- *   orc_reflect_hour_*  closed-form two-cell case
+ *                       only) against an independent float64 Perez; tilted planes
+ *                       (tests/test_oracle_tilted.py): the beam on walls and slopes
+ *                       facing the sun, away from it and due south at noon from the
+ *                       geometric angle of incidence, and the Perez F1/F2 recovered
+ *                       per hour from two vertical walls and fitted per clearness
+ *                       bin against the published Perez (1990) coefficient table
+ *   orc_reflect_hour_*  closed-form two-cell case; receivers tilted and facing the
+ *                       four compass directions, normals written from geometry
  *   orc_annual_fp32 /
  *   orc_annual_fp32_range  agreement with the FP64 oracle within 1e-4
- *   orc_annual_fp64     linearity identity vs the per-hour FP64 sum on tiny scenes
+ *   orc_annual_fp64     linearity identity vs the per-hour FP64 sum on tiny scenes;
+ *                       agreement with the pinned FP32 terms within 1e-4 on scenes
+ *                       with walls (slope pi/2, every aspect)
+ * Planted mistakes (degrees for radians in the incidence angle, a swapped or
+ * negated normal component, a sign or a 1e-3 typo in the F2 row, the tilt
+ * factor of the sky term) were each run against these tests and each failed.
  */
 #pragma once
 #include <stdint.h>

@@ -343,6 +343,10 @@ class Context:
     def set_kernel_timing(self, on: bool) -> None:
         _check(_lib.sgpu_set_kernel_timing(self._h, int(on)))
 
+    def set_hourly_limit(self, floats: int) -> None:
+        """Diagnostic: cap the hourly buffer the hour passes may use (0 = all of it)."""
+        _check(_lib.sgpu_set_hourly_limit(self._h, ctypes.c_int64(floats)))
+
     def set_step_counting(self, on: bool) -> None:
         _check(_lib.sgpu_set_step_counting(self._h, int(on)))
 

@@ -140,9 +140,17 @@ GeoTiff read_geotiff(const std::string& path) {
         Entry e{r.u16(o), r.u16(o + 2), r.u32(o + 4), 0};
         const size_t sz = type_size(e.type) * e.count;
         e.data = (sz <= 4) ? o + 8 : r.u32(o + 8);
+        /* a tag of a type this reader does not know is kept but never read (typed() refuses it) */
         if (type_size(e.type) && e.data + sz > b.size()) fail(SGPU_ERR_FORMAT, "%s: tag %u out of file", path.c_str(), e.tag);
         es.push_back(e);
     }
+    /* A tag whose values are read as `type`: its type must be that one (so its byte range was
+       checked above against the file size). */
+    auto typed = [&](const Entry* e, uint16_t type) -> const Entry* {
+        if (e && e->type != type)
+            fail(SGPU_ERR_FORMAT, "%s: tag %u has type %u, expected %u", path.c_str(), e->tag, e->type, type);
+        return e;
+    };
     auto find = [&](uint16_t tag) -> const Entry* {
         for (const Entry& e : es)
             if (e.tag == tag) return &e;
@@ -164,16 +172,16 @@ GeoTiff read_geotiff(const std::string& path) {
              path.c_str());
     if (g.compression != 1 && g.compression != 8 && g.compression != 32946)
         fail(SGPU_ERR_FORMAT, "%s: unsupported compression %d", path.c_str(), g.compression);
-    if (const Entry* e = find(33550))
+    if (const Entry* e = typed(find(33550), 12))
         for (uint32_t i = 0; i < e->count; ++i) g.pixel_scale.push_back(r.f64(e->data + 8 * i));
-    if (const Entry* e = find(33922))
+    if (const Entry* e = typed(find(33922), 12))
         for (uint32_t i = 0; i < e->count; ++i) g.tiepoints.push_back(r.f64(e->data + 8 * i));
-    if (const Entry* e = find(34735))
+    if (const Entry* e = typed(find(34735), 3))
         for (uint32_t i = 0; i < e->count; ++i) g.geo_keys.push_back(r.u16(e->data + 2 * i));
-    if (const Entry* e = find(34736))
+    if (const Entry* e = typed(find(34736), 12))
         for (uint32_t i = 0; i < e->count; ++i) g.geo_doubles.push_back(r.f64(e->data + 8 * i));
-    if (const Entry* e = find(34737)) g.geo_ascii.assign((const char*)&b[e->data], e->count);
-    if (const Entry* e = find(42113)) g.nodata.assign((const char*)&b[e->data], e->count);
+    if (const Entry* e = typed(find(34737), 2)) g.geo_ascii.assign((const char*)b.data() + e->data, e->count);
+    if (const Entry* e = typed(find(42113), 2)) g.nodata.assign((const char*)b.data() + e->data, e->count);
     g.data.assign((size_t)g.width * g.height, 0.0f);
     const bool tiled = find(322) != nullptr;
     const int bw = tiled ? (int)one(322, 0) : g.width;
@@ -194,10 +202,12 @@ GeoTiff read_geotiff(const std::string& path) {
             const int rows = tiled ? bh : std::min(bh, g.height - by * bh);
             const size_t expect = (size_t)bw * rows * 4;
             if (offs[k] + cnts[k] > b.size()) fail(SGPU_ERR_FORMAT, "%s: block %d out of file", path.c_str(), k);
+            if (g.compression == 1 && cnts[k] < expect)   // before any byte of the block is read
+                fail(SGPU_ERR_FORMAT, "%s: block %d holds %llu bytes, expected %zu", path.c_str(), k,
+                     (unsigned long long)cnts[k], expect);
             std::vector<uint8_t> blk = (g.compression == 1)
                                            ? std::vector<uint8_t>(b.begin() + offs[k], b.begin() + offs[k] + expect)
-                                           : inflate_all(&b[offs[k]], cnts[k], expect);
-            if (g.compression == 1 && cnts[k] < expect) fail(SGPU_ERR_FORMAT, "%s: short strip", path.c_str());
+                                           : inflate_all(b.data() + offs[k], cnts[k], expect);
             unpredict(blk, bw, rows, predictor, r.le);
             for (int y = 0; y < rows && by * bh + y < g.height; ++y) {
                 const int cw = std::min(bw, g.width - bx * bw);

@@ -92,8 +92,9 @@ struct Recv {
     float gx, gy, z, nx, ny, nz;
 };
 
-/* Division of a cell id by the grid width via a float reciprocal and one fix-up
- * (exact for ids < 2^26 and widths <= 8192). */
+/* Division of a cell id by the grid width via a float reciprocal and one fix-up: exact on
+ * every supported grid (grid_supported: at most 2^22 rows, ids below 2^30), where the float
+ * estimate is off by at most one row. */
 __device__ __forceinline__ void cell_xy(int id, int W, float inv_w, int& x, int& y) {
     int q = (int)((float)id * inv_w);
     int r = id - q * W;

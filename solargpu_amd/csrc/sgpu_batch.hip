@@ -105,8 +105,8 @@ void add_tile(sgpu_batch* b, const float* grid, int w, int h, int64_t n_total, i
               const float* wall_z, float maxz, const float* slopes, const float* aspects, const float* az_alt,
               const float* dni, const float* dhi, int32_t* index) {
     need(grid && pp && slopes && aspects && az_alt && dni && dhi, SGPU_ERR_ARG, "null argument");
-    need(w > 0 && h > 0 && grid_off32((unsigned)w, (unsigned)h), SGPU_ERR_ARG,
-         "grid size out of range (width x (height + 2) must not exceed 2^30 cells)");
+    need(w > 0 && h > 0 && grid_supported((unsigned)w, (unsigned)h), SGPU_ERR_ARG,
+         "grid size out of range (width x (height + 2) must not exceed 2^30 cells, height 2^22 rows)");
     need(n_top == (int64_t)w * h, SGPU_ERR_ARG, "num_top_voxels must equal grid_width * grid_height");
     need(n_total >= n_top && n_total < (int64_t)1 << 31, SGPU_ERR_ARG, "num_total_voxels out of range");
     need(n_total == n_top || wall_z != nullptr, SGPU_ERR_ARG, "wall_z required when there are wall voxels");

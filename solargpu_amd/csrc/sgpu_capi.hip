@@ -109,6 +109,7 @@ HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
     a.svf = c->svf;
     a.hourly_top = c->hourly_top;
     a.hourly_top_cap = (int64_t)c->hourly_top_elems;
+    if (c->hourly_limit > 0) a.hourly_top_cap = std::min<int64_t>(a.hourly_top_cap, c->hourly_limit);
     a.sum = c->sum;
     a.comp = c->comp;
     a.waves = c->tune.waves_per_group;
@@ -984,8 +985,8 @@ int sgpu_set_scene(sgpu_ctx* c, const float* grid, int w, int h, int64_t n_total
                    const float* wall_z, float maxz, const float* slopes, const float* aspects) {
     return guarded([&] {
         need(c && grid && pp && slopes && aspects, SGPU_ERR_ARG, "null argument");
-        need(w > 0 && h > 0 && grid_off32((unsigned)w, (unsigned)h), SGPU_ERR_ARG,
-             "grid size out of range (width x (height + 2) must not exceed 2^30 cells)");
+        need(w > 0 && h > 0 && grid_supported((unsigned)w, (unsigned)h), SGPU_ERR_ARG,
+             "grid size out of range (width x (height + 2) must not exceed 2^30 cells, height 2^22 rows)");
         need(n_top == (int64_t)w * h, SGPU_ERR_ARG, "num_top_voxels must equal grid_width * grid_height");
         need(n_total >= n_top && n_total < (int64_t)1 << 31, SGPU_ERR_ARG, "num_total_voxels out of range");
         need(n_total == n_top || wall_z != nullptr, SGPU_ERR_ARG, "wall_z required when there are wall voxels");
@@ -1019,8 +1020,8 @@ int sgpu_set_scene(sgpu_ctx* c, const float* grid, int w, int h, int64_t n_total
 int sgpu_set_scene_heights(sgpu_ctx* c, const float* heights, int w, int h, double scale_x, double scale_y) {
     return guarded([&] {
         need(c && heights, SGPU_ERR_ARG, "null argument");
-        need(w > 0 && h > 0 && grid_off32((unsigned)w, (unsigned)h), SGPU_ERR_ARG,
-             "grid size out of range (width x (height + 2) must not exceed 2^30 cells)");
+        need(w > 0 && h > 0 && grid_supported((unsigned)w, (unsigned)h), SGPU_ERR_ARG,
+             "grid size out of range (width x (height + 2) must not exceed 2^30 cells, height 2^22 rows)");
         need(scale_x > 0.0 && scale_y > 0.0, SGPU_ERR_ARG, "pixel scales must be positive");
         c->set_device();
         c->drop_products();
@@ -1399,6 +1400,13 @@ int sgpu_set_kernel_timing(sgpu_ctx* c, int enable) {
     return guarded([&] {
         need(c != nullptr, SGPU_ERR_ARG, "null context");
         c->kernel_timing = enable != 0;
+    });
+}
+
+int sgpu_set_hourly_limit(sgpu_ctx* c, int64_t floats) {
+    return guarded([&] {
+        need(c != nullptr && floats >= 0, SGPU_ERR_ARG, "bad argument");
+        c->hourly_limit = floats;
     });
 }
 

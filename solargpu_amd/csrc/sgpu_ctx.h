@@ -154,6 +154,7 @@ struct sgpu_ctx {
     float* comp = nullptr;
     float* hourly_top = nullptr;
     size_t hourly_top_elems = 0;
+    int64_t hourly_limit = 0;          // sgpu_set_hourly_limit: floats the passes may use (0 = all)
     sgpu::HourConst* hours_d = nullptr;
     size_t hours_d_cap = 0;
     bool irr_ok = false;

@@ -94,6 +94,13 @@ void synth_weather(uint64_t seed, const float* az_alt, float* dni, float* dhi);
 __device__ __host__ inline bool grid_off32(unsigned long long w, unsigned long long h) {
     return w * (h + 2ull) <= (1ull << 30);
 }
+/* Rows of a supported grid.  The reflection pass splits a cell id into (x, y) with a float
+   reciprocal of the width and one fix-up step (cell_xy, k_hourly.hip): the estimate's relative
+   error is below 3 * 2^-24, so it is off by at most one row while the row index is below 2^22. */
+constexpr unsigned kMaxGridRows = 1u << 22;
+__device__ __host__ inline bool grid_supported(unsigned long long w, unsigned long long h) {
+    return grid_off32(w, h) && h <= kMaxGridRows;
+}
 
 struct TraceArgs {
     const float* grid;

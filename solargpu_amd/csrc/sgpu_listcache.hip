@@ -9,7 +9,10 @@
  * (a position-weighted 64-bit sum of the words of the heightmap, parents and wall heights; of
  * the dense sun-bin map; the scene's also takes the height limit), so lists are only loaded
  * into the scene and sky they were made for.
- * Sections stream through a pinned host buffer in 64 MB pieces. */
+ * Sections stream through a pinned host buffer in 64 MB pieces.  A load checks the file's
+ * length against its header and every loaded section on the device (offsets monotone and
+ * ending at the section size, ids within the grid, ba-RLE streams walking block by block to
+ * their ends) before the lists are used: a damaged file is refused with SGPU_ERR_FORMAT. */
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -85,6 +88,83 @@ uint64_t sky_checksum(const sgpu_ctx* c) {
         sum += (uint64_t)(uint32_t)c->sky.full_to_dense[i] * (2ull * i + 1ull);
     return sum;
 }
+
+/* Checks of loaded sections, on the device so that 8192^2-sized sections need no host copy.
+ * Each records the first bad index in *bad (left at ~0 when the section is sound). */
+
+/* offsets[0..n]: off[0] == 0, 0 <= off[i+1] - off[i] <= max_len, off[n] == total. */
+__global__ __launch_bounds__(256) void k_check_offsets(const int64_t* __restrict__ off, int64_t n, int64_t total,
+                                                       int64_t max_len, unsigned long long* bad) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i <= n; i += (int64_t)gridDim.x * 256) {
+        bool b;
+        if (i == 0) {
+            b = off[0] != 0;
+        } else {
+            const int64_t len = off[i] - off[i - 1];
+            b = len < 0 || len > max_len || (i == n && off[n] != total);
+        }
+        if (b) atomicMin(bad, (unsigned long long)i);
+    }
+}
+
+/* CSR ids are 1-based top-cell ids: 1..cells. */
+__global__ __launch_bounds__(256) void k_check_ids(const int32_t* __restrict__ ids, int64_t nnz, int32_t cells,
+                                                   unsigned long long* bad) {
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * 256) {
+        const int32_t id = ids[k];
+        if (id <= 0 || id > cells) atomicMin(bad, (unsigned long long)k);
+    }
+}
+
+/* Every ba-RLE stream (offsets already checked) walks block by block to exactly its end:
+ * 3 header bytes of width codes (0..4, 7 = pad), a length byte per run, then the payload --
+ * the structure the reflection pass's decoder relies on to stay inside a stream.  Every byte
+ * read is inside [off[v], off[v+1]). */
+__global__ __launch_bounds__(256) void k_check_barle(const int64_t* __restrict__ off,
+                                                     const uint8_t* __restrict__ bytes, int64_t n,
+                                                     unsigned long long* bad) {
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
+        const int64_t b = off[v + 1];
+        int64_t i = off[v];
+        bool ok = true;
+        while (ok && i < b) {
+            if (i + 3 > b) { ok = false; break; }
+            const unsigned bits = ((unsigned)bytes[i] << 16) | ((unsigned)bytes[i + 1] << 8) | bytes[i + 2];
+            i += 3;
+            unsigned w[8];
+            int nc = 0;
+            for (int j = 0; j < 8; ++j) {
+                const unsigned cd = (bits >> (21 - 3 * j)) & 7u;
+                if (cd == 7u) break;
+                if (cd > 4u) ok = false;
+                w[nc++] = cd;
+            }
+            if (!ok || i + nc > b) { ok = false; break; }
+            int64_t payload = 0;
+            for (int j = 0; j < nc; ++j) {
+                const int64_t len = bytes[i + j];
+                payload += (w[j] == 0) ? (len + 1) / 2 : len * (int64_t)w[j];
+            }
+            i += nc + payload;
+            if (i > b) ok = false;
+        }
+        if (!ok) atomicMin(bad, (unsigned long long)v);
+    }
+}
+
+/* Runs one check kernel and fails with SGPU_ERR_FORMAT naming the section and index. */
+template <typename Launch>
+void check_section(sgpu_ctx* c, unsigned long long* flag, const char* what, Launch&& launch) {
+    SGPU_HIP(hipMemsetAsync(flag, 0xFF, sizeof(unsigned long long), c->stream));
+    launch();
+    SGPU_HIP(hipGetLastError());
+    unsigned long long first = 0;
+    SGPU_HIP(hipMemcpyAsync(&first, flag, sizeof first, hipMemcpyDeviceToHost, c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+    if (first != ~0ull) fail(SGPU_ERR_FORMAT, "list file: %s damaged (first bad entry %llu)", what, first);
+}
+
+unsigned check_blocks(int64_t items) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + 255) / 256, 8192)); }
 
 struct File {
     FILE* f = nullptr;
@@ -190,33 +270,72 @@ int sgpu_load_lists(sgpu_ctx* c, const char* path) {
         need(hd.sky_sum == sky_checksum(c) && hd.svf_bytes == c->sky.svf_bytes, SGPU_ERR_ARG,
              "list file is for another sky (sun bins)");
         need((hd.formats & ~(unsigned)(SGPU_FMT_CSR | SGPU_FMT_BARLE)) == 0 && hd.formats != 0 && hd.nnz >= 0 &&
-                 hd.barle_bytes >= 0,
+                 hd.barle_bytes >= 0 && hd.nnz <= (int64_t)kDirs * c->n_total &&
+                 hd.barle_bytes <= ((int64_t)1 << 20) * c->n_total,
              SGPU_ERR_FORMAT, "list file: bad header");
+        if ((hd.formats & SGPU_FMT_CSR) == 0) need(hd.nnz == 0, SGPU_ERR_FORMAT, "list file: bad header");
+        if ((hd.formats & SGPU_FMT_BARLE) == 0) need(hd.barle_bytes == 0, SGPU_ERR_FORMAT, "list file: bad header");
+        const size_t offs = (size_t)(c->n_total + 1) * sizeof(int64_t);
+        {   /* the sections the header announces must be exactly the file's length */
+            uint64_t want = sizeof hd + (uint64_t)c->sky.svf_bytes * c->n_total;
+            if (hd.formats & SGPU_FMT_CSR) want += offs + (uint64_t)hd.nnz * sizeof(int32_t);
+            if (hd.formats & SGPU_FMT_BARLE) want += offs + (uint64_t)hd.barle_bytes;
+            need(std::fseek(in.f, 0, SEEK_END) == 0, SGPU_ERR_IO, "list file: cannot seek");
+            const long long size = std::ftell(in.f);
+            if (size < 0 || (uint64_t)size != want)
+                fail(SGPU_ERR_FORMAT, "list file: %lld bytes, its header announces %llu", size,
+                     (unsigned long long)want);
+            need(std::fseek(in.f, (long)sizeof hd, SEEK_SET) == 0, SGPU_ERR_IO, "list file: cannot seek");
+        }
         c->drop_products();
         Pinned buf;
         SGPU_HIP(hipHostMalloc(&buf.p, kPiece, hipHostMallocDefault));
-        const size_t offs = (size_t)(c->n_total + 1) * sizeof(int64_t);
+        unsigned long long* flag = nullptr;
         try {
+            flag = dalloc<unsigned long long>(1);
             c->svf = dalloc<uint8_t>((size_t)std::max(c->sky.svf_bytes, 1) * c->n_total);
             read_section(c, in.f, c->svf, (size_t)c->sky.svf_bytes * c->n_total, buf.p);
+            const int64_t N = c->n_total;
             if (hd.formats & SGPU_FMT_CSR) {
-                c->csr_off = dalloc<int64_t>(c->n_total + 1);
+                c->csr_off = dalloc<int64_t>(N + 1);
                 c->csr_ids = dalloc<int32_t>((size_t)std::max<int64_t>(hd.nnz, 1));
                 read_section(c, in.f, c->csr_off, offs, buf.p);
                 read_section(c, in.f, c->csr_ids, (size_t)hd.nnz * sizeof(int32_t), buf.p);
+                const int64_t* o = c->csr_off;
+                const int32_t* ids = c->csr_ids;
+                check_section(c, flag, "CSR offsets", [&] {
+                    hipLaunchKernelGGL(k_check_offsets, dim3(check_blocks(N + 1)), dim3(256), 0, c->stream, o, N,
+                                       hd.nnz, (int64_t)kDirs, flag);
+                });
+                check_section(c, flag, "CSR ids", [&] {
+                    hipLaunchKernelGGL(k_check_ids, dim3(check_blocks(hd.nnz)), dim3(256), 0, c->stream, ids,
+                                       hd.nnz, (int32_t)c->n_top, flag);
+                });
                 c->nnz = hd.nnz;
             }
             if (hd.formats & SGPU_FMT_BARLE) {
-                c->br_off = dalloc<int64_t>(c->n_total + 1);
+                c->br_off = dalloc<int64_t>(N + 1);
                 c->br_bytes = dalloc<uint8_t>((size_t)std::max<int64_t>(hd.barle_bytes, 1));
                 read_section(c, in.f, c->br_off, offs, buf.p);
                 read_section(c, in.f, c->br_bytes, (size_t)hd.barle_bytes, buf.p);
+                const int64_t* o = c->br_off;
+                const uint8_t* bytes = c->br_bytes;
+                check_section(c, flag, "ba-RLE offsets", [&] {
+                    hipLaunchKernelGGL(k_check_offsets, dim3(check_blocks(N + 1)), dim3(256), 0, c->stream, o, N,
+                                       hd.barle_bytes, (int64_t)1 << 20, flag);
+                });
+                check_section(c, flag, "ba-RLE streams", [&] {
+                    hipLaunchKernelGGL(k_check_barle, dim3(check_blocks(N)), dim3(256), 0, c->stream, o, bytes, N,
+                                       flag);
+                });
                 c->br_size = hd.barle_bytes;
             }
         } catch (...) {
+            dfree(flag);
             c->drop_products();
             throw;
         }
+        dfree(flag);
         c->svf_ok = true;
         c->info.nnz = hd.nnz;
         c->info.barle_bytes = hd.barle_bytes;

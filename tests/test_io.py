@@ -128,3 +128,53 @@ def test_cli_matches_drop_in_entry_point(tmp_path):
     assert (n_total, n_top) == (sc.num_total, W * H)
     rec = np.frombuffer(raw[8:], np.float32).reshape(-1, 4)
     np.testing.assert_allclose(rec[:, 3], want, rtol=1e-5, atol=1e-3)
+
+
+def _tiff_bytes(w, h, strip_bytes, scale_type=12, extra=()):
+    """A little-endian, uncompressed, single-strip float32 TIFF written field by field, with a
+    ModelPixelScale tag of 1 m (stored as `scale_type`).  `strip_bytes` is what the
+    StripByteCounts tag claims; `extra` adds (tag, type, count, value-or-offset) entries verbatim."""
+    import struct
+    pixels = np.arange(w * h, dtype="<f4").tobytes()
+    entries = [(256, 4, 1, w), (257, 4, 1, h), (258, 3, 1, 32), (259, 3, 1, 1), (262, 3, 1, 1),
+               (273, 4, 1, None), (277, 3, 1, 1), (278, 4, 1, h), (279, 4, 1, strip_bytes), (339, 3, 1, 3),
+               (33550, scale_type, 3, None)]
+    entries = sorted(entries + list(extra))
+    ifd_at = 8
+    scale_at = ifd_at + 2 + 12 * len(entries) + 4
+    data_at = scale_at + 24
+    out = bytearray(b"II" + struct.pack("<HI", 42, ifd_at))
+    out += struct.pack("<H", len(entries))
+    for tag, typ, count, val in entries:
+        if tag == 273:
+            val = data_at
+        elif tag == 33550:
+            val = scale_at
+        out += struct.pack("<HHII", tag, typ, count, val)
+    out += struct.pack("<I", 0)
+    out += struct.pack("<3d", 1.0, 1.0, 0.0)
+    out += pixels[:strip_bytes]
+    return bytes(out)
+
+
+def test_damaged_tiffs_are_refused_before_reading_past_the_file(tmp_path):
+    good = tmp_path / "good.tif"
+    good.write_bytes(_tiff_bytes(64, 64, 64 * 64 * 4))
+    np.testing.assert_array_equal(sg.GeoTiff(str(good)).read().ravel(), np.arange(64 * 64, dtype=np.float32))
+    short = tmp_path / "short.tif"          # StripByteCounts = 16 for a 16 KiB strip
+    short.write_bytes(_tiff_bytes(64, 64, 16))
+    with pytest.raises(sg.SolarGPUError) as e:
+        sg.GeoTiff(str(short)).read()
+    assert e.value.code == 6
+    # GeoAsciiParams with an unknown type (13) and a count that reaches far past the file
+    odd = tmp_path / "odd.tif"
+    odd.write_bytes(_tiff_bytes(8, 8, 8 * 8 * 4, extra=[(34737, 13, 1 << 20, 0x10)]))
+    with pytest.raises(sg.SolarGPUError) as e:
+        sg.GeoTiff(str(odd)).read()
+    assert e.value.code == 6
+    # ModelPixelScale stored as LONG instead of DOUBLE
+    bad_type = tmp_path / "bad_type.tif"
+    bad_type.write_bytes(_tiff_bytes(8, 8, 8 * 8 * 4, scale_type=4))
+    with pytest.raises(sg.SolarGPUError) as e:
+        sg.GeoTiff(str(bad_type)).read()
+    assert e.value.code == 6

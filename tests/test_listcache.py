@@ -106,3 +106,52 @@ def test_damaged_files_are_refused(tmp_path):
             with pytest.raises(sg.SolarGPUError):
                 b.load_lists(str(tmp_path / name))
         b.load_lists(str(path))   # the intact file still loads
+
+
+def test_damaged_payload_sections_are_refused(tmp_path):
+    """Damage inside the sections (not just the header or the length) is found on load, before
+    the lists are used: an offset that breaks monotonicity, a CSR id outside the grid, a ba-RLE
+    offset moved by one byte (two streams no longer end on a block boundary) and a ba-RLE width
+    code of 5 are each refused with SGPU_ERR_FORMAT, and the context is left without lists."""
+    sc = synth_scene(36, 48, 40)
+    path = tmp_path / "lists.sgl"
+    with _ctx(sc) as a:
+        a.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
+        a.save_lists(str(path))
+        off, ids = a.csr()
+        boff, data = a.barle()
+    raw = bytearray(path.read_bytes())
+    n = sc.num_total
+    svf_bytes = sky()[3]["svf_bytes"]
+    csr_off_at = 80 + svf_bytes * n
+    ids_at = csr_off_at + 8 * (n + 1)
+    br_off_at = ids_at + 4 * ids.size
+    br_at = br_off_at + 8 * (n + 1)
+    assert br_at + data.size == len(raw)
+    v = int(np.flatnonzero(np.diff(boff) > 8)[len(boff) // 3])   # a voxel in the middle with a stream
+
+    def damaged(name, at, value, fmt):
+        buf = bytearray(raw)
+        buf[at:at + len(value)] = value
+        p = tmp_path / name
+        p.write_bytes(bytes(buf))
+        return str(p)
+
+    cases = [
+        damaged("csr_off.sgl", csr_off_at + 8 * (v + 1), np.int64(off[v] - 1).tobytes(), "<q"),
+        damaged("csr_id0.sgl", ids_at + 4 * 17, np.int32(0).tobytes(), "<i"),
+        damaged("csr_idhi.sgl", ids_at + 4 * 23, np.int32(sc.num_top + 1).tobytes(), "<i"),
+        damaged("br_off.sgl", br_off_at + 8 * v, np.int64(boff[v] + 1).tobytes(), "<q"),
+        damaged("br_code.sgl", br_at + int(boff[v]), bytes([(data[boff[v]] & 0x1F) | (5 << 5)]), "B"),
+    ]
+    with _ctx(sc) as b:
+        for p in cases:
+            with pytest.raises(sg.SolarGPUError) as e:
+                b.load_lists(p)
+            assert e.value.code == 6 and "damaged" in str(e.value), (p, str(e.value))
+            with pytest.raises(sg.SolarGPUError) as e:   # nothing half-loaded is left behind
+                b.hour_loop(sg.FMT_BARLE, 64)
+            assert e.value.code == 4
+        b.load_lists(str(path))
+        o2, i2 = b.csr()
+        np.testing.assert_array_equal(i2, ids)

@@ -223,6 +223,10 @@ def test_fp64_linearity_matches_per_hour_fp64_sum():
 @pytest.mark.parametrize("mode", [0, 1])
 def test_fp32_oracle_within_1e4_of_fp64(mode):
     sc = synth_scene(24, 14, 12)
+    # walls (slope pi/2) facing every quadrant carry the FP64 tilt terms into the comparison
+    walls = sc.aspects[sc.num_top:]
+    assert sc.num_total > 2 * sc.num_top and np.all(sc.slopes[sc.num_top:] == np.float32(np.pi / 2))
+    assert np.unique(np.floor(np.mod(walls, 2 * np.pi) / (np.pi / 2))).size == 4
     az_alt, dni, dhi, bins = sky()
     ref = orc.viewshed(sc, bins)
     boff, data = orc.barle_encode_all(ref["off"], ref["ids"])

@@ -67,3 +67,32 @@ def test_receiver_assignment_is_bit_identical():
             for x in out[1:]:
                 np.testing.assert_array_equal(x, out[0])
             out.clear()
+
+
+def test_undersized_hourly_buffer_is_refused_before_any_launch(tmp_path, monkeypatch):
+    """A pass of K hours writes and gathers n_top x K floats of hourly rows.  The autotuner once
+    ran K = 128 against a buffer sized for K = 64, and the reflection pass read past the buffer
+    (a GPU memory fault); every pass now checks the capacity before it launches.  With the
+    usable buffer capped at the K = 64 size, a K = 128 loop must be refused with SGPU_ERR_STATE
+    and leave the context usable; the tuner's own sequence (a K = 64 loop, then trials of up to
+    256 hours per pass) then runs, with results within 1e-4 of the FP64 oracle."""
+    monkeypatch.setenv("SGPU_AUTOTUNE_CACHE", str(tmp_path / "tune.txt"))
+    sc = synth_scene(11, 48, 40)
+    ref = oracle_lists(11, 48, 40)
+    az_alt, dni, dhi, bins = sky()
+    exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
+    with _ctx(sc) as ctx:
+        ctx.viewshed(sg.FMT_BARLE)
+        ctx.hour_loop(sg.FMT_BARLE, 64)
+        ctx.set_hourly_limit(sc.num_top * 64)
+        for K in (128, 256):
+            with pytest.raises(sg.SolarGPUError) as e:
+                ctx.hour_loop(sg.FMT_BARLE, K)
+            assert e.value.code == 4 and "hourly irradiance buffer" in str(e.value)
+        ctx.hour_loop(sg.FMT_BARLE, 64)          # fits the cap: runs
+        assert (np.abs(ctx.irradiance() - exact) / np.maximum(exact, 1.0)).max() <= 1e-4
+        ctx.set_hourly_limit(0)
+        t = ctx.autotune(sg.FMT_BARLE)
+        assert t["hours_per_pass"] in (64, 128, 256)
+        ctx.hour_loop(sg.FMT_BARLE, 0)
+        assert (np.abs(ctx.irradiance() - exact) / np.maximum(exact, 1.0)).max() <= 1e-4
