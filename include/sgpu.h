@@ -181,6 +181,18 @@ int sgpu_barle_encode(sgpu_ctx* ctx);
    hour_stride > 1 computes every hour_stride-th sunlit hour only (bounded samples).
    Annual sums are FP32 Kahan-compensated. */
 int sgpu_hour_loop(sgpu_ctx* ctx, unsigned format, int hours_per_pass, int hour_stride);
+/* Allocation-free hour loop.  No counterpart in the reference: it is for callers that cannot
+   allocate device memory while they run (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  The full sunlit-hour loop of sgpu_hour_loop (same kernels,
+   same results bit for bit) over the context's device-resident lists of `format`; the annual
+   sums go to irradiance_out, a device buffer of num_total floats owned by the caller.  The
+   per-pass scratch (num_top x hours_per_pass floats of hourly rows + num_total floats of Kahan
+   compensation, each rounded up to 256 bytes) lives in a static device array of 128 MiB per
+   GPU, held from the first context on that GPU to the end of the process; a scene that needs
+   more is refused with SGPU_ERR_ARG.  hours_per_pass 0 = the tuned value, halved until it fits.
+   Calls no hipMalloc/hipFree; enqueued on the context stream (sgpu_sync waits).  Calls that
+   overlap on one GPU share its array and race. */
+int sgpu_hour_loop_noalloc(sgpu_ctx* ctx, unsigned format, int hours_per_pass, float* irradiance_out);
 /* Record the hour loop as one hipGraph and replay it (new). */
 int sgpu_hour_loop_graph(sgpu_ctx* ctx, unsigned format, int hours_per_pass, int replays);
 int sgpu_get_irradiance(sgpu_ctx* ctx, float* out /* num_total */);

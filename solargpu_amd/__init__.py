@@ -406,6 +406,12 @@ class Context:
     def hour_loop(self, fmt: int, hours_per_pass: int = 0, hour_stride: int = 1) -> None:
         _check(_lib.sgpu_hour_loop(self._h, ctypes.c_uint(fmt), hours_per_pass, hour_stride))
 
+    def hour_loop_noalloc(self, fmt: int, hours_per_pass: int, out_device_ptr: int) -> None:
+        """Allocation-free hour loop into a caller-owned device buffer of num_total floats
+        (enqueued on the context stream; call sync() before reading the buffer)."""
+        _check(_lib.sgpu_hour_loop_noalloc(self._h, ctypes.c_uint(fmt), hours_per_pass,
+                                           ctypes.c_void_p(out_device_ptr)))
+
     def hour_loop_graph(self, fmt: int, hours_per_pass: int = 0, replays: int = 1) -> None:
         _check(_lib.sgpu_hour_loop_graph(self._h, ctypes.c_uint(fmt), hours_per_pass, replays))
 

@@ -942,6 +942,7 @@ int sgpu_create(int device, sgpu_ctx** out) {
         c->set_device();
         SGPU_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->dirs = dalloc<DirConst>(kDirs);
+        c->noalloc_scratch = noalloc_scratch_setup(device);
         *out = c.release();
     });
 }
@@ -1107,7 +1108,16 @@ int sgpu_set_sky(sgpu_ctx* c, const float* az_alt, const float* dni, const float
         std::vector<DirConst> d(kDirs);
         directions(c->sky, d.data());
         SGPU_HIP(hipMemcpyAsync(c->dirs, d.data(), kDirs * sizeof(DirConst), hipMemcpyHostToDevice, c->stream));
+        /* every sunlit hour, for the allocation-free hour loop (which may not upload) */
         SGPU_HIP(hipStreamSynchronize(c->stream));
+        dfree(c->hours_all_d);
+        c->hours_all_n = 0;
+        c->hours_all_d = dalloc<HourConst>(std::max<size_t>(c->hours.size(), 1));
+        if (!c->hours.empty())
+            SGPU_HIP(hipMemcpyAsync(c->hours_all_d, c->hours.data(), c->hours.size() * sizeof(HourConst),
+                                    hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->hours_all_n = c->hours.size();
         c->drop_products();
         c->sky_set = true;
     });

@@ -157,6 +157,9 @@ struct sgpu_ctx {
     int64_t hourly_limit = 0;          // sgpu_set_hourly_limit: floats the passes may use (0 = all)
     sgpu::HourConst* hours_d = nullptr;
     size_t hours_d_cap = 0;
+    sgpu::HourConst* hours_all_d = nullptr;   // every sunlit hour of the sky, uploaded by sgpu_set_sky
+    size_t hours_all_n = 0;
+    void* noalloc_scratch = nullptr;          // this device's allocation-free scratch (sgpu_noalloc.hip)
     bool irr_ok = false;
 
     /* Viewshed workspace for chunks of up to ws_chunk voxels (raw first-hit table, per-voxel
@@ -223,6 +226,7 @@ struct sgpu_ctx {
         drop_scene();
         sgpu::dfree(dirs);
         sgpu::dfree(hours_d);
+        sgpu::dfree(hours_all_d);
         if (stream) (void)hipStreamDestroy(stream);
         if (copy_stream) (void)hipStreamDestroy(copy_stream);
     }
@@ -236,6 +240,9 @@ HourArgs stage2_args(sgpu_ctx* c, unsigned fmt);
 void ensure_accumulators(sgpu_ctx* c, int K);
 void ensure_hours_d(sgpu_ctx* c, const std::vector<HourConst>& hs);
 void zero_floats(float* p, int64_t n, hipStream_t s);
+/* The current device's address of the allocation-free hour loop's scratch array, looked up
+   once per device (sgpu_noalloc.hip); called when a context is created. */
+void* noalloc_scratch_setup(int device);
 /* Viewshed of a voxel subset (ids on the device) into compact CSR lists or, with barle,
    into ba-RLE streams only (encoded chunk by chunk), indexed by subset position. */
 struct SubsetLists {
