@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -49,82 +50,129 @@ def reflect_pass_bytes(n_total, n_top, stream_bytes, nh):
             + (4 * nh + 4) * n_top)
 
 
-def read_traffic():
-    """HBM bytes/launch of the reflection kernel from the committed rocprofv3 PMC summary."""
-    path = os.path.join(PROFILES, "reflect_barle_c2_traffic.txt")
+def read_profile(name, key):
+    """A figure (key = value line) from a committed rocprofv3 summary under profiles/."""
+    path = os.path.join(PROFILES, name)
     if not os.path.exists(path):
         return None
-    vals = {}
     with open(path) as f:
         for line in f:
             if "=" in line and not line.lstrip().startswith("#"):
                 k, v = line.split("=", 1)
-                vals[k.strip()] = v.strip()
+                if k.strip() == key:
+                    try:
+                        return float(v.strip())
+                    except ValueError:
+                        return None
+    return None
+
+
+def read_traffic():
+    """HBM bytes/launch of the reflection kernel from the committed rocprofv3 PMC summary."""
+    return read_profile("reflect_barle_c2_traffic.txt", "traffic_bytes_per_launch")
+
+
+# VALU issue of one wave64 instruction per CU per cycle: 256 CUs x 2.4 GHz (4 SIMD16 x 4 cycles)
+VALU_NOMINAL_PEAK = 256 * 2.4e9
+
+
+def host_cores():
+    """Host cores this run may use: what `nproc` reports (the CPU affinity, bounded by
+    OMP_NUM_THREADS where the host sets it -- a shared machine's CPU share), and the machine's
+    logical CPU count.  Read before the GPU is touched."""
     try:
-        return float(vals["traffic_bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
+        n = int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout.strip())
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        n = len(os.sched_getaffinity(0))
+    return max(n, 1), os.cpu_count() or n
 
 
-def cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, threads, budget_s):
-    """Oracle (scalar C++ port, `threads` threads) on a bounded sample of the same scene."""
+CPU_SAMPLE_SEED = 20260199
+CPU_SAMPLE_FRACTION = 0.01
+CPU_SAMPLE_BLOCK = 512
+CPU_HOUR_STRIDE = 24
+
+
+def cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, threads, host_cpus):
+    """The oracle (a multithreaded C++ port of the reference method) on a fixed random 1% of the
+    scene's voxels (random 512-voxel blocks): stage 1 (the 2592-direction DDA + first-hit lists) of the sample, then the FP32
+    hour loop of the sample (its own lists) on every 24th sunlit hour, with the solar term of every
+    top cell per sampled hour (the neighbours' values).  Rates are extrapolated linearly to all
+    voxels and hours."""
     bins = orc.sky_bins(az_alt)
-    svf = ctx.svf()
-    N, ntop = sc.num_total, sc.num_top
+    N = sc.num_total
     hours_total = orc.sunlit_hours(bins["nudged"]).size
-    # stage 2 sample: 50000 top + 50000 wall voxels, every 40th sunlit hour
-    stride = 40
-    rates = []
-    t_top = t_rng = 0.0
-    nvox = 0
-    for v0 in (ntop // 2, ntop + (N - ntop) // 2):
-        cnt = min(50000, N - v0)
-        off, ids = ctx.csr_range(v0, cnt)
-        _, nh, tt, tr = orc.annual_fp32_range(sc, bins, dni, dhi, svf, v0, off, ids, stride=stride,
-                                              nthreads=threads)
-        t_top += tt
-        t_rng += tr
-        nvox += cnt
-    # extrapolate: per sampled hour, all-top solar once + per-voxel cost for all N voxels
-    per_hour = t_top / (2 * nh) + (t_rng / nh) * (N / nvox)
-    vh_rate = N / per_hour
-    # stage 1 sample: DDA rays of 4 x 40000 voxels (all 2592 directions)
+    # 1% of the voxels in randomly chosen blocks of 512 consecutive voxels: unbiased over the scene,
+    # with the memory locality of a full in-order run (single random voxels would each miss the
+    # CPU caches and understate the port)
+    rng = np.random.default_rng(CPU_SAMPLE_SEED)
+    nblk = (N + CPU_SAMPLE_BLOCK - 1) // CPU_SAMPLE_BLOCK
+    pick = np.sort(rng.choice(nblk, max(int(nblk * CPU_SAMPLE_FRACTION), 1), replace=False))
+    vox = (pick[:, None] * CPU_SAMPLE_BLOCK + np.arange(CPU_SAMPLE_BLOCK)[None, :]).ravel()
+    vox = vox[vox < N].astype(np.int32)
+    n = vox.size
     t0 = time.perf_counter()
-    for v0 in (ntop // 3, 2 * ntop // 3, ntop + (N - ntop) // 3, ntop + 2 * (N - ntop) // 3):
-        orc.trace_raw(sc, bins, v0, 40000, nthreads=threads)
+    sub = orc.viewshed_subset(sc, bins, vox, nthreads=threads)
     t1 = time.perf_counter()
-    rays_rate = 160000 * 2592 / (t1 - t0)
-    sample = (f"oracle FP32 hour loop on 2 x 50000 voxels (top, wall) x {nh} sunlit hours (every {stride}th of "
-              f"{hours_total}), all-top solar per hour, extrapolated linearly to {N} voxels; "
-              f"DDA on 160000 voxels x 2592 rays; {t_top + t_rng + (t1 - t0):.1f} s wall on {threads} threads")
+    rays_rate = n * 2592 / (t1 - t0)
+    svf = ctx.svf()   # the SVF bits of every voxel (the sample's own equal the oracle's; tested)
+    _, nh, t_top, t_vox = orc.annual_fp32_subset(sc, bins, dni, dhi, svf, vox, sub["off"], sub["ids"],
+                                                 stride=CPU_HOUR_STRIDE, nthreads=threads)
+    # per sampled hour: the all-top solar term once + the per-voxel cost scaled to all N voxels
+    per_hour = t_top / nh + (t_vox / nh) * (N / n)
+    vh_rate = N / per_hour
+    sample = (f"oracle on a fixed random {CPU_SAMPLE_FRACTION:.0%} of the voxels ({n} of {N} in {pick.size} "
+              f"random blocks of {CPU_SAMPLE_BLOCK}, seed {CPU_SAMPLE_SEED}): DDA + lists on all 2592 directions ({n * 2592:.3g} rays, {sub['ids'].size} ids), "
+              f"then the FP32 hour loop of the sample on every {CPU_HOUR_STRIDE}th sunlit hour ({nh} of "
+              f"{hours_total}) with the all-top solar term per hour; extrapolated linearly to {N} voxels x "
+              f"{hours_total} hours; {t1 - t0 + t_top + t_vox:.1f} s wall on {threads} threads "
+              f"(nproc; the host has {host_cpus} logical CPUs)")
     return {"value": vh_rate, "unit": "voxel-hours/s", "cores": threads, "kind": "port", "sample": sample,
-            "rays_per_s": rays_rate}
+            "rays_per_s": rays_rate, "host_logical_cpus": host_cpus,
+            "extrapolated_year_s": per_hour * hours_total + N * 2592 / rays_rate}
 
 
 def surrogate_leg(sg, sc, az_alt, dni, dhi, cell, tuning, exact, plain_s, hours):
-    """The surrogate form on the bench scene in a fresh context, with both exact fallbacks."""
+    """The surrogate form on the bench scene in a fresh context, with both exact fallbacks; the
+    network alone (untimed) gives the per-voxel error the gate has to catch."""
     from solargpu_amd import surrogate as sm
-    params, gate, _ = sm.load(WEIGHTS)
-    mlp = sm.MLP(seed=1)
-    mlp.set_params(params)
-    out = {"weights": os.path.relpath(WEIGHTS, ROOT), "plain_gpu_s": plain_s}
+    from solargpu_amd.eval_surrogate import NOWHERE, gate_quality, rel_errors
+    mlp, err_mlp, gate, _ = sm.load_models(WEIGHTS)
+    out = {"weights": os.path.relpath(WEIGHTS, ROOT), "plain_gpu_s": plain_s,
+           "gate": "error network" if err_mlp is not None else "score", "tau": gate.tau,
+           "network": f"{sm.FEATURES} features (padded to {sm.IN_PAD}) -> 4 x {sm.HIDDEN} GELU -> {sm.OUTPUTS}, "
+                      "bf16 weights/activations, fp32 accumulation",
+           "inputs_vs_config": f"{sm.FEATURES} inputs where the surrogate configuration names 48: its listed "
+                               "inputs (32 horizon-sector angles, normal xyz, height, a neighbour count -- here the "
+                               "log count of sun-bin directions that hit --, sky-view factor, latitude, albedo: 40 "
+                               "values) plus 13 more: wall flag, slope, the exact annual direct + diffuse and its "
+                               "diffuse share, the direct + diffuse of the horizon cells summed over four quadrants, "
+                               "a Lambertian estimate over the horizon cells, the cell size, for walls the direct "
+                               "+ diffuse of the ground cell in front and the height above it, and the mean direct "
+                               "+ diffuse of the 8 surrounding cells"}
     with sg.Context(0) as c2:
         c2.set_scene(sc)
         c2.set_sky(az_alt, dni, dhi)
         c2.set_tuning(tuning["dirs_per_group"], tuning["hours_per_pass"], tuning["waves_per_group"],
                       tuning["xcd_band"], tuning["group_blocks"])
         sm.prepare(c2, SITE[1], cell)
+        raw, _, _ = sm.run(c2, mlp, NOWHERE, sm.FALLBACK_ANNUAL)
+        raw_err = rel_errors(raw, exact)
+        out["network_alone"] = {"p95": float(np.percentile(raw_err, 95)), "max": float(raw_err.max()),
+                                "share_above_2pct": float((raw_err > 0.02).mean())}
         for name, k in (("annual_fallback", sm.FALLBACK_ANNUAL), ("hourly_fallback", 0)):
-            tot, fb, st = sm.run(c2, mlp, gate, k)
-            rel = np.abs(tot.astype(np.float64) - exact) / np.maximum(np.abs(exact.astype(np.float64)), 1.0)
+            tot, fb, st = sm.run(c2, mlp, gate, k, err_mlp)
+            rel = rel_errors(tot, exact)
             gs = st["t_total_ms"] / 1e3
             out[name] = {"gpu_s": gs, "gpu_seconds_ratio": plain_s / gs,
                          "voxel_hours_per_s": sc.num_total * hours / gs, "fallback_share": float(fb.mean()),
                          "p50": float(np.percentile(rel, 50)), "p95": float(np.percentile(rel, 95)),
-                         "max": float(rel.max())}
+                         "max": float(rel.max()), **gate_quality(fb, raw_err, rel)}
             log(f"surrogate ({name}): {gs * 1e3:.0f} ms, p95 {out[name]['p95']:.4f}, "
-                f"fallback {out[name]['fallback_share']:.3f}")
+                f"fallback {out[name]['fallback_share']:.3f}, recall {out[name]['recall_above_2pct']:.3f}")
     mlp.close()
+    if err_mlp is not None:
+        err_mlp.close()
     return out
 
 
@@ -146,10 +194,10 @@ def main():
                     help="G,K,waves,band,group: use this launch shape instead of autotuning (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer drop-in timing")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
+    cores, host_cpus = host_cores()   # before anything touches the GPU
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, "
               "'How the path would shard'); use --gpus 1", file=sys.stderr)
@@ -237,6 +285,23 @@ def main():
         extra["surrogate"] = surrogate_leg(sg, sc, az_alt, dni, dhi, cell, ctx.tuning(), exact,
                                            (t_vs + t_hr) / args.steps, H)
     copy_gbps = ctx.copy_bandwidth(1 << 30, 20)
+    valu_peak = ctx.valu_rate()
+    # DDA roofline: VALU issue.  Instructions per trace launch from the committed PMC summary of the
+    # same workload; the launch time live (HIP events around each chunk's trace launch).
+    chunks = recs[-1][0]["viewshed_chunks"]
+    avg_trace = t_tr / (chunks * args.steps)
+    valu_per_launch = read_profile("trace_pool_c2_valu.txt", "valu_insts_per_launch")
+    dda_roof = {"bound": "valu", "unit": "wave64 VALU instructions/s", "peak": valu_peak,
+                "peak_nominal": VALU_NOMINAL_PEAK, "kernel": f"k_trace_pool<G={ctx.tuning()['dirs_per_group']}>",
+                "avg_launch_ms": avg_trace * 1e3, "launches_per_viewshed": chunks,
+                "valu_insts_per_launch": valu_per_launch,
+                "achieved": (valu_per_launch / avg_trace) if valu_per_launch else None,
+                "rays_per_s": N * 2592 * args.steps / t_tr,
+                "cell_steps_per_s": dda_steps * args.steps / t_tr,
+                "l2_hit_rate": read_profile("trace_pool_c2_valu.txt", "l2_hit_rate"),
+                "hbm_traffic_gbps_profiled": read_profile("trace_pool_c2_valu.txt", "hbm_traffic_gbps")}
+    dda_roof["frac"] = dda_roof["achieved"] / valu_peak if dda_roof["achieved"] else None
+    dda_roof["frac_nominal"] = dda_roof["achieved"] / VALU_NOMINAL_PEAK if dda_roof["achieved"] else None
     if not args.no_dropin:
         # the drop-in entry point end to end: host arrays in (PCIe), viewshed, hour loop, host out
         t0 = time.perf_counter()
@@ -284,14 +349,15 @@ def main():
                      # launch time, against the copy bandwidth measured in this run
                      "traffic_gbps": (traffic / avg_launch / 1e9) if traffic else None,
                      "traffic_frac_of_copy": (traffic / avg_launch / 1e9 / copy_gbps) if traffic else None},
+        "roofline_dda": dda_roof,
         "measured_copy_gbps": copy_gbps,
+        "measured_valu_wave_insts_per_s": valu_peak,
     }
     res.update(extra)
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as orc
-        threads = min(16, os.cpu_count() or 1)
-        res["cpu_baseline"] = cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, threads, args.cpu_budget)
+        res["cpu_baseline"] = cpu_baseline(sg, orc, ctx, sc, az_alt, dni, dhi, cores, host_cpus)
     ctx.close()
     print(json.dumps(res), flush=True)
 

@@ -236,6 +236,9 @@ int sgpu_get_tuning(sgpu_ctx* ctx, sgpu_tuning* out);
 /* Plain device copy of `bytes` (float4 loads/stores), `reps` launches; *gbps = read+write
    bytes / mean launch time.  Calibrates HBM counters and the measured copy bandwidth (new). */
 int sgpu_copy_bandwidth(sgpu_ctx* ctx, int64_t bytes, int reps, double* gbps);
+/* VALU issue rate of the device in wave64 instructions/s, measured with an FMA-only kernel of
+   known instruction count that fills every SIMD (new; the peak the DDA's VALU roofline uses). */
+int sgpu_valu_rate(sgpu_ctx* ctx, double* wave_insts_per_s);
 
 /* ---- per-voxel surrogate MLP (new; no reference counterpart) ----
    64 inputs (53 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and
@@ -287,9 +290,14 @@ int sgpu_surrogate_labels(sgpu_ctx* ctx, unsigned format, float* reflected);
    hours_per_pass > 0 runs the sunlit-hour loop on them, K hours per pass;
    hours_per_pass == SGPU_FALLBACK_ANNUAL walks each list once with the neighbours' annual
    direct + diffuse totals in place of one hour's values (the reflection is linear in them
-   and its weights do not depend on the hour, so the annual sums are the same). */
+   and its weights do not depend on the hour, so the annual sums are the same).
+   error_mlp (may be NULL): a network of the same shape whose first output is log10 of the
+   predicted relative error of each voxel's surrogate total; voxels whose predicted error
+   exceeds gate->tau go to the exact path (the gate's other fields are then unused).  Without
+   it the gate scores each voxel from the surrogate's own outputs. */
 #define SGPU_FALLBACK_ANNUAL (-1)
-int sgpu_surrogate_run(sgpu_ctx* ctx, sgpu_mlp* mlp, const sgpu_gate* gate, int hours_per_pass, float* out,
+int sgpu_surrogate_run(sgpu_ctx* ctx, sgpu_mlp* mlp, sgpu_mlp* error_mlp, const sgpu_gate* gate, int hours_per_pass,
+                       float* out,
                        int32_t* fallback, sgpu_surrogate_stats* stats);
 
 /* ---- batched tiles (new; no reference counterpart: the reference runs one DSM per call,

@@ -659,23 +659,33 @@ void orc_trace_raw(const float* grid, int w, int h, int n_top, const int* pp, co
     });
 }
 
-int64_t orc_viewshed(const float* grid, int w, int h, int n_total, int n_top, const int* pp, const float* wall_z,
-                     float maxz, const float* cdir, const int* up, const int* dense, int ndir, int hgs,
-                     int nthreads, uint8_t* svf, int64_t* off, int32_t* ids, int64_t cap, int64_t* total_steps) {
+}  // extern "C"
+
+namespace {
+
+/* Viewshed of n voxels (voxel_of(i) for i < n): SVF bits at svf[i + byte * n], CSR lists by
+ * position.  The loop of run_viewshed_out_of_core + the CSR consumer (viewshed_stage.cuh:56-121,
+ * gpu_irradiance_uncompressed.cu:79-95). */
+template <typename VoxelOf>
+int64_t viewshed_impl(VoxelOf voxel_of, int64_t n, const float* grid, int w, int h, int n_top, const int* pp,
+                      const float* wall_z, float maxz, const float* cdir, const int* up, const int* dense, int ndir,
+                      int hgs, int nthreads, uint8_t* svf, int64_t* off, int32_t* ids, int64_t cap,
+                      int64_t* total_steps) {
     std::vector<DirConst> dirs(ndir);
     for (int d = 0; d < ndir; ++d) dirs[d] = make_dir(cdir + 3 * d, up[d]);
-    std::memset(svf, 0, (size_t)hgs * n_total);
+    if (svf) std::memset(svf, 0, (size_t)hgs * n);
     const int64_t kBlock = 512;
-    int64_t nblocks = ((int64_t)n_total + kBlock - 1) / kBlock;
+    int64_t nblocks = (n + kBlock - 1) / kBlock;
     std::vector<std::vector<int32_t>> block_ids(nblocks);
-    std::vector<int32_t> counts(n_total);
+    std::vector<int32_t> counts(n);
     int nt = resolve_threads(nthreads);
     std::vector<int64_t> steps_per_thread(nt, 0);
-    parallel_blocks(n_total, kBlock, nt, [&](int64_t a, int64_t b, int tid) {
+    parallel_blocks(n, kBlock, nt, [&](int64_t a, int64_t b, int tid) {
         IdSet set(8192);
         std::vector<int32_t>& out = block_ids[a / kBlock];
         int64_t st = 0;
-        for (int64_t t = a; t < b; ++t) {
+        for (int64_t i = a; i < b; ++i) {
+            const int64_t t = voxel_of(i);
             int top = pp[t];
             int gx = top % w, gy = top / w;
             float bz = (t < n_top) ? grid[gx + gy * w] : wall_z[t - n_top];
@@ -686,16 +696,16 @@ int64_t orc_viewshed(const float* grid, int w, int h, int n_total, int n_top, co
                 int hit = trace(grid, w, h, maxz, (float)gx, (float)gy, bz, dirs[d], &s);
                 st += s;
                 if (!hit) continue;
-                if (dense[d] >= 0) svf[t + (int64_t)(dense[d] / 8) * n_total] |= (uint8_t)(1u << (dense[d] % 8));
+                if (svf && dense[d] >= 0) svf[i + (int64_t)(dense[d] / 8) * n] |= (uint8_t)(1u << (dense[d] % 8));
                 if (set.insert(hit)) { out.push_back(hit); ++c; }
             }
-            counts[t] = c;
+            counts[i] = c;
         }
         steps_per_thread[tid] += st;
     });
     int64_t nnz = 0;
-    for (int64_t t = 0; t < n_total; ++t) { off[t] = nnz; nnz += counts[t]; }
-    off[n_total] = nnz;
+    for (int64_t i = 0; i < n; ++i) { off[i] = nnz; nnz += counts[i]; }
+    off[n] = nnz;
     if (total_steps) {
         int64_t s = 0;
         for (int64_t x : steps_per_thread) s += x;
@@ -708,6 +718,25 @@ int64_t orc_viewshed(const float* grid, int w, int h, int n_total, int n_top, co
         pos += (int64_t)v.size();
     }
     return nnz;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_viewshed(const float* grid, int w, int h, int n_total, int n_top, const int* pp, const float* wall_z,
+                     float maxz, const float* cdir, const int* up, const int* dense, int ndir, int hgs,
+                     int nthreads, uint8_t* svf, int64_t* off, int32_t* ids, int64_t cap, int64_t* total_steps) {
+    return viewshed_impl([](int64_t i) { return i; }, n_total, grid, w, h, n_top, pp, wall_z, maxz, cdir, up, dense,
+                         ndir, hgs, nthreads, svf, off, ids, cap, total_steps);
+}
+
+int64_t orc_viewshed_subset(const float* grid, int w, int h, int n_top, const int* pp, const float* wall_z,
+                            float maxz, const float* cdir, const int* up, const int* dense, int ndir, int hgs,
+                            const int32_t* voxels, int64_t n, int nthreads, uint8_t* svf, int64_t* off,
+                            int32_t* ids, int64_t cap, int64_t* total_steps) {
+    return viewshed_impl([voxels](int64_t i) { return (int64_t)voxels[i]; }, n, grid, w, h, n_top, pp, wall_z, maxz,
+                         cdir, up, dense, ndir, hgs, nthreads, svf, off, ids, cap, total_steps);
 }
 
 int64_t orc_barle_encode_voxel(const int32_t* ids, int n, uint8_t* out, int64_t cap) {
@@ -856,11 +885,20 @@ int orc_annual_fp32(const float* grid, int w, int h, int n_total, int n_top, con
     return done;
 }
 
-int orc_annual_fp32_range(const float* grid, int w, int h, int n_total, int n_top, const int* pp,
-                          const float* wall_z, const float* slopes, const float* aspects, const float* naa,
-                          const int* sdi, const float* dni, const float* dhi, const uint8_t* svf, int64_t v0,
-                          int64_t count, const int64_t* off, const int32_t* ids, int stride, int nthreads,
-                          float* out, double* t_top_s, double* t_range_s) {
+}  // extern "C"
+
+namespace {
+
+/* The reference's FP32 hour loop (gpu_irradiance_compressed.cu:119-143 with the CSR walk of
+ * reflection_kernels.cu:70-89) for `count` voxels voxel_of(k), with their lists by position:
+ * per sampled hour, the solar term of every top cell (the neighbours' values), then each
+ * voxel's solar + reflection + Kahan accumulate.  Times the two parts separately. */
+template <typename VoxelOf>
+int annual_fp32_sample(VoxelOf voxel_of, int64_t count, const float* grid, int w, int h, int n_total, int n_top,
+                       const int* pp, const float* wall_z, const float* slopes, const float* aspects,
+                       const float* naa, const int* sdi, const float* dni, const float* dhi, const uint8_t* svf,
+                       const int64_t* off, const int32_t* ids, int stride, int nthreads, float* out,
+                       double* t_top_s, double* t_range_s) {
     std::vector<int> hs = sunlit(naa);
     std::vector<float> hourly(n_top), comp(count, 0.0f);
     double tt = 0.0, tr = 0.0;
@@ -876,7 +914,7 @@ int orc_annual_fp32_range(const float* grid, int w, int h, int n_total, int n_to
         const int di = sdi[idx], day = idx / 24;
         parallel_blocks(count, 1024, nthreads, [&](int64_t a, int64_t b, int) {
             for (int64_t k = a; k < b; ++k) {
-                const int64_t t = v0 + k;
+                const int64_t t = voxel_of(k);
                 bool blk = (di >= 0) ? svf_bit(svf, t, di, n_total) : false;
                 float hv = solar_voxel(az, alt, di, dni[idx], dhi[idx], day, slopes[t], aspects[t], blk);
                 Receiver r = receiver(t, w, n_top, pp, grid, wall_z, slopes, aspects);
@@ -897,6 +935,30 @@ int orc_annual_fp32_range(const float* grid, int w, int h, int n_total, int n_to
     if (t_top_s) *t_top_s = tt;
     if (t_range_s) *t_range_s = tr;
     return done;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_annual_fp32_range(const float* grid, int w, int h, int n_total, int n_top, const int* pp,
+                          const float* wall_z, const float* slopes, const float* aspects, const float* naa,
+                          const int* sdi, const float* dni, const float* dhi, const uint8_t* svf, int64_t v0,
+                          int64_t count, const int64_t* off, const int32_t* ids, int stride, int nthreads,
+                          float* out, double* t_top_s, double* t_range_s) {
+    return annual_fp32_sample([v0](int64_t k) { return v0 + k; }, count, grid, w, h, n_total, n_top, pp, wall_z,
+                              slopes, aspects, naa, sdi, dni, dhi, svf, off, ids, stride, nthreads, out, t_top_s,
+                              t_range_s);
+}
+
+int orc_annual_fp32_subset(const float* grid, int w, int h, int n_total, int n_top, const int* pp,
+                           const float* wall_z, const float* slopes, const float* aspects, const float* naa,
+                           const int* sdi, const float* dni, const float* dhi, const uint8_t* svf,
+                           const int32_t* voxels, int64_t count, const int64_t* off, const int32_t* ids, int stride,
+                           int nthreads, float* out, double* t_top_s, double* t_range_s) {
+    return annual_fp32_sample([voxels](int64_t k) { return (int64_t)voxels[k]; }, count, grid, w, h, n_total, n_top,
+                              pp, wall_z, slopes, aspects, naa, sdi, dni, dhi, svf, off, ids, stride, nthreads, out,
+                              t_top_s, t_range_s);
 }
 
 void orc_annual_fp64(const float* grid, int w, int h, int n_total, int n_top, const int* pp, const float* wall_z,

@@ -91,6 +91,12 @@ int64_t orc_viewshed(const float* grid, int w, int h, int n_total, int n_top, co
                      const float* wall_z, float maxz, const float* cdir, const int* is_upper,
                      const int* dense, int ndir, int hgrid_size_s, int nthreads, uint8_t* svf,
                      int64_t* csr_off, int32_t* ids, int64_t ids_cap, int64_t* total_steps);
+/* Viewshed of an arbitrary voxel list: SVF bits at svf[i + byte * n] (svf may be NULL), CSR
+ * lists by position; returns nnz (-nnz when above cap). */
+int64_t orc_viewshed_subset(const float* grid, int w, int h, int n_top, const int* pix_parent, const float* wall_z,
+                            float maxz, const float* cdir, const int* up, const int* dense, int ndir, int hgs,
+                            const int32_t* voxels, int64_t n, int nthreads, uint8_t* svf, int64_t* off,
+                            int32_t* ids, int64_t cap, int64_t* total_steps);
 
 /* Raw per-direction hits for voxels [v0, v0+count): raw[(v-v0)*ndir + d] = id+1 or 0. */
 void orc_trace_raw(const float* grid, int w, int h, int n_top, const int* pix_parent, const float* wall_z,
@@ -154,6 +160,14 @@ int orc_annual_fp32_range(const float* grid, int w, int h, int n_total, int n_to
                           const float* dhi, const uint8_t* svf, int64_t v0, int64_t count,
                           const int64_t* off, const int32_t* ids, int hour_stride, int nthreads, float* out,
                           double* t_top_s, double* t_range_s);
+/* Same loop for an arbitrary voxel list (voxels[0..count)), lists by position (the CPU
+ * baseline's random sample). */
+int orc_annual_fp32_subset(const float* grid, int w, int h, int n_total, int n_top, const int* pix_parent,
+                           const float* wall_z, const float* slopes, const float* aspects,
+                           const float* nudged_az_alt, const int* sun_dense_idx, const float* dni,
+                           const float* dhi, const uint8_t* svf, const int32_t* voxels, int64_t count,
+                           const int64_t* off, const int32_t* ids, int stride, int nthreads, float* out,
+                           double* t_top_s, double* t_range_s);
 
 /* FP64 oracle.  Same discrete decisions as the FP32 path (sunlit-hour set, SVF bits, neighbour
    sets, Perez epsilon bin taken from the FP32 hour constants); all continuous arithmetic in

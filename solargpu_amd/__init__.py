@@ -451,6 +451,12 @@ class Context:
         _check(_lib.sgpu_copy_bandwidth(self._h, _I64(nbytes), reps, ctypes.byref(g)))
         return g.value
 
+    def valu_rate(self) -> float:
+        """Measured VALU issue rate, wave64 instructions/s (FMA-only probe filling every SIMD)."""
+        r = ctypes.c_double()
+        _check(_lib.sgpu_valu_rate(self._h, ctypes.byref(r)))
+        return r.value
+
 
 class BatchInfo(ctypes.Structure):
     _fields_ = [("tiles", ctypes.c_int32), ("chunks", ctypes.c_int32), ("voxels", ctypes.c_int64),

@@ -364,9 +364,12 @@ __global__ __launch_bounds__(256) void k_features(SurArgs a) {
 }
 
 /* Surrogate total = exact direct + diffuse + predicted reflected share x (direct + diffuse).
- * Predicted error of the total: the calibrated relative error of the reflected share
- * times the predicted reflected fraction of the total, plus the miss on the diffuse share
- * (known exactly here; a large miss marks an input the network does not represent). */
+ * Predicted relative error of that total: with an error network, 10^(its first output), a
+ * regression of the log10 per-voxel error the surrogate makes on held-out tiles; without one,
+ * a score -- the calibrated relative error of the reflected share times the predicted
+ * reflected fraction of the total, plus the miss on the diffuse share (known exactly here; a
+ * large miss marks an input the network does not represent).  Voxels whose predicted error
+ * exceeds tau go to the exact path. */
 __global__ __launch_bounds__(256) void k_gate(SurArgs a) {
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (v >= a.n_total) return;
@@ -374,12 +377,17 @@ __global__ __launch_bounds__(256) void k_gate(SurArgs a) {
     const float share = fmaxf(a.Y[2 * v], 0.0f);
     const float refl = share * dd;
     const float total = dd + refl;
-    const float exact_dshare = dd > 0.0f ? a.diff[v] / dd : 0.0f;
-    const float dmiss = fabsf(a.Y[2 * v + 1] - exact_dshare);
-    const float frac = total > 0.0f ? refl / total : 0.0f;
-    const float err = a.refl_rel_err * frac + a.diff_miss_weight * dmiss + a.share_abs_err * dd / fmaxf(total, 1e-3f);
+    float err;
+    if (a.Yerr != nullptr) {
+        err = exp10f(fminf(a.Yerr[2 * v], 30.0f));
+    } else {
+        const float exact_dshare = dd > 0.0f ? a.diff[v] / dd : 0.0f;
+        const float dmiss = fabsf(a.Y[2 * v + 1] - exact_dshare);
+        const float frac = total > 0.0f ? refl / total : 0.0f;
+        err = a.refl_rel_err * frac + a.diff_miss_weight * dmiss + a.share_abs_err * dd / fmaxf(total, 1e-3f);
+    }
     a.out[v] = total;
-    a.fallback[v] = (err > a.tau || !(total > 0.0f)) ? 1 : 0;
+    a.fallback[v] = (!(err <= a.tau) || !(total > 0.0f)) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const int32_t* ids, const float* vals, int64_t n, float* out) {

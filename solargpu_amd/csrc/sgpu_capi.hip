@@ -42,6 +42,27 @@ __global__ __launch_bounds__(256) void k_copy_f4(const float4* __restrict__ src,
         dst[i] = src[i];
 }
 
+/* VALU issue-rate probe: eight independent FMA chains per lane, 16 x 8 = 128 FMAs per loop turn
+ * (the loop's own counter and branch are scalar instructions), so every wave issues 128 VALU
+ * instructions per turn: the calibration kernel for the VALU-bound DDA. */
+constexpr int kProbeFmaPerTurn = 128;
+__global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int turns, float m, float a) {
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = (float)(threadIdx.x + k);
+    for (int t = 0; t < turns; ++t) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) x[k] = fmaf(x[k], m, a);
+        }
+    }
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum += x[k];
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
 int64_t choose_chunk(const sgpu_ctx* c, int64_t n) {
     if (n <= 0) return 1;
     if (c->chunk_override > 0) return std::min<int64_t>(c->chunk_override, n);
@@ -1460,6 +1481,34 @@ int sgpu_copy_bandwidth(sgpu_ctx* c, int64_t bytes, int reps, double* gbps) {
         }
         dfree(src);
         dfree(dst);
+    });
+}
+
+int sgpu_valu_rate(sgpu_ctx* c, double* wave_insts_per_s) {
+    return guarded([&] {
+        need(c && wave_insts_per_s, SGPU_ERR_ARG, "null argument");
+        c->set_device();
+        int cus = 0;
+        SGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        const int blocks = std::max(cus, 1) * 8;   // 32 waves per CU: every SIMD holds 8
+        const int turns = 4000, reps = 5;
+        float* out = dalloc<float>((size_t)blocks * 256);
+        try {
+            hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, c->stream, out, turns, 0.999f, 1e-3f);
+            SGPU_HIP(hipGetLastError());
+            EventPair ev;
+            SGPU_HIP(hipEventRecord(ev.a, c->stream));
+            for (int r = 0; r < reps; ++r)
+                hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, c->stream, out, turns, 0.999f, 1e-3f);
+            SGPU_HIP(hipEventRecord(ev.b, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            const double insts = (double)blocks * 4.0 * turns * kProbeFmaPerTurn * reps;
+            *wave_insts_per_s = insts / (ev.ms() * 1e-3);
+        } catch (...) {
+            dfree(out);
+            throw;
+        }
+        dfree(out);
     });
 }
 

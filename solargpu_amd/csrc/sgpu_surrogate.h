@@ -33,6 +33,7 @@ struct SurArgs {
     uint16_t* X;               // features [n_total][64] bf16
     float lat, albedo, cell_m;
     const float* Y;            // MLP output [n_total][2]
+    const float* Yerr;         // error network output [n_total][2] (log10 of the predicted error), or null
     float* out;                // surrogate annual total
     int32_t* fallback;         // 1 = take the exact plain result
     float tau, refl_rel_err, diff_miss_weight, share_abs_err;

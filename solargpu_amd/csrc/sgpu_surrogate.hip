@@ -137,8 +137,8 @@ int sgpu_surrogate_labels(sgpu_ctx* c, unsigned fmt, float* reflected) {
     });
 }
 
-int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, const sgpu_gate* gate, int hours_per_pass, float* out_host,
-                       int32_t* fallback_host, sgpu_surrogate_stats* st) {
+int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, sgpu_mlp* error_mlp, const sgpu_gate* gate, int hours_per_pass,
+                       float* out_host, int32_t* fallback_host, sgpu_surrogate_stats* st) {
     return guarded([&] {
         need(c && mlp && gate && out_host, SGPU_ERR_ARG, "null argument");
         c->set_device();
@@ -153,6 +153,7 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, const sgpu_gate* gate, int ho
         SGPU_HIP(hipEventRecord(all.a, c->stream));
         prepare(c, c->lat, c->cell_m, s);
         float* Y = dalloc<float>((size_t)N * 2);
+        float* Yerr = nullptr;
         float* out = nullptr;
         int32_t* flags = nullptr;
         int64_t* off = nullptr;
@@ -161,6 +162,7 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, const sgpu_gate* gate, int ho
         float* exact = nullptr;
         SubsetLists L;
         try {
+            if (error_mlp) Yerr = dalloc<float>((size_t)N * 2);
             out = dalloc<float>(N);
             flags = dalloc<int32_t>(N);
             off = dalloc<int64_t>(N + 1);
@@ -170,8 +172,14 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, const sgpu_gate* gate, int ho
             if (sgpu_mlp_forward_device(mlp, c->sur.X, N, Y, &mlp_ms) != SGPU_OK)
                 fail(SGPU_ERR_HIP, "MLP forward failed: %s", sgpu_mlp_last_error());
             s->t_mlp_ms = mlp_ms;
+            if (error_mlp) {
+                if (sgpu_mlp_forward_device(error_mlp, c->sur.X, N, Yerr, &mlp_ms) != SGPU_OK)
+                    fail(SGPU_ERR_HIP, "error-network forward failed: %s", sgpu_mlp_last_error());
+                s->t_mlp_ms += mlp_ms;
+            }
             SurArgs a = sur_args(c);
             a.Y = Y;
+            a.Yerr = Yerr;
             a.out = out;
             a.fallback = flags;
             a.tau = gate->tau;
@@ -208,11 +216,11 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, const sgpu_gate* gate, int ho
             s->t_total_ms = all.ms();
         } catch (...) {
             L.release();
-            dfree(Y); dfree(out); dfree(flags); dfree(off); dfree(scr); dfree(ids); dfree(exact);
+            dfree(Y); dfree(Yerr); dfree(out); dfree(flags); dfree(off); dfree(scr); dfree(ids); dfree(exact);
             throw;
         }
         L.release();
-        dfree(Y); dfree(out); dfree(flags); dfree(off); dfree(scr); dfree(ids); dfree(exact);
+        dfree(Y); dfree(Yerr); dfree(out); dfree(flags); dfree(off); dfree(scr); dfree(ids); dfree(exact);
     });
 }
 

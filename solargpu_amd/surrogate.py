@@ -117,9 +117,10 @@ class MLP:
 # ---------------- surrogate pipeline ----------------
 
 class Gate(ctypes.Structure):
-    """Fallback rule: exact path where the predicted relative error of the annual total,
+    """Fallback rule: exact path where the predicted relative error of the annual total exceeds
+    tau.  With an error network the prediction is 10^(its first output); without one it is
     refl_rel_err * predicted reflected fraction + diff_miss_weight * |diffuse-share miss|
-    + share_abs_err * direct_diffuse / total, exceeds tau."""
+    + share_abs_err * direct_diffuse / total."""
     _fields_ = [("tau", ctypes.c_float), ("refl_rel_err", ctypes.c_float),
                 ("diff_miss_weight", ctypes.c_float), ("share_abs_err", ctypes.c_float)]
 
@@ -161,25 +162,60 @@ def targets(refl, dd, diff) -> np.ndarray:
     return np.stack([refl / scale, diff / scale], 1).astype(np.float32)
 
 
-def run(ctx, mlp: MLP, gate: Gate, hours_per_pass: int = 0):
+def run(ctx, mlp: MLP, gate: Gate, hours_per_pass: int = 0, error_mlp: MLP | None = None):
     """Surrogate annual totals.  hours_per_pass > 0: per-hour exact fallback with K hours per
-    pass (0 = the context's tuned K); FALLBACK_ANNUAL: one-walk annual fallback."""
+    pass (0 = the context's tuned K); FALLBACK_ANNUAL: one-walk annual fallback.  error_mlp:
+    the error network (first output = log10 of the predicted relative error of the surrogate
+    total); voxels whose predicted error exceeds gate.tau go to the exact path."""
     n = ctx.info()["num_total"]
     out = np.zeros(n, np.float32)
     fb = np.zeros(n, np.int32)
     st = Stats()
-    _check(_lib.sgpu_surrogate_run(ctx._h, mlp._h, ctypes.byref(gate), hours_per_pass, _ptr(out), _ptr(fb),
-                                   ctypes.byref(st)))
+    _check(_lib.sgpu_surrogate_run(ctx._h, mlp._h, error_mlp._h if error_mlp is not None else None,
+                                   ctypes.byref(gate), hours_per_pass, _ptr(out), _ptr(fb), ctypes.byref(st)))
     return out, fb, st.as_dict()
 
 
-def save(path: str, params: np.ndarray, gate: Gate, meta: dict) -> None:
+# error network target: log10 of the relative error of the annual total, floored here
+ERR_FLOOR = 1e-4
+
+
+def error_target(Y: np.ndarray, dd: np.ndarray, refl: np.ndarray) -> np.ndarray:
+    """Per-voxel relative error of the surrogate total dd (1 + max(share, 0)) against the plain
+    total dd + refl (floor 1 Wh/m^2), as the error network's targets [log10(err), 0]."""
+    total = dd.astype(np.float64) * (1.0 + np.maximum(Y[:, 0], 0.0))
+    exact = dd.astype(np.float64) + refl
+    err = np.abs(total - exact) / np.maximum(np.abs(exact), 1.0)
+    return np.stack([np.log10(np.maximum(err, ERR_FLOOR)), np.zeros_like(err)], 1).astype(np.float32)
+
+
+def save(path: str, params: np.ndarray, gate: Gate, meta: dict, error_params: np.ndarray | None = None) -> None:
+    extra = {} if error_params is None else {"error_params": error_params.astype(np.float32)}
     np.savez(path, params=params.astype(np.float32),
              gate=np.array([gate.tau, gate.refl_rel_err, gate.diff_miss_weight, gate.share_abs_err], np.float32),
-             meta=np.frombuffer(repr(meta).encode(), np.uint8))
+             meta=np.frombuffer(repr(meta).encode(), np.uint8), **extra)
 
 
 def load(path: str) -> tuple[np.ndarray, Gate, str]:
     z = np.load(path, allow_pickle=False)
     g = z["gate"]
     return z["params"], Gate(*[float(x) for x in g]), z["meta"].tobytes().decode()
+
+
+def load_error_params(path: str) -> np.ndarray | None:
+    """The error network's parameters of a weights file (None when it has none)."""
+    z = np.load(path, allow_pickle=False)
+    return z["error_params"] if "error_params" in z.files else None
+
+
+def load_models(path: str, device: int = 0):
+    """(network, error network or None, gate, meta) of a weights file, on the device."""
+    params, gate, meta = load(path)
+    net = MLP(seed=1, device=device)
+    net.set_params(params)
+    ep = load_error_params(path)
+    err = None
+    if ep is not None:
+        err = MLP(seed=2, device=device)
+        err.set_params(ep)
+    return net, err, gate, meta

@@ -35,8 +35,8 @@ def load_oracle():
     if not os.path.exists(ORACLE_SO):
         raise ImportError(f"{ORACLE_SO} missing: run make -C oracle")
     lib = ctypes.CDLL(ORACLE_SO)
-    for name in ("orc_viewshed", "orc_detect_walls", "orc_barle_encode_voxel", "orc_barle_encode_all",
-                 "orc_check_setup_division"):
+    for name in ("orc_viewshed", "orc_viewshed_subset", "orc_detect_walls", "orc_barle_encode_voxel",
+                 "orc_barle_encode_all", "orc_check_setup_division"):
         getattr(lib, name).restype = ctypes.c_int64
     return lib
 
@@ -132,6 +132,29 @@ def viewshed(sc, bins, nthreads=0):
                              ctypes.byref(steps))
         if nnz >= 0:
             return dict(svf=svf[: hgs * N], off=off, ids=ids[:nnz].copy(), steps=steps.value)
+        cap = -nnz
+
+
+def viewshed_subset(sc, bins, voxels, nthreads=0, with_svf=False):
+    """CSR lists (by position) of the voxels `voxels`, optionally with their SVF bits
+    (word-major over the subset); also the DDA step count."""
+    cdir, up, dense = directions(bins["full_to_dense"])
+    vox = _c(voxels, np.int32)
+    n = vox.size
+    hgs = bins["svf_bytes"]
+    svf = np.zeros(max(hgs * n, 1), np.uint8) if with_svf else None
+    off = np.zeros(n + 1, np.int64)
+    steps = ctypes.c_int64()
+    cap = 600 * n + 1024
+    while True:
+        ids = np.zeros(cap, np.int32)
+        nnz = O.orc_viewshed_subset(_ptr(sc.grid), sc.width, sc.height, sc.num_top, _ptr(sc.pixel_parent),
+                                    _ptr(sc.wall_z) if sc.wall_z.size else None, _F(sc.max_height), _ptr(cdir),
+                                    _ptr(up), _ptr(dense), NDIRS, hgs, _ptr(vox), _I64(n), nthreads, _ptr(svf),
+                                    _ptr(off), _ptr(ids), _I64(cap), ctypes.byref(steps))
+        if nnz >= 0:
+            return dict(svf=None if svf is None else svf[: hgs * n], off=off, ids=ids[:nnz].copy(),
+                        steps=steps.value)
         cap = -nnz
 
 
@@ -241,6 +264,21 @@ def annual_fp32_range(sc, bins, dni, dhi, svf, v0, off_slice, ids, stride=1, nth
                                 _ptr(_c(dni, np.float32)), _ptr(_c(dhi, np.float32)), _ptr(_c(svf, np.uint8)),
                                 _I64(v0), _I64(count), _ptr(_c(off_slice, np.int64)), _ptr(ids), stride, nthreads,
                                 _ptr(out), ctypes.byref(tt), ctypes.byref(tr))
+    return out, n, tt.value, tr.value
+
+
+def annual_fp32_subset(sc, bins, dni, dhi, svf, voxels, off_sub, ids, stride=1, nthreads=0):
+    """Reference-structured FP32 hour loop for the voxels `voxels` with their lists by position;
+    returns (annual, hours done, all-top solar seconds, per-voxel seconds)."""
+    vox = _c(voxels, np.int32)
+    out = np.zeros(vox.size, np.float32)
+    ids = _c(ids, np.int32)
+    tt, tr = ctypes.c_double(), ctypes.c_double()
+    n = O.orc_annual_fp32_subset(*_scene_args(sc), _ptr(bins["nudged"]), _ptr(bins["sun_dense_idx"]),
+                                 _ptr(_c(dni, np.float32)), _ptr(_c(dhi, np.float32)), _ptr(_c(svf, np.uint8)),
+                                 _ptr(vox), _I64(vox.size), _ptr(_c(off_sub, np.int64)),
+                                 _ptr(ids) if ids.size else None, stride, nthreads, _ptr(out), ctypes.byref(tt),
+                                 ctypes.byref(tr))
     return out, n, tt.value, tr.value
 
 

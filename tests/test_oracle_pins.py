@@ -309,3 +309,23 @@ def test_perez_epsilon_bins_agree_with_fp64_away_from_edges(site, seed):
     print(f"site {site}: {hours.size} sunlit hours, {int(near.sum())} within 1e-5 of a Perez bin edge "
           f"(hours {hours[near].tolist()})")
     np.testing.assert_array_equal(bin32[~near], bin64[~near])
+
+
+def test_subset_forms_equal_the_whole_scene_forms():
+    """The CPU baseline's random-sample forms (viewshed and FP32 hour loop over a voxel list)
+    give the whole-scene results at those voxels."""
+    sc = synth_scene(25, 20, 16)
+    az_alt, dni, dhi, bins = sky()
+    ref = orc.viewshed(sc, bins)
+    vox = np.random.default_rng(5).choice(sc.num_total, 60, replace=False).astype(np.int32)
+    sub = orc.viewshed_subset(sc, bins, vox, with_svf=True)
+    lens = np.diff(ref["off"])[vox]
+    np.testing.assert_array_equal(np.diff(sub["off"]), lens)
+    want_ids = np.concatenate([ref["ids"][ref["off"][v]:ref["off"][v + 1]] for v in vox])
+    np.testing.assert_array_equal(sub["ids"], want_ids)
+    N, n, hgs = sc.num_total, vox.size, bins["svf_bytes"]
+    np.testing.assert_array_equal(sub["svf"].reshape(hgs, n), ref["svf"].reshape(hgs, N)[:, vox])
+    full, _ = orc.annual_fp32(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"], stride=7)
+    part, nh, _, _ = orc.annual_fp32_subset(sc, bins, dni, dhi, ref["svf"], vox, sub["off"], sub["ids"], stride=7)
+    assert nh == len(range(0, orc.sunlit_hours(bins["nudged"]).size, 7))
+    np.testing.assert_array_equal(part, full[vox])

@@ -112,6 +112,30 @@ def test_no_fallback_is_direct_diffuse_plus_prediction():
     np.testing.assert_allclose(out, want, rtol=1e-5)
 
 
+def test_error_network_gate_sends_high_predicted_error_to_the_exact_path():
+    """With an error network the gate falls back exactly where 10^(its first output) exceeds tau,
+    and those voxels get the plain method's annual totals."""
+    sc = synth_scene(11, 48, 40)
+    ref = oracle_lists(11, 48, 40)
+    az_alt, dni, dhi, bins = sky()
+    exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
+    mlp = sm.MLP(seed=3)
+    err = sm.MLP(seed=4)
+    with _ctx(sc, lists=False) as ctx:
+        sm.prepare(ctx, SITE[1])
+        X, dd, _ = sm.features(ctx)
+        p = err.forward(X)[:, 0]
+        tau = float(10.0 ** np.percentile(p, 70))        # about 30% of the voxels above it
+        out, fb, st = sm.run(ctx, mlp, sm.Gate(tau, 0.0, 0.0, 0.0), sm.FALLBACK_ANNUAL, err)
+    want_fb = ~(10.0 ** p.astype(np.float64) <= tau * (1 + 1e-6))
+    near = np.abs(p - np.log10(tau)) < 1e-5                 # exp10f vs numpy at the threshold
+    np.testing.assert_array_equal(fb.astype(bool)[~near], want_fb[~near])
+    assert 0.2 < fb.mean() < 0.4 and st["fallback"] == fb.sum()
+    assert _rel(out[fb == 1], exact[fb == 1]).max() <= 1e-4
+    Y = mlp.forward(X)
+    np.testing.assert_allclose(out[fb == 0], (dd * (1.0 + np.maximum(Y[:, 0], 0.0)))[fb == 0], rtol=1e-5)
+
+
 def test_trained_network_beats_the_method_without_it():
     """Train on small seeded tiles labelled by the plain method; on held-out tiles the
     trained surrogate's p95 error of the annual total must beat (a) dropping the
