@@ -167,6 +167,8 @@ int sgpu_get_csr(sgpu_ctx* ctx, int64_t* offsets /* num_total+1 */, int32_t* ids
 int sgpu_get_barle(sgpu_ctx* ctx, int64_t* offsets /* num_total+1 */, uint8_t* bytes);
 /* CSR lists of voxels [v0, v0+count): offsets rebased to 0 (count+1), ids up to cap (new). */
 int sgpu_get_csr_range(sgpu_ctx* ctx, int64_t v0, int64_t count, int64_t* offsets, int32_t* ids, int64_t cap);
+/* ba-RLE streams of voxels [v0, v0+count): offsets rebased to 0 (count+1), bytes up to cap (new). */
+int sgpu_get_barle_range(sgpu_ctx* ctx, int64_t v0, int64_t count, int64_t* offsets, uint8_t* bytes, int64_t cap);
 /* Raw first-hit ids (id+1, 0 = none) of voxels [v0, v0+count) x 2592 directions (new; tests). */
 int sgpu_trace_raw(sgpu_ctx* ctx, int64_t v0, int64_t count, int32_t* raw_out);
 /* Supply lists/SVF from elsewhere (tests, a persisted cache) instead of tracing. (new) */

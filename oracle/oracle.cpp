@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -961,14 +962,22 @@ int orc_annual_fp32_subset(const float* grid, int w, int h, int n_total, int n_t
                               t_top_s, t_range_s);
 }
 
-void orc_annual_fp64(const float* grid, int w, int h, int n_total, int n_top, const int* pp, const float* wall_z,
-                     const float* slopes, const float* aspects, const float* naa, const int* sdi, const float* dni,
-                     const float* dhi, const uint8_t* svf, const int64_t* off, const int32_t* ids,
-                     const int32_t* voxels, int n_sel, int nthreads, double* out) {
+}  // extern "C"
+
+namespace {
+
+/* FP64 annual totals by single-bounce linearity (the Q24 identity, checked against the per-hour
+ * FP64 sum): each output voxel's own annual direct + diffuse plus the sum over its neighbours of
+ * the hour-invariant weight times the neighbour cell's annual direct + diffuse.  lists(i, t)
+ * gives output i's (voxel t's) neighbour ids as a [begin, end) pair of pointers. */
+template <typename Lists>
+void annual_fp64_impl(const float* grid, int w, int h, int n_total, int n_top, const int* pp, const float* wall_z,
+                      const float* slopes, const float* aspects, const float* naa, const int* sdi, const float* dni,
+                      const float* dhi, const uint8_t* svf, const int32_t* voxels, int64_t n_out, int nthreads,
+                      Lists lists, double* out) {
     std::vector<int> hs = sunlit(naa);
     std::vector<HourD> hd;
     for (int idx : hs) hd.push_back(hour_d(naa, sdi, dni, dhi, idx));
-    const int64_t n_out = voxels ? n_sel : n_total;
     auto voxel_of = [&](int64_t i) -> int64_t { return voxels ? voxels[i] : i; };
     auto annual_direct = [&](int64_t t) -> double {
         double s = 0.0;
@@ -981,9 +990,9 @@ void orc_annual_fp64(const float* grid, int w, int h, int n_total, int n_top, co
     /* Annual hourly sums of every top cell some selected voxel sees. */
     std::vector<char> need((size_t)n_top, 0);
     for (int64_t i = 0; i < n_out; ++i) {
-        int64_t t = voxel_of(i);
-        for (int64_t k = off[t]; k < off[t + 1]; ++k) {
-            int c = ids[k] - 1;
+        auto l = lists(i, voxel_of(i));
+        for (const int32_t* p = l.first; p < l.second; ++p) {
+            int c = *p - 1;
             if (c >= 0 && c < n_top) need[c] = 1;
         }
     }
@@ -998,14 +1007,40 @@ void orc_annual_fp64(const float* grid, int w, int h, int n_total, int n_top, co
             int64_t t = voxel_of(i);
             double s = (t < n_top && need[t]) ? A[t] : annual_direct(t);
             double r = 0.0;
-            for (int64_t k = off[t]; k < off[t + 1]; ++k) {
-                int c = ids[k] - 1;
+            auto l = lists(i, t);
+            for (const int32_t* p = l.first; p < l.second; ++p) {
+                int c = *p - 1;
                 double wgt = weight_d(t, c, w, h, n_top, pp, grid, wall_z, slopes, aspects);
                 if (wgt != 0.0) r += wgt * A[c];
             }
             out[i] = s + r;
         }
     });
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_annual_fp64(const float* grid, int w, int h, int n_total, int n_top, const int* pp, const float* wall_z,
+                     const float* slopes, const float* aspects, const float* naa, const int* sdi, const float* dni,
+                     const float* dhi, const uint8_t* svf, const int64_t* off, const int32_t* ids,
+                     const int32_t* voxels, int n_sel, int nthreads, double* out) {
+    const int64_t n_out = voxels ? n_sel : n_total;
+    annual_fp64_impl(grid, w, h, n_total, n_top, pp, wall_z, slopes, aspects, naa, sdi, dni, dhi, svf, voxels, n_out,
+                     nthreads,
+                     [&](int64_t, int64_t t) { return std::make_pair(ids + off[t], ids + off[t + 1]); }, out);
+}
+
+void orc_annual_fp64_subset(const float* grid, int w, int h, int n_total, int n_top, const int* pp,
+                            const float* wall_z, const float* slopes, const float* aspects, const float* naa,
+                            const int* sdi, const float* dni, const float* dhi, const uint8_t* svf,
+                            const int32_t* voxels, int64_t n_sel, const int64_t* off_sub, const int32_t* ids_sub,
+                            int nthreads, double* out) {
+    annual_fp64_impl(grid, w, h, n_total, n_top, pp, wall_z, slopes, aspects, naa, sdi, dni, dhi, svf, voxels, n_sel,
+                     nthreads,
+                     [&](int64_t i, int64_t) { return std::make_pair(ids_sub + off_sub[i], ids_sub + off_sub[i + 1]); },
+                     out);
 }
 
 void orc_annual_fp64_direct(const float* grid, int w, int h, int n_total, int n_top, const int* pp,

@@ -178,6 +178,13 @@ void orc_annual_fp64(const float* grid, int w, int h, int n_total, int n_top, co
                      const float* nudged_az_alt, const int* sun_dense_idx, const float* dni,
                      const float* dhi, const uint8_t* svf, const int64_t* off, const int32_t* ids,
                      const int32_t* voxels, int n_sel, int nthreads, double* out);
+/* Same for a voxel list with its lists by position (off_sub: n_sel + 1, ids_sub): for scenes
+ * whose whole CSR does not fit host memory. */
+void orc_annual_fp64_subset(const float* grid, int w, int h, int n_total, int n_top, const int* pix_parent,
+                            const float* wall_z, const float* slopes, const float* aspects,
+                            const float* nudged_az_alt, const int* sun_dense_idx, const float* dni,
+                            const float* dhi, const uint8_t* svf, const int32_t* voxels, int64_t n_sel,
+                            const int64_t* off_sub, const int32_t* ids_sub, int nthreads, double* out);
 
 /* Per-hour FP64 sum without the linearity shortcut (tiny scenes; pins orc_annual_fp64). */
 void orc_annual_fp64_direct(const float* grid, int w, int h, int n_total, int n_top,

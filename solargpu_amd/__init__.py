@@ -373,6 +373,14 @@ class Context:
         _check(_lib.sgpu_get_csr_range(self._h, _I64(v0), _I64(count), _ptr(off), _ptr(ids), _I64(cap)))
         return off, ids[: off[-1]].copy()
 
+    def barle_range(self, v0: int, count: int) -> tuple[np.ndarray, np.ndarray]:
+        """ba-RLE streams of voxels [v0, v0 + count), offsets rebased to 0."""
+        off = np.zeros(count + 1, np.int64)
+        cap = count * (6 * NDIRS + 8)          # a stream of n ids takes at most 5.375 n + 3 bytes
+        b = np.zeros(max(cap, 1), np.uint8)
+        _check(_lib.sgpu_get_barle_range(self._h, _I64(v0), _I64(count), _ptr(off), _ptr(b), _I64(cap)))
+        return off, b[: off[-1]].copy()
+
     def barle(self) -> tuple[np.ndarray, np.ndarray]:
         inf = self.info()
         off = np.zeros(inf["num_total"] + 1, np.int64)

@@ -1249,6 +1249,26 @@ int sgpu_get_csr_range(sgpu_ctx* c, int64_t v0, int64_t count, int64_t* off, int
     });
 }
 
+int sgpu_get_barle_range(sgpu_ctx* c, int64_t v0, int64_t count, int64_t* off, uint8_t* bytes, int64_t cap) {
+    return guarded([&] {
+        need(c && off, SGPU_ERR_ARG, "null argument");
+        need(c->br_size >= 0 && c->br_off, SGPU_ERR_STATE, "no device-resident ba-RLE streams");
+        need(v0 >= 0 && count >= 0 && v0 + count <= c->n_total, SGPU_ERR_ARG, "voxel range out of bounds");
+        c->set_device();
+        SGPU_HIP(hipMemcpyAsync(off, c->br_off + v0, (count + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        const int64_t base = off[0], n = off[count] - base;
+        need(n <= cap, SGPU_ERR_ARG, "bytes capacity too small");
+        for (int64_t i = 0; i <= count; ++i) off[i] -= base;
+        if (n > 0) {
+            need(bytes != nullptr, SGPU_ERR_ARG, "null bytes");
+            SGPU_HIP(hipMemcpyAsync(bytes, c->br_bytes + base, n, hipMemcpyDeviceToHost, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
 int sgpu_trace_raw(sgpu_ctx* c, int64_t v0, int64_t count, int32_t* out) {
     return guarded([&] {
         need(c && out, SGPU_ERR_ARG, "null argument");

@@ -294,6 +294,18 @@ def annual_fp64(sc, bins, dni, dhi, svf, off, ids, voxels=None, nthreads=0):
     return out
 
 
+def annual_fp64_subset(sc, bins, dni, dhi, svf, voxels, off_sub, ids_sub, nthreads=0):
+    """FP64 annual totals of the voxels `voxels` from their lists by position."""
+    vox = _c(voxels, np.int32)
+    ids = _c(ids_sub, np.int32)
+    out = np.zeros(vox.size, np.float64)
+    O.orc_annual_fp64_subset(*_scene_args(sc), _ptr(bins["nudged"]), _ptr(bins["sun_dense_idx"]),
+                             _ptr(_c(dni, np.float32)), _ptr(_c(dhi, np.float32)), _ptr(_c(svf, np.uint8)),
+                             _ptr(vox), _I64(vox.size), _ptr(_c(off_sub, np.int64)), _ptr(ids) if ids.size else None,
+                             nthreads, _ptr(out))
+    return out
+
+
 def annual_fp64_direct(sc, bins, dni, dhi, svf, off, ids):
     ids = _c(ids, np.int32)
     out = np.zeros(sc.num_total, np.float64)

@@ -325,6 +325,9 @@ def test_subset_forms_equal_the_whole_scene_forms():
     np.testing.assert_array_equal(sub["ids"], want_ids)
     N, n, hgs = sc.num_total, vox.size, bins["svf_bytes"]
     np.testing.assert_array_equal(sub["svf"].reshape(hgs, n), ref["svf"].reshape(hgs, N)[:, vox])
+    np.testing.assert_array_equal(
+        orc.annual_fp64_subset(sc, bins, dni, dhi, ref["svf"], vox, sub["off"], sub["ids"]),
+        orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"], voxels=vox))
     full, _ = orc.annual_fp32(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"], stride=7)
     part, nh, _, _ = orc.annual_fp32_subset(sc, bins, dni, dhi, ref["svf"], vox, sub["off"], sub["ids"], stride=7)
     assert nh == len(range(0, orc.sunlit_hours(bins["nudged"]).size, 7))

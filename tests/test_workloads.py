@@ -1,0 +1,200 @@
+"""The three larger workloads at their full sizes on the GPU.
+
+* Batched tiles: 64 seeded tiles, sides log-uniform in 128..1024 cells of 1 m, building density
+  U(10%, 60%), each with its own site and weather; one persistent viewshed and one hipGraph
+  hour loop over all of them.  Every tile's CSR lists, SVF bits, ba-RLE streams and annual sums
+  equal a context of its own bit for bit, and sampled voxels of three tiles equal the oracle.
+* Surrogate evaluation: the shipped networks (solargpu_amd/weights/surrogate.npz) on the 20
+  held-out 1024^2 tiles against the plain pipeline on the same GPU: p95 per-voxel annual error
+  within 2%, at least 5x fewer GPU-seconds (one-walk fallback), the gate catching most voxels
+  whose error would exceed 2%, and the plain annual sums of sampled voxels within 1e-4 of FP64.
+* 8192 x 8192 at 0.5 m, 50% buildings (194.6M voxels, ba-RLE lists only): sampled voxels'
+  streams are the oracle encoder's bytes for the oracle's first-hit lists, their SVF bits the
+  oracle's, their annual sums within 1e-4 of FP64; the surrogate against the full plain pass.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import solargpu_amd as sg
+import solargpu_amd.surrogate as sm
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "solargpu_amd", "weights", "surrogate.npz")
+TUNED = (32, 128, 4, 256, 32)       # trace group, hours per pass, waves, XCD band, receiver group
+ANNUAL_RTOL = 1e-4
+
+
+def _first_occurrences(row):
+    hits = row[row != 0]
+    _, first = np.unique(hits, return_index=True)
+    return hits[np.sort(first)]
+
+
+def _svf_bits(raw, dense, nb):
+    want = np.zeros(nb, np.uint8)
+    for d in np.nonzero((raw != 0) & (dense >= 0))[0]:
+        want[dense[d] >> 3] |= np.uint8(1 << (dense[d] & 7))
+    return want
+
+
+def _sample(sc, n, seed):
+    rng = np.random.default_rng(seed)
+    walls = np.arange(sc.num_top, sc.num_total)
+    k = n // 2 if walls.size else 0
+    picks = [rng.choice(sc.num_top, n - k, replace=False)]
+    if k:
+        picks.append(rng.choice(walls, min(k, walls.size), replace=False))
+    return np.sort(np.concatenate(picks)).astype(np.int32)
+
+
+def _oracle_sample(sc, bins, vox):
+    """The oracle's first-hit lists (by position) and SVF bits of the sampled voxels."""
+    _, _, dense = orc.directions(bins["full_to_dense"])
+    nb = bins["svf_bytes"]
+    rows, svf = [], []
+    for v in vox:
+        raw = orc.trace_raw(sc, bins, int(v), 1)[0]
+        rows.append(_first_occurrences(raw))
+        svf.append(_svf_bits(raw, dense, nb))
+    off = np.zeros(vox.size + 1, np.int64)
+    off[1:] = np.cumsum([r.size for r in rows])
+    ids = np.concatenate(rows).astype(np.int32) if off[-1] else np.zeros(0, np.int32)
+    return rows, np.array(svf), off, ids
+
+
+@pytest.mark.timeout(900)
+def test_batched_64_tiles_equal_per_tile_contexts_and_the_oracle():
+    from solargpu_amd.bench_batch import c3_tiles
+    tiles = c3_tiles(64)
+    sides = [sc.width for sc, *_ in tiles]
+    print(f"batched: 64 tiles, sides {min(sides)}..{max(sides)}, "
+          f"{sum(sc.num_total for sc, *_ in tiles)} voxels", flush=True)
+    G, K, waves = TUNED[0], TUNED[1], TUNED[2]
+    with sg.Batch(0) as b:
+        for sc, az, dni, dhi in tiles:
+            b.add_tile(sc, az, dni, dhi)
+        b.viewshed()
+        b.hour_loop(sg.FMT_BARLE, K, waves, graph=True)
+        for t, (sc, az, dni, dhi) in enumerate(tiles):
+            with sg.Context(0) as ctx:
+                ctx.set_scene(sc)
+                ctx.set_sky(az, dni, dhi)
+                ctx.set_tuning(G, K, waves, 0, 0)
+                ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
+                off, ids = b.csr(t)
+                eoff, eids = ctx.csr()
+                np.testing.assert_array_equal(off, eoff, err_msg=f"tile {t} offsets")
+                np.testing.assert_array_equal(ids, eids, err_msg=f"tile {t} ids")
+                oracle = t in (0, 31, 63)                 # three tiles against the oracle
+                if oracle:
+                    bins = orc.sky_bins(az)
+                    vox = _sample(sc, 128, 7 + t)
+                    rows, svf_want, o_off, o_ids = _oracle_sample(sc, bins, vox)
+                    for i, v in enumerate(vox):
+                        np.testing.assert_array_equal(ids[off[v]:off[v + 1]], rows[i], err_msg=f"tile {t} voxel {v}")
+                del ids, eids
+                np.testing.assert_array_equal(b.svf(t), ctx.svf(), err_msg=f"tile {t} svf")
+                boff, data = b.barle(t)
+                e_boff, e_data = ctx.barle()
+                np.testing.assert_array_equal(boff, e_boff, err_msg=f"tile {t} ba-RLE offsets")
+                np.testing.assert_array_equal(data, e_data, err_msg=f"tile {t} ba-RLE bytes")
+                del data, e_data
+                ctx.hour_loop(sg.FMT_BARLE, K)
+                irr = b.irradiance(t)
+                np.testing.assert_array_equal(irr, ctx.irradiance(), err_msg=f"tile {t} annual")
+                if oracle:
+                    svf = ctx.svf()
+                    nb, N = bins["svf_bytes"], sc.num_total
+                    np.testing.assert_array_equal(svf.reshape(nb, N)[:, vox].T, svf_want)
+                    exact = orc.annual_fp64_subset(sc, bins, dni, dhi, svf, vox, o_off, o_ids)
+                    rel = np.abs(irr[vox].astype(np.float64) - exact) / np.maximum(np.abs(exact), 1.0)
+                    assert rel.max() <= ANNUAL_RTOL, (t, rel.max())
+        print(f"batched: info {b.info()}", flush=True)
+
+
+@pytest.mark.timeout(900)
+def test_surrogate_on_the_20_held_out_1024_tiles():
+    from solargpu_amd.eval_surrogate import EVAL_SEED, evaluate
+    from solargpu_amd.train_surrogate import make_tile
+    mlp, err_mlp, gate, _ = sm.load_models(WEIGHTS)
+    assert err_mlp is not None, "the shipped weights carry the error network"
+    seeds = [EVAL_SEED + 1000 * k for k in range(20)]
+    summary, per_tile = evaluate(seeds, 1024, 1.0, mlp, gate, 0, log=lambda s: print(s, flush=True),
+                                 tuning=TUNED, exact_all=False, err_mlp=err_mlp)
+    an, hr = summary["annual_fallback"], summary["hourly_fallback"]
+    print(f"surrogate 20 x 1024^2: {an}", flush=True)
+    assert an["voxels"] > 40_000_000
+    assert an["p95"] <= 0.02 and hr["p95"] <= 0.02
+    assert an["gpu_seconds_ratio"] >= 5.0, an
+    assert hr["recall_above_2pct"] >= 0.7, hr                # the gate catches most voxels above 2%
+    # the plain labels (the judge of the surrogate) against the FP64 oracle on sampled voxels
+    ctx, sc, lat = make_tile(seeds[0], 1024, 1.0)
+    with ctx:
+        ctx.set_tuning(*TUNED)
+        ctx.hour_loop(sg.FMT_BARLE, 0)
+        irr = ctx.irradiance()
+        off, ids = ctx.csr()
+        svf = ctx.svf()
+    from solargpu_amd.train_surrogate import tile_site
+    density, lat, lon, tz = tile_site(seeds[0])
+    az = sg.sun_positions(lon, lat, tz)
+    dni, dhi = sg.synth_weather(seeds[0], az)
+    bins = orc.sky_bins(az)
+    vox = _sample(sc, 1024, 3)
+    exact = orc.annual_fp64(sc, bins, dni, dhi, svf, off, ids, voxels=vox)
+    rel = np.abs(irr[vox].astype(np.float64) - exact) / np.maximum(np.abs(exact), 1.0)
+    assert rel.max() <= ANNUAL_RTOL
+    mlp.close()
+    err_mlp.close()
+
+
+@pytest.mark.timeout(1200)
+def test_8192_barle_only_sampled_oracle_and_surrogate_vs_plain():
+    W, cell, density, seed = 8192, 0.5, 0.5, 20260105
+    site = (15.65, 46.56, 1)
+    trees = int(round(40 * (W * cell / 256.0) ** 2))
+    sc = sg.assemble_scene(sg.synth_dsm(seed, W, W, cell, density, trees), cell)
+    az = sg.sun_positions(*site)
+    dni, dhi = sg.synth_weather(seed, az)
+    bins = orc.sky_bins(az)
+    print(f"8192^2: {sc.num_total} voxels ({sc.num_top} top)", flush=True)
+    mlp, err_mlp, gate, _ = sm.load_models(WEIGHTS)
+    with sg.Context(0) as ctx:
+        ctx.set_scene(sc)
+        ctx.set_sky(az, dni, dhi)
+        ctx.set_tuning(*TUNED)
+        ctx.viewshed(sg.FMT_BARLE)
+        t_vs = ctx.info()["t_viewshed_ms"]
+        assert ctx.info()["barle_bytes"] > (1 << 31)          # beyond int32 byte offsets
+        vox = _sample(sc, 512, 5)
+        rows, svf_want, o_off, o_ids = _oracle_sample(sc, bins, vox)
+        for i, v in enumerate(vox):
+            boff, stream = ctx.barle_range(int(v), 1)
+            np.testing.assert_array_equal(stream, orc.barle_encode_voxel(rows[i]), err_msg=f"voxel {v}")
+        svf = ctx.svf()
+        N, nb = sc.num_total, bins["svf_bytes"]
+        np.testing.assert_array_equal(svf.reshape(nb, N)[:, vox].T, svf_want)
+        ctx.hour_loop(sg.FMT_BARLE, 0)
+        t_hr = ctx.info()["t_hours_ms"]
+        plain = ctx.irradiance()
+        print(f"8192^2 plain: viewshed {t_vs:.0f} ms, hour loop {t_hr:.0f} ms", flush=True)
+        exact = orc.annual_fp64_subset(sc, bins, dni, dhi, svf, vox, o_off, o_ids)
+        del svf
+        rel = np.abs(plain[vox].astype(np.float64) - exact) / np.maximum(np.abs(exact), 1.0)
+        assert rel.max() <= ANNUAL_RTOL, rel.max()
+        sm.prepare(ctx, site[1], cell)
+        out, fb, st = sm.run(ctx, mlp, gate, sm.FALLBACK_ANNUAL, err_mlp)
+    e = np.abs(out.astype(np.float64) - plain) / np.maximum(np.abs(plain.astype(np.float64)), 1.0)
+    ratio = (t_vs + t_hr) / st["t_total_ms"]
+    print(f"8192^2 surrogate: {st['t_total_ms']:.0f} ms ({ratio:.1f}x fewer GPU-seconds), fallback "
+          f"{fb.mean():.3f}, p50 {np.percentile(e, 50):.5f} p95 {np.percentile(e, 95):.5f} max {e.max():.4f}",
+          flush=True)
+    assert np.percentile(e, 95) <= 0.02
+    assert ratio >= 5.0
+    mlp.close()
+    err_mlp.close()
