@@ -154,7 +154,7 @@ def surrogate_leg(sg, sc, az_alt, dni, dhi, cell, tuning, exact, plain_s, hours)
         c2.set_scene(sc)
         c2.set_sky(az_alt, dni, dhi)
         c2.set_tuning(tuning["dirs_per_group"], tuning["hours_per_pass"], tuning["waves_per_group"],
-                      tuning["xcd_band"], tuning["group_blocks"])
+                      tuning["xcd_band"], tuning["group_blocks"], tuning["receiver_queue"])
         sm.prepare(c2, SITE[1], cell)
         raw, _, _ = sm.run(c2, mlp, NOWHERE, sm.FALLBACK_ANNUAL)
         raw_err = rel_errors(raw, exact)
@@ -191,7 +191,7 @@ def main():
     ap.add_argument("--no-csr", action="store_true", help="skip the CSR hour-loop report")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="",
-                    help="G,K,waves,band,group: use this launch shape instead of autotuning (profiling runs)")
+                    help="G,K,waves,band,group[,queue]: use this launch shape instead of autotuning (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer drop-in timing")
     args = ap.parse_args()
@@ -230,8 +230,8 @@ def main():
 
     tuning = None
     if args.tuning:
-        G, Kt, wv, band, group = (int(x) for x in args.tuning.split(","))
-        ctx.set_tuning(G, Kt, wv, band, group)
+        vals = [int(x) for x in args.tuning.split(",")]
+        ctx.set_tuning(*vals)
         tuning = ctx.tuning()
         log(f"tuning: {tuning}")
     elif not args.no_autotune:
@@ -285,13 +285,16 @@ def main():
         extra["surrogate"] = surrogate_leg(sg, sc, az_alt, dni, dhi, cell, ctx.tuning(), exact,
                                            (t_vs + t_hr) / args.steps, H)
     copy_gbps = ctx.copy_bandwidth(1 << 30, 20)
-    valu_peak = ctx.valu_rate()
+    valu_fma, valu_mix, valu_cmp = ctx.valu_rate(0), ctx.valu_rate(1), ctx.valu_rate(2)
+    valu_peak = max(valu_fma, valu_mix, valu_cmp)
     # DDA roofline: VALU issue.  Instructions per trace launch from the committed PMC summary of the
     # same workload; the launch time live (HIP events around each chunk's trace launch).
     chunks = recs[-1][0]["viewshed_chunks"]
     avg_trace = t_tr / (chunks * args.steps)
     valu_per_launch = read_profile("trace_pool_c2_valu.txt", "valu_insts_per_launch")
     dda_roof = {"bound": "valu", "unit": "wave64 VALU instructions/s", "peak": valu_peak,
+                "peak_probes": {"fp32_fma": valu_fma, "int_add_compare_select": valu_mix,
+                                "int_add_compare_to_sgpr": valu_cmp},
                 "peak_nominal": VALU_NOMINAL_PEAK, "kernel": f"k_trace_pool<G={ctx.tuning()['dirs_per_group']}>",
                 "avg_launch_ms": avg_trace * 1e3, "launches_per_viewshed": chunks,
                 "valu_insts_per_launch": valu_per_launch,
@@ -351,7 +354,8 @@ def main():
                      "traffic_frac_of_copy": (traffic / avg_launch / 1e9 / copy_gbps) if traffic else None},
         "roofline_dda": dda_roof,
         "measured_copy_gbps": copy_gbps,
-        "measured_valu_wave_insts_per_s": valu_peak,
+        "measured_valu_wave_insts_per_s": {"fp32_fma": valu_fma, "int_add_compare_select": valu_mix,
+                                           "int_add_compare_to_sgpr": valu_cmp},
     }
     res.update(extra)
     if not args.no_cpu_baseline:

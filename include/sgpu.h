@@ -189,7 +189,8 @@ int sgpu_hour_loop(sgpu_ctx* ctx, unsigned format, int hours_per_pass, int hour_
    same results bit for bit) over the context's device-resident lists of `format`; the annual
    sums go to irradiance_out, a device buffer of num_total floats owned by the caller.  The
    per-pass scratch (num_top x hours_per_pass floats of hourly rows + num_total floats of Kahan
-   compensation, each rounded up to 256 bytes) lives in a static device array of 128 MiB per
+   compensation, each rounded up to 256 bytes, + 32 KiB of reflection queue counters) lives
+   in a static device array of 128 MiB per
    GPU, held from the first context on that GPU to the end of the process; a scene that needs
    more is refused with SGPU_ERR_ARG.  hours_per_pass 0 = the tuned value, halved until it fits.
    Calls no hipMalloc/hipFree; enqueued on the context stream (sgpu_sync waits).  Calls that
@@ -231,6 +232,11 @@ typedef struct sgpu_tuning {
     int32_t from_cache;
     double rays_per_s;           /* best measured trace rate (0 if from cache) */
     double voxel_hours_per_s;    /* best measured stage-2 rate (0 if from cache / not tuned) */
+    int32_t receiver_queue;      /* 1: the reflection waves take receivers one by one from queues per
+                                    XCD in the tile-major order (XCD bands of xcd_band workgroups'
+                                    worth of receivers; group_blocks then caps the resident
+                                    workgroups per CU, 0 = as many as fit); 0: static assignment.
+                                    Both give bit-identical annual sums. */
 } sgpu_tuning;
 int sgpu_autotune(sgpu_ctx* ctx, unsigned format, sgpu_tuning* out);
 int sgpu_set_tuning(sgpu_ctx* ctx, const sgpu_tuning* t);
@@ -238,9 +244,10 @@ int sgpu_get_tuning(sgpu_ctx* ctx, sgpu_tuning* out);
 /* Plain device copy of `bytes` (float4 loads/stores), `reps` launches; *gbps = read+write
    bytes / mean launch time.  Calibrates HBM counters and the measured copy bandwidth (new). */
 int sgpu_copy_bandwidth(sgpu_ctx* ctx, int64_t bytes, int reps, double* gbps);
-/* VALU issue rate of the device in wave64 instructions/s, measured with an FMA-only kernel of
-   known instruction count that fills every SIMD (new; the peak the DDA's VALU roofline uses). */
-int sgpu_valu_rate(sgpu_ctx* ctx, double* wave_insts_per_s);
+/* VALU issue rate of the device in wave64 instructions/s, measured with a kernel of known
+   instruction count that fills every SIMD (new; the peak the DDA's VALU roofline uses):
+   mix 0 = FP32 FMAs only, mix 1 = the DDA's own classes (integer add, compare, select). */
+int sgpu_valu_rate(sgpu_ctx* ctx, int mix, double* wave_insts_per_s);
 
 /* ---- per-voxel surrogate MLP (new; no reference counterpart) ----
    64 inputs (53 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and

@@ -72,7 +72,8 @@ class Tuning(ctypes.Structure):
     _fields_ = [("dirs_per_group", ctypes.c_int32), ("hours_per_pass", ctypes.c_int32),
                 ("waves_per_group", ctypes.c_int32), ("xcd_band", ctypes.c_int32), ("group_blocks", ctypes.c_int32),
                 ("from_cache", ctypes.c_int32),
-                ("rays_per_s", ctypes.c_double), ("voxel_hours_per_s", ctypes.c_double)]
+                ("rays_per_s", ctypes.c_double), ("voxel_hours_per_s", ctypes.c_double),
+                ("receiver_queue", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -449,8 +450,9 @@ class Context:
         return t.as_dict()
 
     def set_tuning(self, dirs_per_group: int = 16, hours_per_pass: int = 64, waves_per_group: int = 4,
-                   xcd_band: int = 0, group_blocks: int = 0) -> None:
-        t = Tuning(dirs_per_group, hours_per_pass, waves_per_group, xcd_band, group_blocks, 0, 0.0, 0.0)
+                   xcd_band: int = 0, group_blocks: int = 0, receiver_queue: int = 0) -> None:
+        t = Tuning(dirs_per_group, hours_per_pass, waves_per_group, xcd_band, group_blocks, 0, 0.0, 0.0,
+                   receiver_queue)
         _check(_lib.sgpu_set_tuning(self._h, ctypes.byref(t)))
 
     def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:
@@ -459,10 +461,11 @@ class Context:
         _check(_lib.sgpu_copy_bandwidth(self._h, _I64(nbytes), reps, ctypes.byref(g)))
         return g.value
 
-    def valu_rate(self) -> float:
-        """Measured VALU issue rate, wave64 instructions/s (FMA-only probe filling every SIMD)."""
+    def valu_rate(self, mix: int = 0) -> float:
+        """Measured VALU issue rate, wave64 instructions/s, of a probe filling every SIMD:
+        mix 0 = FP32 FMAs, mix 1 = integer add / compare / select (the DDA's classes)."""
         r = ctypes.c_double()
-        _check(_lib.sgpu_valu_rate(self._h, ctypes.byref(r)))
+        _check(_lib.sgpu_valu_rate(self._h, mix, ctypes.byref(r)))
         return r.value
 
 

@@ -90,7 +90,9 @@ __global__ __launch_bounds__(kSolarBlock) void k_solar_top(HourArgs a) {
 
 struct Recv {
     float gx, gy, z, nx, ny, nz;
+    float cs, ss, ca, sa;   // cos/sin of slope and aspect (the receiver's own hourly term)
 };
+
 
 /* Division of a cell id by the grid width via a float reciprocal and one fix-up: exact on
  * every supported grid (grid_supported: at most 2^22 rows, ids below 2^30), where the float
@@ -102,6 +104,23 @@ __device__ __forceinline__ void cell_xy(int id, int W, float inv_w, int& x, int&
     else if (r >= W) { ++q; r -= W; }
     x = r;
     y = q;
+}
+
+/* The receiver geometry of voxel v (reflection_kernels.cu:55-63): cell, height, normal. */
+__device__ __forceinline__ Recv receiver_of(const HourArgs& a, int64_t v, float inv_w) {
+    Recv r;
+    const int top = a.pix_parent[v];
+    int gx, gy;
+    cell_xy(top, a.W, inv_w, gx, gy);
+    r.gx = (float)gx;
+    r.gy = (float)gy;
+    r.z = (v < a.n_top) ? a.grid[top] : a.wall_z[v - a.n_top];
+    sincosf(a.slopes[v], &r.ss, &r.cs);
+    sincosf(a.aspects[v], &r.sa, &r.ca);
+    r.nx = r.ss * r.sa;
+    r.ny = -r.ss * r.ca;
+    r.nz = r.cs;
+    return r;
 }
 
 /* Hour-invariant weight of neighbour cell `id` (0-based) for receiver r, 0 when the pair is
@@ -169,22 +188,23 @@ __device__ __forceinline__ int64_t uniform64(int64_t x) {
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-/* K <= 64: lane (g, h) = (lane / K, lane % K) adds w[i] * top[id[i]][h] over the list
- * entries i = g, g + 64/K, ...: each load instruction reads 64/K whole K-hour rows.
- * K = 128, 256: lane l holds hours HPL l .. HPL l + HPL - 1 of every entry (HPL = K / 64:
- * one 8- or 16-byte load per lane, one whole row per instruction).  The lane's partial
- * sums over its hours are returned.  OFF32 (the hourly buffer is below 4 GiB): rows are
- * addressed by a 32-bit byte offset from the buffer base, one shift-add per entry. */
+/* Each lane holds HPL consecutive hours of a row: K >= 16, one 16-byte load of 4 hours, LPE = K / 4
+ * lanes covering a row and G = 64 / LPE rows per load instruction (K = 256: 1, K = 128: 2,
+ * K = 64: 4); K < 16, one hour per lane, G = 64 / K rows.  Lane l adds w[i] * top[id[i]][its
+ * hours] over the list entries i = l / LPE, l / LPE + G, ...; its partial sums over its hours
+ * are returned.  OFF32 (the hourly buffer is below 4 GiB): rows are addressed by a 32-bit byte
+ * offset from the buffer base, one shift-add per entry. */
 template <int K, bool OFF32>
 __device__ __forceinline__ float apply_list_at(const int* ids, const float* ws, int cnt,
                                                const float* __restrict__ top, int lane) {
-    constexpr int HPL = (K > 64) ? K / 64 : 1;
-    constexpr int G = (K > 64) ? 1 : 64 / K;
+    constexpr int HPL = (K >= 16) ? 4 : 1;
+    constexpr int LPE = K / HPL;
+    constexpr int G = 64 / LPE;
     typedef float vec __attribute__((ext_vector_type(HPL)));
-    /* 16 VGPRs of rows in flight per wave: 16, 8 or 4 loads (measured best for K = 64, 128, 256) */
+    /* 16 VGPRs of rows in flight per wave: 4 loads of 16 bytes (K >= 16) or 16 of 4 bytes */
     constexpr int kUnroll = 8 / HPL;
     /* byte offset of this lane's hours within a row, and of row `id` */
-    const unsigned lane_off = (K > 64) ? (unsigned)(HPL * lane) * 4u : (unsigned)(lane & (K - 1)) * 4u;
+    const unsigned lane_off = (unsigned)((lane % LPE) * HPL) * 4u;
     const char* lane_top = reinterpret_cast<const char*>(top) + lane_off;   // 64-bit rows: one shift-add each
     constexpr int kRowShift = __builtin_ctz(K * 4);
     auto load = [&](int id) __attribute__((always_inline)) {
@@ -196,7 +216,7 @@ __device__ __forceinline__ float apply_list_at(const int* ids, const float* ws, 
         }
     };
     vec a = {}, b = {};
-    int i = (K > 64) ? 0 : lane / K;
+    int i = lane / LPE;
     if constexpr (!OFF32) {
         /* 64-bit rows: the batch's loads issued before its FMAs (the scheduler otherwise pairs
            them), entries still alternating between a and b in list order */
@@ -242,7 +262,7 @@ __device__ __forceinline__ float apply_list_at(const int* ids, const float* ws, 
 template <int K>
 __device__ __forceinline__ float apply_list(const int* ids, const float* ws, int cnt, const float* __restrict__ top,
                                             int lane, bool off32) {
-    static_assert(K <= 64 || K == 128 || K == 256, "one, two or four hours per lane");
+    static_assert(K >= 1 && K <= 256 && (K & (K - 1)) == 0, "a power of two hours per pass");
     return off32 ? apply_list_at<K, true>(ids, ws, cnt, top, lane) : apply_list_at<K, false>(ids, ws, cnt, top, lane);
 }
 
@@ -357,20 +377,131 @@ __device__ __forceinline__ void wave_decode(StreamWindow& win, int64_t s0, int64
     }
 }
 
+/* One wave's LDS: the stream window, the compacted (id, weight) list, decoded ids not yet
+ * weighed. */
+template <int FMT, int WAVES, int STAGE = kStageBytes>
+struct WaveLds {
+    static constexpr int kCap = (FMT == SGPU_FMT_BARLE) ? STAGE / WAVES : 16;
+    int list_id[WAVES][kListCap];
+    float list_w[WAVES][kListCap];
+    __align__(16) uint8_t stage[WAVES][kCap];
+    int pend_id[WAVES][FMT == SGPU_FMT_BARLE ? 128 : 1];
+};
+
+/* One receiver's pass, by the whole wave: lane l's part of the receiver's own irradiance (hours
+ * l, l + 64, ... of the pass) and of the reflected irradiance summed over the pass's hours.
+ * Shared by the static and the queued kernels, so every receiver's total does not depend on
+ * which wave takes it (bit-identical annual sums under every receiver assignment). */
+template <int K, int FMT, int WAVES, int STAGE>
+__device__ __forceinline__ void receiver_parts(const HourArgs& a, const HourConst* hours, const Recv& r, int64_t v,
+                                               int64_t s0, int64_t s1, StreamWindow& win, WaveLds<FMT, WAVES, STAGE>& L,
+                                               int wave, int lane, bool off32, float inv_w, int cells, float& own,
+                                               float& refl) {
+    own = 0.0f;
+    for (int h = lane; h < K && h < a.nh; h += 64) {
+        const HourConst& hc = hours[h];
+        own += hour_irradiance(hc, r.cs, r.ss, r.ca, r.sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
+    }
+    float acc = 0.0f;
+    int cnt = 0;
+    auto take = [&](bool valid, int id) __attribute__((always_inline)) {
+        const float w = valid ? neighbour_weight(id, r, a.grid, a.W, cells, inv_w) : 0.0f;
+        const bool keep = w > 0.0f;
+        const uint64_t m = __ballot(keep);
+        if (keep) {
+            const int at = cnt + lanes_below(m);
+            L.list_id[wave][at] = id;
+            L.list_w[wave][at] = w;
+        }
+        cnt += __popcll(m);
+        if (cnt > kListCap - 64) {
+            wave_lds_sync();
+            acc += apply_list<K>(L.list_id[wave], L.list_w[wave], cnt, a.hourly_top, lane, off32);
+            wave_lds_sync();
+            cnt = 0;
+        }
+    };
+    if constexpr (FMT == SGPU_FMT_CSR) {
+        for (int64_t b = s0; b < s1; b += 64) {
+            const int64_t k = b + lane;
+            const bool valid = k < s1;
+            take(valid, valid ? a.nb_ids[k] - 1 : -1);
+        }
+    } else {
+        /* a ba-RLE block holds ~30 ids: decoded ids wait in pend_id so that the weights are
+           computed on full 64-id chunks */
+        int pend = 0;
+        auto gather = [&](bool valid, int id) __attribute__((always_inline)) {
+            const uint64_t m = __ballot(valid);
+            if (valid) L.pend_id[wave][pend + lanes_below(m)] = id;
+            pend += __popcll(m);
+            if (pend >= 64) {
+                wave_lds_sync();
+                const int x = L.pend_id[wave][lane];
+                const bool more = 64 + lane < pend;
+                const int y = more ? L.pend_id[wave][64 + lane] : 0;
+                wave_lds_sync();
+                if (more) L.pend_id[wave][lane] = y;
+                pend -= 64;
+                take(true, x);
+            }
+        };
+        wave_decode(win, s0, s1, lane, gather);
+        if (pend > 0) {
+            wave_lds_sync();
+            const bool valid = lane < pend;
+            take(valid, valid ? L.pend_id[wave][lane] : -1);
+        }
+    }
+    if (cnt > 0) {
+        wave_lds_sync();
+        acc += apply_list<K>(L.list_id[wave], L.list_w[wave], cnt, a.hourly_top, lane, off32);
+        wave_lds_sync();
+    }
+    refl = acc;
+}
+
+/* The pass's hour constants (nh <= K of them, 48 bytes each) copied into LDS by the workgroup;
+   the caller synchronises before use. */
+template <int K>
+__device__ __forceinline__ void stage_hours(const HourArgs& a, HourConst* lds) {
+    static_assert(sizeof(HourConst) == 48, "three 16-byte pieces per hour");
+    const int pieces = min(a.nh, K) * 3;
+    const float4* src = reinterpret_cast<const float4*>(a.hours);
+    float4* dst = reinterpret_cast<float4*>(lds);
+    for (int i = threadIdx.x; i < pieces; i += blockDim.x) dst[i] = src[i];
+}
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
+    return x;
+}
+
+/* total += own + reflected of the pass, FP32 Kahan (the annual accumulator). */
+__device__ __forceinline__ void kahan_add(float* sum, float* comp, int64_t q, float x) {
+    float s = sum[q], c = comp[q];
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+    sum[q] = s;
+    comp[q] = c;
+}
+
 /* WAVES waves per workgroup (2 or 4): the knob the autotuner picks.
+ * Static assignment: each wave takes 64 receivers (receiver groups interleave them, see
+ * HourArgs::group) and walks them one after another.
  * ba-RLE: the 64 voxels of a wave own one contiguous stretch of the stream; it passes through
  * an LDS window of kStageBytes / WAVES bytes filled with 16-byte loads, block by block.
  * DBG (K = 1 only): write the hour's own and reflected irradiance instead of accumulating. */
 template <int K, int FMT, bool DBG, int WAVES>
 __device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
-    constexpr int kCap = (FMT == SGPU_FMT_BARLE) ? kStageBytes / WAVES : 16;
     __shared__ Recv rec[WAVES][64];
     __shared__ int64_t beg[WAVES][64];   // each receiver's list [beg, fin)
     __shared__ int64_t fin[WAVES][64];
-    __shared__ int list_id[WAVES][kListCap];
-    __shared__ float list_w[WAVES][kListCap];
-    __shared__ __align__(16) uint8_t stage[WAVES][kCap];
-    __shared__ int pend_id[WAVES][FMT == SGPU_FMT_BARLE ? 128 : 1];   // decoded ids not yet weighed
+    __shared__ WaveLds<FMT, WAVES> L;
+    __shared__ HourConst hrs[K];
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t n = a.n_sub > 0 ? a.n_sub : (a.subset ? 0 : a.n_total);
@@ -403,126 +534,124 @@ __device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
     const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);   // below 4 GiB: 32-bit row offsets
 
     /* phase 1: lane = voxel */
-    float own = 0.0f;
+    stage_hours<K>(a, hrs);
     {
         Recv r{};
         if (live) {
-            const int top = a.pix_parent[v];
-            int gx, gy;
-            cell_xy(top, a.W, inv_w, gx, gy);
-            r.gx = (float)gx;
-            r.gy = (float)gy;
-            r.z = (v < a.n_top) ? a.grid[top] : a.wall_z[v - a.n_top];
-            float ss, cs, sa, ca;
-            sincosf(a.slopes[v], &ss, &cs);
-            sincosf(a.aspects[v], &sa, &ca);
-            r.nx = ss * sa;
-            r.ny = -ss * ca;
-            r.nz = cs;
+            r = receiver_of(a, v, inv_w);
             beg[wave][lane] = a.nb_off[q];
             fin[wave][lane] = a.nb_off[q + 1];
-            for (int h = 0; h < K; ++h) {
-                if (h < a.nh) {
-                    const HourConst& hc = a.hours[h];
-                    own += hour_irradiance(hc, cs, ss, ca, sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
-                }
-            }
         }
         rec[wave][lane] = r;
     }
     __syncthreads();
 
     /* phase 2: one receiver at a time */
-    float refl = 0.0f;   // lane j: receiver j's reflected irradiance summed over the pass
+    float total = 0.0f, own_j = 0.0f, refl_j = 0.0f;   // lane j: receiver j's sums over the pass
     /* consecutive receivers (S == 1) mostly have neighbouring streams: the window runs ahead
        over the buffer; interleaved receivers (S > 1) each stage only their own list */
     const int64_t wend = (S == 1) ? a.nb_len : -1;
-    StreamWindow win{stage[wave], a.nb_bytes, 0, 0, 0, kCap};
+    StreamWindow win{L.stage[wave], a.nb_bytes, 0, 0, 0, WaveLds<FMT, WAVES>::kCap};
     for (int j = 0; j < nv; ++j) {
         const Recv r = rec[wave][j];
+        const int64_t vj = __shfl(v, j, 64);
         const int64_t s0 = uniform64(beg[wave][j]), s1 = uniform64(fin[wave][j]);
         win.end = wend >= 0 ? wend : s1;
-        float acc = 0.0f;
-        int cnt = 0;
-        auto take = [&](bool valid, int id) __attribute__((always_inline)) {
-            const float w = valid ? neighbour_weight(id, r, a.grid, a.W, cells, inv_w) : 0.0f;
-            const bool keep = w > 0.0f;
-            const uint64_t m = __ballot(keep);
-            if (keep) {
-                const int at = cnt + lanes_below(m);
-                list_id[wave][at] = id;
-                list_w[wave][at] = w;
-            }
-            cnt += __popcll(m);
-            if (cnt > kListCap - 64) {
-                wave_lds_sync();
-                acc += apply_list<K>(list_id[wave], list_w[wave], cnt, a.hourly_top, lane, off32);
-                wave_lds_sync();
-                cnt = 0;
-            }
-        };
-        if constexpr (FMT == SGPU_FMT_CSR) {
-            for (int64_t b = s0; b < s1; b += 64) {
-                const int64_t k = b + lane;
-                const bool valid = k < s1;
-                take(valid, valid ? a.nb_ids[k] - 1 : -1);
-            }
+        float own, refl;
+        receiver_parts<K, FMT, WAVES, kStageBytes>(a, hrs, r, vj, s0, s1, win, L, wave, lane, off32, inv_w, cells, own,
+                                                   refl);
+        if constexpr (DBG) {
+            own = wave_sum(own);
+            refl = wave_sum(refl);
+            if (lane == j) { own_j = own; refl_j = refl; }
         } else {
-            /* a ba-RLE block holds ~30 ids: decoded ids wait in pend_id so that the weights are
-               computed on full 64-id chunks */
-            int pend = 0;
-            auto gather = [&](bool valid, int id) __attribute__((always_inline)) {
-                const uint64_t m = __ballot(valid);
-                if (valid) pend_id[wave][pend + lanes_below(m)] = id;
-                pend += __popcll(m);
-                if (pend >= 64) {
-                    wave_lds_sync();
-                    const int x = pend_id[wave][lane];
-                    const bool more = 64 + lane < pend;
-                    const int y = more ? pend_id[wave][64 + lane] : 0;
-                    wave_lds_sync();
-                    if (more) pend_id[wave][lane] = y;
-                    pend -= 64;
-                    take(true, x);
-                }
-            };
-            wave_decode(win, s0, s1, lane, gather);
-            if (pend > 0) {
-                wave_lds_sync();
-                const bool v = lane < pend;
-                take(v, v ? pend_id[wave][lane] : -1);
-            }
+            const float t = wave_sum(own + refl);
+            if (lane == j) total = t;
         }
-        if (cnt > 0) {
-            wave_lds_sync();
-            acc += apply_list<K>(list_id[wave], list_w[wave], cnt, a.hourly_top, lane, off32);
-            wave_lds_sync();
-        }
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
-        if (lane == j) refl = acc;
     }
 
     /* phase 3: lane = voxel */
     if (!live) return;
     if constexpr (DBG) {
-        a.dbg_hourly[q] = own;
-        a.dbg_reflected[q] = refl;
+        a.dbg_hourly[q] = own_j;
+        a.dbg_reflected[q] = refl_j;
         return;
     }
-    float s = a.sum[q], c = a.comp[q];
-    const float x = refl + own;
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-    a.sum[q] = s;
-    a.comp[q] = c;
+    kahan_add(a.sum, a.comp, q, total);
 }
 
 template <int K, int FMT, bool DBG, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_reflect(HourArgs a) {
     reflect_block<K, FMT, DBG, WAVES>(a, xcd_block(blockIdx.x, gridDim.x, (unsigned)a.band));
+}
+
+/* Position of the item-th receiver XCD x takes from its queue: XCD x walks bands x, x + 8,
+ * x + 16, ... of `band` consecutive positions of the processing order (band == 0: one contiguous
+ * eighth of the positions per XCD). */
+__device__ __forceinline__ int64_t queue_position(uint64_t item, unsigned x, int64_t n, int64_t band) {
+    if (band <= 0) {
+        const int64_t per = (n + kXcds - 1) / kXcds;
+        return item < (uint64_t)per ? (int64_t)x * per + (int64_t)item : n;
+    }
+    const int64_t k = (int64_t)(item / (uint64_t)band), o = (int64_t)(item % (uint64_t)band);
+    return (k * kXcds + (int64_t)x) * band + o;
+}
+
+/* Queued assignment (full-scene order mode): a persistent grid whose waves each take the next
+ * receiver from a queue of their XCD (block b is on XCD b % 8), so that at any moment the
+ * resident waves of an XCD work on a compact window of the processing order and share its
+ * neighbours' rows in the L2.  An XCD's positions are split over kQueuesPerXcd interleaved
+ * sub-queues (item i of sub-queue c is the XCD's (i * kQueuesPerXcd + c)-th position), each a
+ * counter on its own 256-byte line zeroed before the launch, wave w of the XCD pulling from
+ * sub-queue w % kQueuesPerXcd: a single counter per XCD serialises its atomics.  A wave asks for
+ * its next item before it works on the current one.  Positions grow with the item, so a wave
+ * leaves at its first position past the end: every wave reaches the exit, whatever the grid. */
+/* The queued kernel's stream window: a receiver's own stream only (~300 bytes on average), 1 KB
+   per wave; blocks larger than that are read from global memory. */
+constexpr int kQueueStageBytes = 4096;
+
+template <int K, int FMT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(8)))
+void k_reflect_queue(HourArgs a) {
+    __shared__ WaveLds<FMT, WAVES, kQueueStageBytes> L;
+    __shared__ HourConst hrs[K];
+    stage_hours<K>(a, hrs);
+    __syncthreads();
+    /* the wave index as a wave-uniform (scalar) value: the counter address must be scalar for the
+       compiler to combine the lanes' atomic into one */
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const unsigned x = blockIdx.x % kXcds;
+    const unsigned c = ((blockIdx.x / kXcds) * WAVES + (unsigned)wave) % kQueuesPerXcd;
+    unsigned* counter = a.queue + (x * kQueuesPerXcd + c) * kQueueStride;
+    const int64_t n = a.n_total;
+    const float inv_w = 1.0f / (float)a.W;
+    const int cells = a.W * a.H;
+    const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);
+    StreamWindow win{L.stage[wave], a.nb_bytes, 0, 0, 0, WaveLds<FMT, WAVES, kQueueStageBytes>::kCap};
+    /* every lane takes part in the atomic (only lane 0 adds), so no lane-divergent branch
+       precedes the broadcast: lane 0's old value is the wave's item.  (With the atomic inside
+       `if (lane == 0)`, the compiler split the loop so that lanes 1..63 re-entered the body with
+       a stale item and never left.) */
+    unsigned pending = atomicAdd(counter, lane == 0 ? 1u : 0u);
+    for (int64_t turn = 0; turn <= n; ++turn) {   // each turn takes a new item: at most n + 1
+        const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)pending);
+        const int64_t p = queue_position((uint64_t)item * kQueuesPerXcd + c, x, n, a.queue_band);
+        if (p >= n) break;
+        pending = atomicAdd(counter, lane == 0 ? 1u : 0u);   // the next item, while this one is worked
+        const int64_t v = a.order[p];
+        const Recv r = receiver_of(a, v, inv_w);
+        const int64_t s0 = uniform64(a.nb_off[v]), s1 = uniform64(a.nb_off[v + 1]);
+        win.end = s1;
+        float own, refl;
+        receiver_parts<K, FMT, WAVES, kQueueStageBytes>(a, hrs, r, v, s0, s1, win, L, wave, lane, off32, inv_w, cells,
+                                                        own, refl);
+        const float t = wave_sum(own + refl);
+        if (lane == 0) kahan_add(a.sum, a.comp, v, t);
+    }
+}
+
+__global__ void k_clear_queue(unsigned* q) {
+    if (threadIdx.x < kXcds * kQueuesPerXcd) q[threadIdx.x * kQueueStride] = 0u;
 }
 
 /* ---- batched tiles: one launch per pass covers every tile of a batch ---- */
@@ -612,8 +741,51 @@ void batch_k(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks,
     }
 }
 
+/* The persistent grid of the queued kernel: the workgroups every CU holds at once (waves take
+   work from the queues, so more would only start after the work is gone), a multiple of the XCD
+   count. */
+struct GridShape {
+    int cus, per_cu;
+};
+template <typename Kernel>
+GridShape occupancy(Kernel kernel, int threads) {
+    int dev = 0, cus = 0, per_cu = 0;
+    SGPU_HIP(hipGetDevice(&dev));
+    SGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 2;
+    return {std::max(cus, 1), per_cu};
+}
+/* blocks_per_cu > 0 caps the resident workgroups per CU: fewer waves in flight keep the
+   XCD's neighbour rows within its L2 (a knob of the autotuner) */
+unsigned queue_grid(GridShape g, int blocks_per_cu) {
+    const int per = blocks_per_cu > 0 ? std::min(blocks_per_cu, g.per_cu) : g.per_cu;
+    const unsigned n = (unsigned)(g.cus * per);
+    return (n + kXcds - 1) / kXcds * kXcds;
+}
+
+template <int K, int WAVES>
+void reflect_queue_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(kXcds * kQueuesPerXcd), 0, s, a.queue);
+    if (fmt == SGPU_FMT_CSR) {
+        static const GridShape g = occupancy(k_reflect_queue<K, SGPU_FMT_CSR, WAVES>, 64 * WAVES);
+        hipLaunchKernelGGL((k_reflect_queue<K, SGPU_FMT_CSR, WAVES>), dim3(queue_grid(g, a.queue_blocks_per_cu)),
+                           dim3(64 * WAVES), 0, s, a);
+    } else {
+        static const GridShape g = occupancy(k_reflect_queue<K, SGPU_FMT_BARLE, WAVES>, 64 * WAVES);
+        hipLaunchKernelGGL((k_reflect_queue<K, SGPU_FMT_BARLE, WAVES>), dim3(queue_grid(g, a.queue_blocks_per_cu)),
+                           dim3(64 * WAVES), 0, s, a);
+    }
+}
+
 template <int K>
 void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
+    const bool queued = a.queue != nullptr && a.order != nullptr && a.subset == nullptr && a.n_sub == 0;
+    if (queued) {
+        if (a.waves == 2) reflect_queue_kw<K, 2>(a, fmt, s);
+        else reflect_queue_kw<K, 4>(a, fmt, s);
+        return;
+    }
     if (a.waves == 2) reflect_kw<K, false, 2>(a, fmt, s);
     else reflect_kw<K, false, 4>(a, fmt, s);
 }

@@ -63,6 +63,64 @@ __global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int
     out[(int64_t)blockIdx.x * 256 + threadIdx.x] = sum;
 }
 
+/* The DDA's instruction classes (integer adds, compares into the lane mask, selects): per chain
+ * and step y = (x += a) < t ? x : y, which compiles to one v_add_u32, one v_cmp_gt_u32 and one
+ * v_cndmask_b32 (the gfx950 assembly of this build has exactly 384 VALU instructions in the loop
+ * body: eight chains, 16 steps); the loop's own counter and branch are scalar. */
+constexpr int kProbeMixPerTurn = 8 * 16 * 3;
+__global__ __launch_bounds__(256) void k_valu_probe_mix(unsigned* __restrict__ out, int turns, unsigned a,
+                                                        unsigned t) {
+    unsigned x[8], y[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        x[k] = threadIdx.x + (unsigned)k;
+        y[k] = threadIdx.x ^ (unsigned)(k * 977);
+    }
+    for (int r = 0; r < turns; ++r) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                x[k] += a;
+                y[k] = (x[k] < t) ? x[k] : y[k];
+            }
+        }
+    }
+    unsigned sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum += x[k] ^ y[k];
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
+/* Compares into SGPR lane masks, OR-ed by the scalar unit (the DDA's exit tests): per chain and
+ * step x += a (v_add_u32) and m |= (x < t) (v_cmp_gt_u32_e64 into an SGPR pair, s_or_b64), with
+ * every chain's mask live, so the compares cannot share VCC; 256 VALU instructions per turn. */
+constexpr int kProbeCmpPerTurn = 8 * 16 * 2;
+__global__ __launch_bounds__(256) void k_valu_probe_cmp(unsigned* __restrict__ out, int turns, unsigned a,
+                                                        unsigned t) {
+    unsigned x[8];
+    uint64_t m[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        x[k] = threadIdx.x * 2654435761u + (unsigned)k;
+        m[k] = 0;
+    }
+    for (int r = 0; r < turns; ++r) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                x[k] += a;
+                m[k] |= __ballot(x[k] < t);
+            }
+        }
+    }
+    uint64_t any = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) any ^= m[k];
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = x[0] ^ (unsigned)any ^ (unsigned)(any >> 32);
+}
+
 int64_t choose_chunk(const sgpu_ctx* c, int64_t n) {
     if (n <= 0) return 1;
     if (c->chunk_override > 0) return std::min<int64_t>(c->chunk_override, n);
@@ -104,6 +162,7 @@ void ensure_hours_d(sgpu_ctx* c, const std::vector<HourConst>& hs) {
 void ensure_accumulators(sgpu_ctx* c, int K) {
     if (!c->sum) c->sum = dalloc<float>(c->n_total);
     if (!c->comp) c->comp = dalloc<float>(c->n_total);
+    if (!c->reflect_queue) c->reflect_queue = dalloc<unsigned>(kQueueWords);
     const size_t need_elems = (size_t)c->n_top * K;
     if (need_elems > c->hourly_top_elems) {
         dfree(c->hourly_top);
@@ -137,6 +196,11 @@ HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
     a.band = c->tune.xcd_band;
     a.order = c->order;   // subset and streamed callers clear it
     a.group = c->tune.group_blocks;
+    if (c->tune.receiver_queue && c->reflect_queue) {
+        a.queue = c->reflect_queue;
+        a.queue_band = (int64_t)c->tune.xcd_band * 64 * (c->tune.waves_per_group == 2 ? 2 : 4);
+        a.queue_blocks_per_cu = c->tune.group_blocks;   // group_blocks in queue mode: workgroups per CU
+    }
     if (fmt == SGPU_FMT_CSR) {
         a.nb_off = c->csr_off;
         a.nb_ids = c->csr_ids;
@@ -1504,9 +1568,10 @@ int sgpu_copy_bandwidth(sgpu_ctx* c, int64_t bytes, int reps, double* gbps) {
     });
 }
 
-int sgpu_valu_rate(sgpu_ctx* c, double* wave_insts_per_s) {
+int sgpu_valu_rate(sgpu_ctx* c, int mix, double* wave_insts_per_s) {
     return guarded([&] {
         need(c && wave_insts_per_s, SGPU_ERR_ARG, "null argument");
+        need(mix >= 0 && mix <= 2, SGPU_ERR_ARG, "mix must be 0 (FMA), 1 (add/compare/select) or 2 (compare to SGPR)");
         c->set_device();
         int cus = 0;
         SGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -1514,15 +1579,25 @@ int sgpu_valu_rate(sgpu_ctx* c, double* wave_insts_per_s) {
         const int turns = 4000, reps = 5;
         float* out = dalloc<float>((size_t)blocks * 256);
         try {
-            hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, c->stream, out, turns, 0.999f, 1e-3f);
+            auto launch = [&] {
+                if (mix == 0)
+                    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, c->stream, out, turns, 0.999f, 1e-3f);
+                else if (mix == 1)
+                    hipLaunchKernelGGL(k_valu_probe_mix, dim3(blocks), dim3(256), 0, c->stream,
+                                       reinterpret_cast<unsigned*>(out), turns, 7u, 0x9E3779B9u);
+                else
+                    hipLaunchKernelGGL(k_valu_probe_cmp, dim3(blocks), dim3(256), 0, c->stream,
+                                       reinterpret_cast<unsigned*>(out), turns, 0x9E3779B9u, 0x7FFFFFFFu);
+            };
+            launch();
             SGPU_HIP(hipGetLastError());
             EventPair ev;
             SGPU_HIP(hipEventRecord(ev.a, c->stream));
-            for (int r = 0; r < reps; ++r)
-                hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, c->stream, out, turns, 0.999f, 1e-3f);
+            for (int r = 0; r < reps; ++r) launch();
             SGPU_HIP(hipEventRecord(ev.b, c->stream));
             SGPU_HIP(hipStreamSynchronize(c->stream));
-            const double insts = (double)blocks * 4.0 * turns * kProbeFmaPerTurn * reps;
+            const double per_turn = mix == 0 ? kProbeFmaPerTurn : mix == 1 ? kProbeMixPerTurn : kProbeCmpPerTurn;
+            const double insts = (double)blocks * 4.0 * turns * per_turn * reps;
             *wave_insts_per_s = insts / (ev.ms() * 1e-3);
         } catch (...) {
             dfree(out);

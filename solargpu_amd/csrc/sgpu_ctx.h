@@ -177,7 +177,10 @@ struct sgpu_ctx {
     bool count_steps = false;
     sgpu_info info{};
     sgpu::SurrogateState sur;
-    sgpu_tuning tune{16, 64, 4, 0, 0, 0, 0.0, 0.0};   // trace group, hours/pass, waves, XCD band, group; from_cache
+    /* trace group, hours/pass, waves, XCD band, group; from_cache, rates, receiver queue: the
+       shape the autotuner picks on the 2048^2 scene, until a scene of its own is tuned */
+    sgpu_tuning tune{32, 128, 4, 32, 0, 0, 0.0, 0.0, 1};
+    unsigned* reflect_queue = nullptr;                   // kXcds counters of the queued reflection
     int residency = SGPU_LISTS_DEVICE;          // where sgpu_viewshed leaves the lists
     sgpu::HostLists host;                       // the lists when host-resident
     hipStream_t copy_stream = nullptr;          // H2D tile copies of the streamed hour loop
@@ -217,6 +220,7 @@ struct sgpu_ctx {
         sgpu::dfree(sum);
         sgpu::dfree(comp);
         sgpu::dfree(hourly_top);
+        sgpu::dfree(reflect_queue);
         hourly_top_elems = 0;
         scene = false;
     }

@@ -246,7 +246,17 @@ struct HourArgs {
        position p is voxel order[p], and voxel ids index nb_off, sum, comp and dbg_*. */
     const int32_t* order;
     int group;
+    /* Queued assignment (full-scene order mode, no subset): when queue is set, a persistent
+       grid takes the receivers one by one from kQueuesPerXcd counters per XCD (kQueueWords
+       unsigned, zeroed by the launcher), XCD x walking bands x, x + 8, ... of queue_band
+       positions (0 = one contiguous eighth per XCD). */
+    unsigned* queue;
+    int64_t queue_band;
+    int queue_blocks_per_cu;   // resident workgroups per CU of the queued grid (0 = as many as fit)
 };
+constexpr int kQueuesPerXcd = 16;
+constexpr int kQueueStride = 64;                       // unsigned per counter: a 256-byte line each
+constexpr int kQueueWords = 8 * kQueuesPerXcd * kQueueStride;   // 8 XCDs: 32 KB
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s);
 
 /* Batched tiles in stage 2: one k_solar_top and one k_reflect launch per pass cover every

@@ -2,15 +2,16 @@
  * allocate device memory while they run -- code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc -- sgpu_hour_loop_noalloc runs the same kernels
  * as sgpu_hour_loop (k_solar_top, k_reflect), with the per-pass hourly rows of the top
- * cells and the Kahan compensation in a file-scope device array instead of context
- * buffers, and writes the annual sums into the caller's device buffer.
+ * cells, the Kahan compensation and the reflection queue counters in a file-scope device
+ * array instead of context buffers, and writes the annual sums into the caller's device
+ * buffer.
  *
  * The array takes kScratchBytes of every GPU the library runs on until the process ends;
  * its address on a device is looked up once, when the first context of that device is
  * created, and kept per device.  Every call on a GPU shares that GPU's array: calls that
  * overlap on one GPU (two contexts on two streams, say) race.  What must start at zero
- * (the compensation) is cleared by a fill kernel at the start of every call; the hourly
- * rows are rewritten by every pass before they are read. */
+ * (the compensation, the queue counters) is cleared by fill kernels at the start of every
+ * call or pass; the hourly rows are rewritten by every pass before they are read. */
 #include <mutex>
 #include <vector>
 
@@ -41,9 +42,11 @@ void clear_floats(float* p, int64_t n, hipStream_t s) {
 
 size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-/* hourly rows (n_top x K floats), then the compensation (n_total floats), 256-byte aligned */
+/* hourly rows (n_top x K floats), then the compensation (n_total floats), then the queue
+   counters of the reflection pass, each 256-byte aligned */
+constexpr size_t kQueueBytes = (size_t)kQueueWords * sizeof(unsigned);
 size_t scratch_bytes(int64_t n_top, int64_t n_total, int K) {
-    return round256((size_t)n_top * K * sizeof(float)) + round256((size_t)n_total * sizeof(float));
+    return round256((size_t)n_top * K * sizeof(float)) + round256((size_t)n_total * sizeof(float)) + kQueueBytes;
 }
 
 }  // namespace
@@ -92,6 +95,12 @@ int sgpu_hour_loop_noalloc(sgpu_ctx* c, unsigned fmt, int K, float* out) {
         a.hourly_top_cap = c->n_top * (int64_t)K;
         a.comp = reinterpret_cast<float*>(base + round256((size_t)c->n_top * K * sizeof(float)));
         a.sum = out;
+        a.queue = nullptr;
+        if (c->tune.receiver_queue) {   // the launcher zeroes the counters with a kernel
+            a.queue = reinterpret_cast<unsigned*>(base + want - kQueueBytes);
+            a.queue_band = (int64_t)c->tune.xcd_band * 64 * (c->tune.waves_per_group == 2 ? 2 : 4);
+            a.queue_blocks_per_cu = c->tune.group_blocks;
+        }
         clear_floats(a.sum, c->n_total, c->stream);
         clear_floats(a.comp, c->n_total, c->stream);
         const int nsel = (int)c->hours_all_n;

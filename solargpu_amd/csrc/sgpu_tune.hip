@@ -6,10 +6,11 @@
  *   stage-2 pass   hours per pass K in {64, 128, 256} x waves per reflection
  *                  workgroup in {2, 4} (LDS footprint and occupancy) x XCD
  *                  band in {0, 64, 256} (contiguous eighths or side-by-side
- *                  bands), then for the best K: waves {2, 4} x receiver groups
- *                  {8, 32, 64, 128} x band {0, 256}: three passes each on the
- *                  live lists, scratch accumulators.
- * Waves, bands and receiver groups give bit-identical results; K only regroups
+ *                  bands), and the queued receivers with bands {32, 256}, then
+ *                  for the best K: waves {2, 4} x receiver groups {8, 32, 64,
+ *                  128} x band {0, 256}: three passes each on the live lists,
+ *                  scratch accumulators.
+ * Waves, bands, receiver groups and the receiver queue give bit-identical results; K only regroups
  * the hours of the Kahan-summed annual totals (same sums to FP32 rounding).  The winners are
  * appended to a text cache keyed by device, format and scene size class. */
 #include <cmath>
@@ -61,6 +62,7 @@ bool lookup(const std::string& key, sgpu_tuning& t) {
         sgpu_tuning r{};
         if (in >> k >> r.dirs_per_group >> r.hours_per_pass >> r.waves_per_group >> r.xcd_band >> r.group_blocks &&
             k == key) {
+            if (!(in >> r.receiver_queue)) r.receiver_queue = 0;   // entries written before the queue existed
             t = r;
             found = true;   // the last entry for a key wins
         }
@@ -71,7 +73,7 @@ bool lookup(const std::string& key, sgpu_tuning& t) {
 void store(const std::string& key, const sgpu_tuning& t) {
     std::ofstream f(cache_path(), std::ios::app);
     if (f) f << key << " " << t.dirs_per_group << " " << t.hours_per_pass << " " << t.waves_per_group << " "
-             << t.xcd_band << " " << t.group_blocks << "\n";
+             << t.xcd_band << " " << t.group_blocks << " " << t.receiver_queue << "\n";
 }
 
 double time_trace(sgpu_ctx* c, int G, int64_t v0, int64_t n, int32_t* raw) {
@@ -86,12 +88,15 @@ double time_trace(sgpu_ctx* c, int G, int64_t v0, int64_t n, int32_t* raw) {
     return e.ms();
 }
 
-double time_passes(sgpu_ctx* c, unsigned fmt, int K, int waves, int band, int group, int passes) {
+double time_passes(sgpu_ctx* c, unsigned fmt, int K, int waves, int band, int group, int queue, int passes) {
     ensure_accumulators(c, K);   // the hourly buffer grows to this trial's n_top x K
     HourArgs a = stage2_args(c, fmt);
     a.waves = waves;
     a.band = band;
     a.group = group;
+    a.queue = queue ? c->reflect_queue : nullptr;
+    a.queue_band = (int64_t)band * 64 * waves;
+    a.queue_blocks_per_cu = queue ? group : 0;
     const int nsel = (int)c->hours.size();
     auto run = [&](int np) {
         for (int p = 0, h0 = 0; p < np && h0 < nsel; ++p, h0 += K) {
@@ -145,8 +150,8 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
         ensure_hours_d(c, c->hours);
         double best_rate = 0.0;
         const int passes = 3;
-        auto trial = [&](int K, int waves, int band, int group) {
-            const double ms = time_passes(c, fmt, K, waves, band, group, passes);
+        auto trial = [&](int K, int waves, int band, int group, int queue) {
+            const double ms = time_passes(c, fmt, K, waves, band, group, queue, passes);
             const int hours = std::min<int>(passes * K, (int)c->hours.size());
             const double rate = (double)c->n_total * hours / (ms * 1e-3);
             if (rate > best_rate) {
@@ -155,15 +160,18 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
                 t.waves_per_group = waves;
                 t.xcd_band = band;
                 t.group_blocks = group;
+                t.receiver_queue = queue;
             }
         };
         for (int K : {64, 128, 256})
-            for (int waves : {2, 4})
-                for (int band : {0, 64, 256}) trial(K, waves, band, 0);
+            for (int waves : {2, 4}) {
+                for (int band : {0, 64, 256}) trial(K, waves, band, 0, 0);
+                for (int band : {32, 256}) trial(K, waves, band, 0, 1);   // queued receivers
+            }
         const int K = t.hours_per_pass;
         for (int waves : {2, 4})   // the best waves count with receiver groups can differ
             for (int group : {8, 32, 64, 128})
-                for (int band : {0, 256}) trial(K, waves, band, group);
+                for (int band : {0, 256}) trial(K, waves, band, group, 0);
         t.voxel_hours_per_s = best_rate;
         c->irr_ok = false;   // the scratch passes overwrote the annual accumulators
     }
@@ -203,8 +211,13 @@ int sgpu_set_tuning(sgpu_ctx* c, const sgpu_tuning* t) {
         need(t->waves_per_group == 2 || t->waves_per_group == 4, SGPU_ERR_ARG, "waves_per_group must be 2 or 4");
         need(t->xcd_band == 0 || (t->xcd_band >= 16 && t->xcd_band <= 4096), SGPU_ERR_ARG,
              "xcd_band must be 0 or 16..4096");
-        need(t->group_blocks == 0 || (t->group_blocks >= 2 && t->group_blocks <= 1024), SGPU_ERR_ARG,
-             "group_blocks must be 0 or 2..1024");
+        need(t->receiver_queue == 0 || t->receiver_queue == 1, SGPU_ERR_ARG, "receiver_queue must be 0 or 1");
+        if (t->receiver_queue)
+            need(t->group_blocks >= 0 && t->group_blocks <= 64, SGPU_ERR_ARG,
+                 "group_blocks (workgroups per CU of the receiver queue) must be 0..64");
+        else
+            need(t->group_blocks == 0 || (t->group_blocks >= 2 && t->group_blocks <= 1024), SGPU_ERR_ARG,
+                 "group_blocks must be 0 or 2..1024");
         c->tune = *t;
     });
 }

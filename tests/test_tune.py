@@ -54,14 +54,16 @@ def test_autotune_measures_then_hits_the_cache(tmp_path, monkeypatch):
 
 def test_receiver_assignment_is_bit_identical():
     """Each voxel's reflection walk and Kahan sum are its own: consecutive receivers per wave,
-    receiver groups over the tile-major order and XCD bands give bit-identical annual sums."""
+    receiver groups over the tile-major order, XCD bands and the per-XCD receiver queue give
+    bit-identical annual sums."""
     sc = synth_scene(13, 150, 120)
     with _ctx(sc) as ctx:
         ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
         out = []
         for fmt in (sg.FMT_CSR, sg.FMT_BARLE):
-            for waves, band, group in ((4, 0, 0), (4, 16, 8), (2, 0, 32), (2, 64, 3)):
-                ctx.set_tuning(16, 32, waves, band, group)
+            for waves, band, group, queue in ((4, 0, 0, 0), (4, 16, 8, 0), (2, 0, 32, 0), (2, 64, 3, 0),
+                                              (4, 0, 0, 1), (4, 16, 0, 1), (2, 32, 0, 1)):
+                ctx.set_tuning(16, 32, waves, band, group, queue)
                 ctx.hour_loop(fmt, 32)
                 out.append(ctx.irradiance())
             for x in out[1:]:

@@ -84,7 +84,7 @@ def test_batched_64_tiles_equal_per_tile_contexts_and_the_oracle():
             with sg.Context(0) as ctx:
                 ctx.set_scene(sc)
                 ctx.set_sky(az, dni, dhi)
-                ctx.set_tuning(G, K, waves, 0, 0)
+                ctx.set_tuning(G, K, waves, 0, 0, 1)   # queued receivers against the batch's static ones
                 ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
                 off, ids = b.csr(t)
                 eoff, eids = ctx.csr()
