@@ -285,6 +285,16 @@ def main():
         extra["surrogate"] = surrogate_leg(sg, sc, az_alt, dni, dhi, cell, ctx.tuning(), exact,
                                            (t_vs + t_hr) / args.steps, H)
     copy_gbps = ctx.copy_bandwidth(1 << 30, 20)
+    l2_gbps = ctx.l2_bandwidth(2 << 20, 3)
+    # the reflection pass's row gathers: L1 -> L2 line requests per launch (committed PMC summary of the
+    # same workload and shape) over the live launch time, against the measured L2 read bandwidth
+    l2_req = read_profile("reflect_barle_c2_traffic.txt", "l1_to_l2_read_requests_per_launch")
+    gather = {"bound": "l2", "unit": "GB/s", "kernel": f"k_reflect<K={Kp}, ba-RLE>", "peak": l2_gbps,
+              "peak_source": "measured: every wave streams 2 MiB (L2-resident) as 1 KB rows of 16-byte lane loads",
+              "l1_to_l2_read_bytes_per_launch": (l2_req * 128) if l2_req else None,
+              "l2_hit_rate": read_profile("reflect_barle_c2_traffic.txt", "l2_hit_rate"),
+              "achieved": (l2_req * 128 / avg_launch / 1e9) if l2_req else None}
+    gather["frac"] = gather["achieved"] / l2_gbps if gather["achieved"] else None
     valu_fma, valu_mix, valu_cmp = ctx.valu_rate(0), ctx.valu_rate(1), ctx.valu_rate(2)
     valu_peak = max(valu_fma, valu_mix, valu_cmp)
     # DDA roofline: VALU issue.  Instructions per trace launch from the committed PMC summary of the
@@ -352,8 +362,10 @@ def main():
                      # launch time, against the copy bandwidth measured in this run
                      "traffic_gbps": (traffic / avg_launch / 1e9) if traffic else None,
                      "traffic_frac_of_copy": (traffic / avg_launch / 1e9 / copy_gbps) if traffic else None},
+        "roofline_gather": gather,
         "roofline_dda": dda_roof,
         "measured_copy_gbps": copy_gbps,
+        "measured_l2_read_gbps": l2_gbps,
         "measured_valu_wave_insts_per_s": {"fp32_fma": valu_fma, "int_add_compare_select": valu_mix,
                                            "int_add_compare_to_sgpr": valu_cmp},
     }

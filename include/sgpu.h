@@ -248,6 +248,10 @@ int sgpu_copy_bandwidth(sgpu_ctx* ctx, int64_t bytes, int reps, double* gbps);
    instruction count that fills every SIMD (new; the peak the DDA's VALU roofline uses):
    mix 0 = FP32 FMAs only, mix 1 = the DDA's own classes (integer add, compare, select). */
 int sgpu_valu_rate(sgpu_ctx* ctx, int mix, double* wave_insts_per_s);
+/* L2 read bandwidth in GB/s: every wave streams a `footprint`-byte buffer (resident in each XCD's
+   L2, beyond a CU's L1) as 1 KB rows of 16-byte lane loads, `reps` launches (new; the peak of
+   the reflection pass's row gathers).  footprint: a multiple of 4 KiB in 64 KiB .. 64 MiB. */
+int sgpu_l2_bandwidth(sgpu_ctx* ctx, int64_t footprint, int reps, double* gbps);
 
 /* ---- per-voxel surrogate MLP (new; no reference counterpart) ----
    64 inputs (53 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and

@@ -461,6 +461,12 @@ class Context:
         _check(_lib.sgpu_copy_bandwidth(self._h, _I64(nbytes), reps, ctypes.byref(g)))
         return g.value
 
+    def l2_bandwidth(self, footprint: int = 2 << 20, reps: int = 3) -> float:
+        """GB/s of L2-resident reads: every wave streams a footprint-byte buffer as 1 KB rows."""
+        g = ctypes.c_double()
+        _check(_lib.sgpu_l2_bandwidth(self._h, _I64(footprint), reps, ctypes.byref(g)))
+        return g.value
+
     def valu_rate(self, mix: int = 0) -> float:
         """Measured VALU issue rate, wave64 instructions/s, of a probe filling every SIMD:
         mix 0 = FP32 FMAs, mix 1 = integer add / compare / select (the DDA's classes)."""

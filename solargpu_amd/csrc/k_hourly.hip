@@ -58,6 +58,14 @@ __device__ __forceinline__ bool svf_blocked(const uint8_t* svf, int64_t v, int d
     return (svf[(int64_t)(dense >> 3) * n + v] >> (dense & 7)) & 1;   // voxel_types.h:36-44
 }
 
+/* The row of cell c in the hourly buffer: c itself, or (swz = S > 0) c with its low 10 bits
+ * XOR-ed with bits S.. (the cell's grid row when S = log2 of the width), a bijection within each
+ * aligned block of 1024 rows.  Cells of one grid column sit W rows apart in cell order; the swizzle
+ * gives them different low address bits, so a patch of neighbour rows spreads over the L2's sets. */
+__device__ __forceinline__ unsigned row_slot(unsigned c, int swz) {
+    return swz > 0 ? c ^ ((c >> swz) & 1023u) : c;
+}
+
 template <int K>
 __device__ __forceinline__ void solar_top_cell(const HourArgs& a, int64_t c) {
     if (c >= a.n_top) return;
@@ -65,7 +73,7 @@ __device__ __forceinline__ void solar_top_cell(const HourArgs& a, int64_t c) {
     float ss, cs, sa, ca;
     sincosf(slope, &ss, &cs);
     sincosf(aspect, &sa, &ca);
-    float* dst = a.hourly_top + c * K;
+    float* dst = a.hourly_top + (int64_t)row_slot((unsigned)c, a.row_swz) * K;
     constexpr int V = (K % 4 == 0) ? 4 : 1;   // hours per store
     for (int h0 = 0; h0 < K; h0 += V) {
         float out[V];
@@ -196,7 +204,7 @@ __device__ __forceinline__ int64_t uniform64(int64_t x) {
  * offset from the buffer base, one shift-add per entry. */
 template <int K, bool OFF32>
 __device__ __forceinline__ float apply_list_at(const int* ids, const float* ws, int cnt,
-                                               const float* __restrict__ top, int lane) {
+                                               const float* __restrict__ top, int lane, int swz) {
     constexpr int HPL = (K >= 16) ? 4 : 1;
     constexpr int LPE = K / HPL;
     constexpr int G = 64 / LPE;
@@ -207,12 +215,13 @@ __device__ __forceinline__ float apply_list_at(const int* ids, const float* ws, 
     const unsigned lane_off = (unsigned)((lane % LPE) * HPL) * 4u;
     const char* lane_top = reinterpret_cast<const char*>(top) + lane_off;   // 64-bit rows: one shift-add each
     constexpr int kRowShift = __builtin_ctz(K * 4);
-    auto load = [&](int id) __attribute__((always_inline)) {
+    auto load = [&](int cell) __attribute__((always_inline)) {
+        const unsigned id = row_slot((unsigned)cell, swz);
         if constexpr (OFF32) {
             const unsigned off = (unsigned)id * (unsigned)(K * 4) + lane_off;
             return *reinterpret_cast<const vec*>(reinterpret_cast<const char*>(top) + off);
         } else {
-            return *reinterpret_cast<const vec*>(lane_top + ((uint64_t)(unsigned)id << kRowShift));
+            return *reinterpret_cast<const vec*>(lane_top + ((uint64_t)id << kRowShift));
         }
     };
     vec a = {}, b = {};
@@ -261,9 +270,10 @@ __device__ __forceinline__ float apply_list_at(const int* ids, const float* ws, 
 
 template <int K>
 __device__ __forceinline__ float apply_list(const int* ids, const float* ws, int cnt, const float* __restrict__ top,
-                                            int lane, bool off32) {
+                                            int lane, bool off32, int swz) {
     static_assert(K >= 1 && K <= 256 && (K & (K - 1)) == 0, "a power of two hours per pass");
-    return off32 ? apply_list_at<K, true>(ids, ws, cnt, top, lane) : apply_list_at<K, false>(ids, ws, cnt, top, lane);
+    return off32 ? apply_list_at<K, true>(ids, ws, cnt, top, lane, swz)
+                 : apply_list_at<K, false>(ids, ws, cnt, top, lane, swz);
 }
 
 /* ba-RLE bytes staged in LDS per workgroup, split evenly over its waves. */
@@ -416,7 +426,7 @@ __device__ __forceinline__ void receiver_parts(const HourArgs& a, const HourCons
         cnt += __popcll(m);
         if (cnt > kListCap - 64) {
             wave_lds_sync();
-            acc += apply_list<K>(L.list_id[wave], L.list_w[wave], cnt, a.hourly_top, lane, off32);
+            acc += apply_list<K>(L.list_id[wave], L.list_w[wave], cnt, a.hourly_top, lane, off32, a.row_swz);
             wave_lds_sync();
             cnt = 0;
         }
@@ -455,7 +465,7 @@ __device__ __forceinline__ void receiver_parts(const HourArgs& a, const HourCons
     }
     if (cnt > 0) {
         wave_lds_sync();
-        acc += apply_list<K>(L.list_id[wave], L.list_w[wave], cnt, a.hourly_top, lane, off32);
+        acc += apply_list<K>(L.list_id[wave], L.list_w[wave], cnt, a.hourly_top, lane, off32, a.row_swz);
         wave_lds_sync();
     }
     refl = acc;
@@ -799,9 +809,11 @@ bool supported_hours_per_pass(int K) {
 /* The hourly irradiance rows of a pass (n_top x K floats) must fit the buffer they are written to
    (k_solar_top) and gathered from (k_reflect). */
 void check_hourly_cap(const HourArgs& a, int K) {
-    if (a.n_top * (int64_t)K > a.hourly_top_cap)
+    if (a.row_swz < 0 || a.row_swz > 30) fail(SGPU_ERR_ARG, "row swizzle shift %d out of range", a.row_swz);
+    const int64_t rows = hourly_rows(a.n_top, a.row_swz);
+    if (rows * (int64_t)K > a.hourly_top_cap)
         fail(SGPU_ERR_STATE, "hourly irradiance buffer holds %lld floats, the pass needs %lld",
-             (long long)a.hourly_top_cap, (long long)(a.n_top * (int64_t)K));
+             (long long)a.hourly_top_cap, (long long)(rows * (int64_t)K));
 }
 
 void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {

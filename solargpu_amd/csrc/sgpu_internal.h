@@ -222,8 +222,9 @@ struct HourArgs {
     const uint8_t* svf;
     const HourConst* hours;    // this pass's hours (nh entries)
     int nh;
-    float* hourly_top;         // n_top x K, cell-major
-    int64_t hourly_top_cap;    // floats available at hourly_top (the launchers check n_top x K)
+    float* hourly_top;         // n_top x K, cell-major (row of cell c: row_slot(c, row_swz))
+    int64_t hourly_top_cap;    // floats available at hourly_top (the launchers check the rows x K)
+    int row_swz;               // 0: row c holds cell c; S > 0: cell c's row is c ^ ((c >> S) & 1023)
     const int64_t* nb_off;     // CSR offsets or ba-RLE byte offsets
     const int32_t* nb_ids;
     const uint8_t* nb_bytes;
@@ -254,6 +255,8 @@ struct HourArgs {
     int64_t queue_band;
     int queue_blocks_per_cu;   // resident workgroups per CU of the queued grid (0 = as many as fit)
 };
+/* Rows of the hourly buffer for n_top cells: swizzled rows stay inside aligned blocks of 1024. */
+inline int64_t hourly_rows(int64_t n_top, int row_swz) { return row_swz > 0 ? (n_top + 1023) / 1024 * 1024 : n_top; }
 constexpr int kQueuesPerXcd = 16;
 constexpr int kQueueStride = 64;                       // unsigned per counter: a 256-byte line each
 constexpr int kQueueWords = 8 * kQueuesPerXcd * kQueueStride;   // 8 XCDs: 32 KB

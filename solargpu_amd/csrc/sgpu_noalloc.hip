@@ -93,6 +93,7 @@ int sgpu_hour_loop_noalloc(sgpu_ctx* c, unsigned fmt, int K, float* out) {
         HourArgs a = stage2_args(c, fmt);
         a.hourly_top = reinterpret_cast<float*>(base);
         a.hourly_top_cap = c->n_top * (int64_t)K;
+        a.row_swz = 0;   // rows in cell order: the scratch holds exactly n_top x K floats
         a.comp = reinterpret_cast<float*>(base + round256((size_t)c->n_top * K * sizeof(float)));
         a.sum = out;
         a.queue = nullptr;

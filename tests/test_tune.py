@@ -72,7 +72,7 @@ def test_receiver_assignment_is_bit_identical():
 
 
 def test_undersized_hourly_buffer_is_refused_before_any_launch(tmp_path, monkeypatch):
-    """A pass of K hours writes and gathers n_top x K floats of hourly rows.  The autotuner once
+    """A pass of K hours writes and gathers K floats per hourly row.  The autotuner once
     ran K = 128 against a buffer sized for K = 64, and the reflection pass read past the buffer
     (a GPU memory fault); every pass now checks the capacity before it launches.  With the
     usable buffer capped at the K = 64 size, a K = 128 loop must be refused with SGPU_ERR_STATE
@@ -83,10 +83,11 @@ def test_undersized_hourly_buffer_is_refused_before_any_launch(tmp_path, monkeyp
     ref = oracle_lists(11, 48, 40)
     az_alt, dni, dhi, bins = sky()
     exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
+    rows = -(-sc.num_top // 1024) * 1024   # the swizzled hourly rows fill whole blocks of 1024
     with _ctx(sc) as ctx:
         ctx.viewshed(sg.FMT_BARLE)
         ctx.hour_loop(sg.FMT_BARLE, 64)
-        ctx.set_hourly_limit(sc.num_top * 64)
+        ctx.set_hourly_limit(rows * 64)
         for K in (128, 256):
             with pytest.raises(sg.SolarGPUError) as e:
                 ctx.hour_loop(sg.FMT_BARLE, K)
