@@ -42,14 +42,33 @@ constexpr int kHashSize = 4096;              // > 1.5 x the 2592 possible distin
  * all idle lanes at once. */
 constexpr int kRefill = 16;
 
+/* x as a wave-uniform value (the same in every lane): lets indexing by it use scalar loads. */
+__device__ __forceinline__ int64_t uniform_i64(int64_t x) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+/* LDS of one trace workgroup: its G directions, and per wave the origins and raw rows of its 64
+   voxels. */
+template <int G>
+struct TraceLds {
+    DirConst dir[G];
+    float org[kTraceBlock / 64][3][64];   // gx, gy, base z of the wave's voxels
+    int row[kTraceBlock / 64][64];        // their raw rows
+};
+
+/* One (direction group grp, voxel block blk) item of a scene's trace, by the whole workgroup:
+   stage the group's directions and the block's voxel origins, then every wave runs the pooled walk
+   over its 64 voxels x G directions.  Contains one workgroup barrier, reached by every thread. */
 template <bool COUNT_STEPS, int G>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
-    __shared__ DirConst sdir[G];
-    __shared__ float sorg[kTraceBlock / 64][3][64];   // gx, gy, base z of the wave's voxels
-    __shared__ int srow[kTraceBlock / 64][64];        // their raw rows
+__device__ __forceinline__ void trace_pool_item(const TraceArgs& a, int grp, int64_t blk, TraceLds<G>& L) {
+    DirConst* sdir = L.dir;
+    float (*sorg)[3][64] = L.org;
+    int (*srow)[64] = L.row;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int i = threadIdx.x; i < G; i += kTraceBlock) sdir[i] = a.dirs[blockIdx.x * G + i];
-    const int64_t lane_pos = (int64_t)blockIdx.y * kTraceBlock + threadIdx.x;
+    for (int i = threadIdx.x; i < G; i += kTraceBlock) sdir[i] = a.dirs[grp * G + i];
+    const int64_t lane_pos = blk * kTraceBlock + threadIdx.x;
     const int64_t wave_pos = lane_pos - lane;
     const int nvalid = (int)min<int64_t>(64, max<int64_t>(0, a.count - wave_pos));
     if (lane < nvalid) {
@@ -86,7 +105,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
                 const int k = j >> 6, vx = j & 63;
                 if (j < total && vx < nvalid) {
                     pool_ray_setup(ray, sdir[k], sorg[wave][0][vx], sorg[wave][1][vx], sorg[wave][2][vx], uW, maxz);
-                    out = a.raw + (int64_t)srow[wave][vx] * kDirs + blockIdx.x * G + k;
+                    out = a.raw + (int64_t)srow[wave][vx] * kDirs + grp * G + k;
                     active = true;
                 }
             }
@@ -105,6 +124,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
         }
     }
     if (COUNT_STEPS) atomicAdd(a.steps, steps);
+}
+
+template <bool COUNT_STEPS, int G>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_pool(TraceArgs a) {
+    __shared__ TraceLds<G> L;
+    trace_pool_item<COUNT_STEPS, G>(a, (int)blockIdx.x, (int64_t)blockIdx.y, L);
 }
 
 /* Inclusive wave64 scan. */
@@ -297,88 +322,26 @@ void launch_list_copy(const int32_t* raw, const int32_t* counts, const int64_t* 
     hipLaunchKernelGGL(k_list_copy, dim3(g), dim3(64 * kCopyWaves), 0, s, raw, counts, offsets, ids, count);
 }
 
-/* Persistent multi-tile DDA: every workgroup takes (256-voxel block, 32-direction group) items
- * from a global counter until the chunk is exhausted; per item each wave runs k_trace_pool's
- * pool over its 64 voxels x 32 directions, every ray with its own tile's heightmap, bounds and
- * height limit (tiles are contiguous in the global voxel order; a binary search finds each
- * voxel's tile). */
-__global__ __launch_bounds__(kTraceBlock) void k_trace_persistent_pool(BatchTraceArgs a) {
-    __shared__ long long item_s;
-    __shared__ int stile[kTraceBlock / 64][64];
-    __shared__ float sorg[kTraceBlock / 64][3][64];
+/* Persistent multi-tile DDA: every workgroup takes (voxel block, direction group) items from a
+ * global counter until the chunk is exhausted.  A chunk is a list of slices, each a run of one
+ * tile's voxels (in the tile's height-sorted lane order) with its own TraceArgs; a block lies in
+ * one slice, so the tile's heightmap, direction table and bounds are uniform over the item and the
+ * item runs exactly the single-scene trace body (trace_pool_item).  Items are block-major: the
+ * direction groups of a block are consecutive, like the single-scene grid. */
+__global__ __launch_bounds__(kTraceBlock) void k_trace_batch_pool(BatchTraceArgs b) {
     constexpr int G = kPersistentDirs;
-    const int64_t groups = kDirs / G;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int64_t groups = kDirs / G;
+    __shared__ TraceLds<G> L;
+    __shared__ long long item_s;
     for (;;) {
-        if (threadIdx.x == 0) item_s = (long long)atomicAdd(a.next, 1ull);
-        __syncthreads();   // also: the previous item's LDS reads are done
-        const int64_t item = item_s;
-        if (item >= a.items) break;   // every wave of the workgroup leaves together
-        const int64_t blk = item / groups, grp = item % groups;
-        const int64_t local = blk * kTraceBlock + threadIdx.x;   // raw row
-        const int nvalid = (int)min<int64_t>(64, max<int64_t>(0, a.count - (local - lane)));
-        if (lane < nvalid) {
-            const int64_t g = a.v0 + local;
-            int lo = 0, hi = a.ntiles - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (a.tiles[mid].vbase <= g) lo = mid;
-                else hi = mid - 1;
-            }
-            const TileDesc& t = a.tiles[lo];
-            const int64_t v = g - t.vbase;
-            const int top = t.pix_parent[v];
-            const int gx = top % t.W, gy = top / t.W;
-            stile[wave][lane] = lo;
-            sorg[wave][0][lane] = (float)gx;
-            sorg[wave][1][lane] = (float)gy;
-            sorg[wave][2][lane] = (v < t.n_top) ? t.grid[top] : t.wall_z[v - t.n_top];
-        }
-        __syncthreads();
-        if (nvalid > 0) {
-            unsigned long long steps = 0;
-            bool active = false;
-            PoolRay ray{};
-            const float* grid = nullptr;
-            unsigned lim_x = 0, lim_y = 0;
-            int32_t* out = nullptr;
-            const int total = G * 64;
-            int next = 0;   // wave-uniform pool cursor
-            for (;;) {
-                const uint64_t idle = __ballot(!active);
-                const int nidle = __popcll(idle);
-                if (next < total && (nidle >= kRefill || nidle == 64)) {
-                    if (!active) {
-                        const int j = next + (int)__builtin_amdgcn_mbcnt_hi(
-                                                 (unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
-                        const int k = j >> 6, vx = j & 63;
-                        if (j < total && vx < nvalid) {
-                            const TileDesc& t = a.tiles[stile[wave][vx]];
-                            const unsigned uW = (unsigned)t.W;
-                            pool_ray_setup(ray, t.dirs[grp * G + k], sorg[wave][0][vx], sorg[wave][1][vx],
-                                           sorg[wave][2][vx], uW, t.maxz);
-                            grid = t.grid;
-                            lim_x = 4u * uW;
-                            lim_y = 4u * uW * (unsigned)t.H;
-                            out = a.raw + (local - lane + vx) * kDirs + grp * G + k;
-                            active = true;
-                        }
-                    }
-                    next += nidle;
-                }
-                if (__ballot(active) == 0) {
-                    if (next >= total) break;
-                    continue;
-                }
-                if (active) {
-                    unsigned res;
-                    if (pool_ray_steps<false>(ray, grid, lim_x, lim_y, steps, res)) {
-                        *out = (int32_t)res;
-                        active = false;
-                    }
-                }
-            }
-        }
+        if (threadIdx.x == 0) item_s = (long long)atomicAdd(b.next, 1ull);
+        __syncthreads();   // also: every wave is done with the previous item's LDS
+        const int64_t item = uniform_i64(item_s);
+        if (item >= b.items) break;   // every wave of the workgroup leaves together
+        const int64_t blk = item / groups;
+        const int grp = (int)(item % groups);
+        const int si = b.block_slice[blk];
+        trace_pool_item<false, G>(b.slices[si], grp, (int64_t)b.block_local[blk], L);
     }
 }
 
@@ -492,8 +455,8 @@ void launch_lists(const ListArgs& a, bool write_pass, hipStream_t s) {
 }
 
 void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s) {
-    if (a.count <= 0) return;
-    hipLaunchKernelGGL(k_trace_persistent_pool, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s, a);
+    if (a.items <= 0) return;
+    hipLaunchKernelGGL(k_trace_batch_pool, dim3((unsigned)std::max(workgroups, 1)), dim3(kTraceBlock), 0, s, a);
     SGPU_HIP(hipGetLastError());
 }
 

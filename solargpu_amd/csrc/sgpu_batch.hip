@@ -43,9 +43,11 @@ struct BatchTile {
     HourConst* hours_d = nullptr;
     uint8_t* svf = nullptr;
     int32_t* order = nullptr;   // reflection-pass processing order (32-cell tiles, walls after their cell)
+    int32_t* perm = nullptr;    // DDA lane order (blocks of kTraceOrderBlock ids sorted by height)
 
     void release() {
         dfree(order);
+        dfree(perm);
         dfree(grid);
         dfree(pix_parent);
         dfree(wall_z);
@@ -150,6 +152,9 @@ void add_tile(sgpu_batch* b, const float* grid, int w, int h, int64_t n_total, i
                                     hipMemcpyHostToDevice, s));
         const std::vector<int32_t> order = processing_order(w, h, n_total, pp, 32);
         SGPU_HIP(hipMemcpyAsync(t.order, order.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        t.perm = dalloc<int32_t>(n_total);
+        const std::vector<int32_t> perm = trace_order(n_total, grid, pp, wall_z, n_top, kTraceOrderBlock);
+        SGPU_HIP(hipMemcpyAsync(t.perm, perm.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice, s));
         SGPU_HIP(hipStreamSynchronize(s));
     } catch (...) {
         t.release();
@@ -165,7 +170,41 @@ int64_t batch_chunk(sgpu_batch* b) {
     size_t free_b = 0, total_b = 0;
     SGPU_HIP(hipMemGetInfo(&free_b, &total_b));
     const int64_t by_mem = (int64_t)(0.4 * (double)free_b / ((double)kDirs * sizeof(int32_t)));
-    return std::max<int64_t>(1, std::min<int64_t>({b->n_all, by_mem, (int64_t)4 << 20}));
+    return std::max<int64_t>(kTraceOrderBlock, std::min<int64_t>({b->n_all, by_mem, (int64_t)4 << 20}));
+}
+
+/* A chunk of the batch's voxel space [c0, c1) and its slices: runs (tile, first tile-local voxel,
+   count) of one tile each, cut at multiples of kTraceOrderBlock within a tile so that every lane
+   order block lies in one slice. */
+struct Slice {
+    int tile;
+    int64_t v0, count;
+};
+struct ChunkPlan {
+    int64_t c0 = 0, c1 = 0;
+    std::vector<Slice> slices;
+};
+std::vector<ChunkPlan> plan_chunks(const sgpu_batch* b, int64_t cap) {
+    std::vector<ChunkPlan> plan(1);
+    for (int t = 0; t < (int)b->tiles.size(); ++t) {
+        const BatchTile& tile = b->tiles[t];
+        for (int64_t v = 0; v < tile.n_total;) {
+            ChunkPlan& c = plan.back();
+            const int64_t space = cap - (c.c1 - c.c0);
+            int64_t take = std::min(tile.n_total - v, space);
+            if (take < tile.n_total - v) take -= take % kTraceOrderBlock;
+            if (take <= 0) {   // the chunk is full: the next one starts here
+                ChunkPlan nc;
+                nc.c0 = nc.c1 = c.c1;
+                plan.push_back(nc);
+                continue;
+            }
+            c.slices.push_back({t, v, take});
+            c.c1 += take;
+            v += take;
+        }
+    }
+    return plan;
 }
 
 void viewshed(sgpu_batch* b) {
@@ -181,12 +220,22 @@ void viewshed(sgpu_batch* b) {
     }
     const int64_t N = b->n_all;
     const int64_t chunk = batch_chunk(b);
+    const std::vector<ChunkPlan> plan = plan_chunks(b, chunk);
+    size_t max_slices = 0, max_blocks = 0;
+    for (const ChunkPlan& c : plan) {
+        size_t blocks = 0;
+        for (const Slice& sl : c.slices) blocks += (size_t)((sl.count + 255) / 256);
+        max_slices = std::max(max_slices, c.slices.size());
+        max_blocks = std::max(max_blocks, blocks);
+    }
     TileDesc* tiles_d = nullptr;
     int32_t* raw = nullptr;
     int32_t* counts = nullptr;
     int64_t* off = nullptr;
     int64_t* scratch = nullptr;
     unsigned long long* next = nullptr;
+    TraceArgs* slices_d = nullptr;
+    int32_t* blocks_d = nullptr;   // block_slice then block_local
     std::vector<std::pair<int32_t*, int64_t>> pieces;
     EventPair all, et, el, ee;
     b->info = sgpu_batch_info{};
@@ -198,14 +247,45 @@ void viewshed(sgpu_batch* b) {
         off = dalloc<int64_t>(chunk + 1);
         scratch = dalloc<int64_t>(scan_scratch_elems(chunk));
         next = dalloc<unsigned long long>(1);
+        slices_d = dalloc<TraceArgs>(std::max<size_t>(max_slices, 1));
+        blocks_d = dalloc<int32_t>(2 * std::max<size_t>(max_blocks, 1));
         b->csr_off = dalloc<int64_t>(N + 1);
         SGPU_HIP(hipEventRecord(all.a, s));
         int64_t base = 0;
-        for (int64_t v0 = 0; v0 < N; v0 += chunk) {
-            const int64_t cnt = std::min(chunk, N - v0);
-            const int64_t items = (cnt + 255) / 256 * (kDirs / kPersistentDirs);
+        for (const ChunkPlan& cp : plan) {
+            const int64_t v0 = cp.c0, cnt = cp.c1 - cp.c0;
+            if (cnt <= 0) continue;
+            std::vector<TraceArgs> sl;
+            std::vector<int32_t> bs, bl;
+            for (const Slice& x : cp.slices) {
+                const BatchTile& t = b->tiles[x.tile];
+                TraceArgs a{};
+                a.grid = t.grid;
+                a.W = t.W;
+                a.H = t.H;
+                a.n_top = t.n_top;
+                a.pix_parent = t.pix_parent;
+                a.wall_z = t.wall_z;
+                a.maxz = t.maxz;
+                a.dirs = t.dirs;
+                a.v0 = x.v0;
+                a.count = x.count;
+                a.raw = raw + (t.vbase + x.v0 - v0) * kDirs;
+                a.group = kPersistentDirs;
+                a.perm = t.perm;
+                for (int64_t k = 0; k < (x.count + 255) / 256; ++k) {
+                    bs.push_back((int32_t)sl.size());
+                    bl.push_back((int32_t)k);
+                }
+                sl.push_back(a);
+            }
+            const int64_t nblocks = (int64_t)bs.size();
+            bs.insert(bs.end(), bl.begin(), bl.end());
+            SGPU_HIP(hipMemcpyAsync(slices_d, sl.data(), sl.size() * sizeof(TraceArgs), hipMemcpyHostToDevice, s));
+            SGPU_HIP(hipMemcpyAsync(blocks_d, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+            const int64_t items = nblocks * (kDirs / kPersistentDirs);
             SGPU_HIP(hipMemsetAsync(next, 0, sizeof(unsigned long long), s));
-            BatchTraceArgs ta{tiles_d, nt, v0, cnt, raw, next, items};
+            BatchTraceArgs ta{slices_d, blocks_d, blocks_d + nblocks, next, items};
             SGPU_HIP(hipEventRecord(et.a, s));
             launch_trace_persistent(ta, (int)std::min<int64_t>(items, (int64_t)b->cus * 8), s);
             SGPU_HIP(hipEventRecord(et.b, s));
@@ -263,6 +343,8 @@ void viewshed(sgpu_batch* b) {
         dfree(off);
         dfree(scratch);
         dfree(next);
+        dfree(slices_d);
+        dfree(blocks_d);
         b->drop_lists();
         throw;
     }
@@ -272,6 +354,8 @@ void viewshed(sgpu_batch* b) {
     dfree(off);
     dfree(scratch);
     dfree(next);
+    dfree(slices_d);
+    dfree(blocks_d);
     b->lists = true;
     b->info.tiles = nt;
     b->info.voxels = N;

@@ -148,13 +148,16 @@ struct TileDesc {
     int svf_bytes;
     int64_t vbase;
 };
+/* One persistent trace launch over a chunk of the batch: slice i is a run of one tile's voxels
+   (TraceArgs with the tile's scene, directions and lane order; v0 a multiple of kTraceOrderBlock,
+   raw pointing at the slice's first raw row), block k of the chunk is block block_local[k] of
+   slice block_slice[k]. */
 struct BatchTraceArgs {
-    const TileDesc* tiles;
-    int ntiles;
-    int64_t v0, count;          // global voxel range of this chunk
-    int32_t* raw;               // count x kDirs
+    const TraceArgs* slices;
+    const int32_t* block_slice;
+    const int32_t* block_local;
     unsigned long long* next;   // work-item counter (zeroed before the launch)
-    int64_t items;              // ceil(count / 256) blocks x (kDirs / 32) direction groups
+    int64_t items;              // blocks x (kDirs / kPersistentDirs) direction groups
 };
 void launch_trace_persistent(const BatchTraceArgs& a, int workgroups, hipStream_t s);
 struct BatchListArgs {
