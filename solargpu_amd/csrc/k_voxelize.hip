@@ -18,7 +18,7 @@
  * heights are bit-identical; slope and aspect go through the device atan2f.
  *
  * The scene's two processing orders, as the host builds them (host_voxel.cpp):
- *   k_order_cells   lane = cell: its rank in Hilbert-tile order (32-cell tiles
+ *   k_order_cells   lane = cell: its rank in Hilbert-tile order (kOrderTile-cell tiles
  *                   along a Hilbert curve, raster inside a tile; tile ranks and
  *                   first ranks from the host) and its voxel count 1 + walls;
  *   (int64 scan of the counts in that order)

@@ -42,7 +42,7 @@ struct BatchTile {
     DirConst* dirs = nullptr;
     HourConst* hours_d = nullptr;
     uint8_t* svf = nullptr;
-    int32_t* order = nullptr;   // reflection-pass processing order (32-cell tiles, walls after their cell)
+    int32_t* order = nullptr;   // reflection-pass processing order (Hilbert-ordered cell tiles, walls after their cell)
     int32_t* perm = nullptr;    // DDA lane order (blocks of kTraceOrderBlock ids sorted by height)
 
     void release() {
@@ -150,7 +150,7 @@ void add_tile(sgpu_batch* b, const float* grid, int w, int h, int64_t n_total, i
         if (!t.hours.empty())
             SGPU_HIP(hipMemcpyAsync(t.hours_d, t.hours.data(), t.hours.size() * sizeof(HourConst),
                                     hipMemcpyHostToDevice, s));
-        const std::vector<int32_t> order = processing_order(w, h, n_total, pp, 32);
+        const std::vector<int32_t> order = processing_order(w, h, n_total, pp, kOrderTile);
         SGPU_HIP(hipMemcpyAsync(t.order, order.data(), n_total * sizeof(int32_t), hipMemcpyHostToDevice, s));
         t.perm = dalloc<int32_t>(n_total);
         const std::vector<int32_t> perm = trace_order(n_total, grid, pp, wall_z, n_top, kTraceOrderBlock);

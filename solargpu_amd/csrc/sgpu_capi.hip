@@ -1099,7 +1099,7 @@ namespace {
 /* The reflection-pass processing order and the DDA lane order of the context's scene, built on
    the host from its heightmap, parents and wall heights, then uploaded. */
 void upload_orders(sgpu_ctx* c, const float* grid, const int32_t* pp, const float* wall_z) {
-    const std::vector<int32_t> order = processing_order(c->W, c->H, c->n_total, pp, 32);
+    const std::vector<int32_t> order = processing_order(c->W, c->H, c->n_total, pp, kOrderTile);
     c->order = dalloc<int32_t>(c->n_total);
     SGPU_HIP(hipMemcpyAsync(c->order, order.data(), c->n_total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     const std::vector<int32_t> tperm = trace_order(c->n_total, grid, pp, wall_z, c->n_top, kTraceOrderBlock);
@@ -1189,7 +1189,7 @@ int sgpu_set_scene_heights(sgpu_ctx* c, const float* heights, int w, int h, doub
         try {
             c->order = dalloc<int32_t>(n_total);
             c->trace_perm = dalloc<int32_t>(n_total);
-            orders_device(c->stream, c->grid, w, h, dv, n_total, 32, kTraceOrderBlock, c->order, c->trace_perm);
+            orders_device(c->stream, c->grid, w, h, dv, n_total, kOrderTile, kTraceOrderBlock, c->order, c->trace_perm);
         } catch (...) {
             dfree(dv.wall_off);
             throw;

@@ -130,7 +130,7 @@ struct sgpu_ctx {
     float* wall_z = nullptr;
     float* slopes = nullptr;
     float* aspects = nullptr;
-    int32_t* order = nullptr;    // reflection-pass processing order (Hilbert-ordered 32-cell tiles, walls after their cell)
+    int32_t* order = nullptr;    // reflection-pass processing order (Hilbert-ordered cell tiles of kOrderTile, walls after their cell)
     int32_t* trace_perm = nullptr;   // DDA lane order (blocks of kTraceOrderBlock ids sorted by height)
     bool scene = false;
 

@@ -74,6 +74,10 @@ void synth_dsm(uint64_t seed, int w, int h, float cell_m, float density, int tre
 /* Reflection-pass processing order: cells in tiles of tile x tile (tiles and cells in raster
    order), each top voxel followed by the wall voxels it parents. */
 std::vector<int32_t> processing_order(int w, int h, int64_t n_total, const int32_t* pix_parent, int tile);
+/* Side of the square cell tiles of the reflection pass's processing order.  Simulated over the
+   2048^2 bench scene's lists (LRU of 4096 one-KB rows per XCD, 896 receivers in flight), tiles
+   of 8 or 16 cells miss 17% less than tiles of 32 and 26% less than tiles of 64. */
+constexpr int kOrderTile = 16;
 /* The tiles (x, y) of a tw x th tile grid along a Hilbert curve (side: the next power of two),
    so that consecutive tiles are neighbours: the order of processing_order's tiles. */
 std::vector<std::pair<int, int>> hilbert_tiles(int tw, int th);
