@@ -129,7 +129,7 @@ int sgpu_get_info(sgpu_ctx* ctx, sgpu_info* out);
 int sgpu_set_scene_heights(sgpu_ctx* ctx, const float* heights, int grid_width, int grid_height,
                            double scale_x, double scale_y);
 /* The context's processing orders (num_total entries each; either may be NULL): the reflection
-   pass's (Hilbert-ordered 32-cell tiles, each top voxel followed by its walls) and the DDA's lane
+   pass's (Hilbert-ordered 16-cell tiles, each top voxel followed by its walls) and the DDA's lane
    order (blocks of 512 ids sorted by parent-cell height, voxel height).  No reference counterpart;
    for tests and tooling. */
 int sgpu_get_orders(sgpu_ctx* ctx, int32_t* order, int32_t* lane_order);
@@ -265,6 +265,10 @@ int sgpu_mlp_param_count(void);
 int sgpu_mlp_get_params(sgpu_mlp* mlp, float* params);
 int sgpu_mlp_set_params(sgpu_mlp* mlp, const float* params);
 int sgpu_mlp_set_loss_weights(sgpu_mlp* mlp, float w0, float w1);
+/* Inference kernel (new): rows = 32 or 64 runs the fused kernel with that many voxels per
+   workgroup (activations kept on chip through all layers; the default is 64), rows = 0 the
+   layer-by-layer GEMMs.  Both give bit-identical outputs. */
+int sgpu_mlp_set_inference(sgpu_mlp* mlp, int rows);
 /* X: n x 64 bf16 (host), Y: n x 2 (host). */
 int sgpu_mlp_forward(sgpu_mlp* mlp, const uint16_t* X, int64_t n, float* Y);
 /* X, Y device pointers; *ms = kernel time (HIP events). */

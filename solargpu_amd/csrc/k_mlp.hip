@@ -193,6 +193,91 @@ __global__ __launch_bounds__(256) void k_out(OutArgs o) {
     }
 }
 
+/* Fused inference: one workgroup takes ROWS voxels through all five layers.  The activations
+ * stay in LDS (two ping-pong tiles of ROWS x 256 bf16, rows padded by 16 bytes so the 16 rows of
+ * an MFMA A fragment fall in different banks); the weights stream from L2.  Wave w computes
+ * output columns 64w..64w+63 of every layer with the MFMA steps, bias, GELU and bf16 rounding of
+ * k_gemm<kEpiForward>, and the 256 -> 2 output layer is k_out's per-row loop, so the outputs equal
+ * the layer-by-layer path bit for bit. */
+constexpr int kActPitch = kHidden + 8;
+
+__device__ __forceinline__ bf16x8 lds8(const uint16_t* p) {
+    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, q);
+}
+
+template <int ROWS>
+__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs f) {
+    static_assert(ROWS == 32 || ROWS == 64, "tile rows");
+    constexpr int MI = ROWS / 16;   // 16-row MFMA blocks per wave
+    __shared__ __align__(16) uint16_t act[2][ROWS * kActPitch];
+    __shared__ float w5[2][kHidden];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, q = lane >> 4;
+    const int64_t row0 = (int64_t)blockIdx.x * ROWS;
+    for (int i = threadIdx.x; i < 2 * kHidden; i += 256) w5[i / kHidden][i % kHidden] = bf2f(f.W5[i]);
+    for (int i = threadIdx.x; i < ROWS * (kInPad / 8); i += 256) {
+        const int rr = i / (kInPad / 8), cc = (i % (kInPad / 8)) * 8;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (row0 + rr < f.B) v = *reinterpret_cast<const uint4*>(f.X + (row0 + rr) * kInPad + cc);
+        *reinterpret_cast<uint4*>(act[0] + rr * kActPitch + cc) = v;
+    }
+    __syncthreads();
+    const int n0 = wave * 64;
+    int cur = 0;
+    for (int l = 0; l < 4; ++l) {
+        const int K = (l == 0) ? kInPad : kHidden;
+        const uint16_t* W = f.W[l];
+        f32x4 acc[MI][4];
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k = 0; k < K; k += 32) {
+            bf16x8 a[MI], b[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = load8(W + (int64_t)(n0 + 16 * j + r) * K + k + 8 * q);
+#pragma unroll
+            for (int i = 0; i < MI; ++i) a[i] = lds8(act[cur] + (16 * i + r) * kActPitch + k + 8 * q);
+#pragma unroll
+            for (int i = 0; i < MI; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        uint16_t* out = act[cur ^ 1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + 16 * j + r;
+            const float bias = f.b[l][n];
+#pragma unroll
+            for (int i = 0; i < MI; ++i)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) out[(16 * i + 4 * q + e) * kActPitch + n] = f2bf(gelu(acc[i][j][e] + bias));
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    /* output layer: one lane per row, k_out's arithmetic */
+    for (int rr = threadIdx.x; rr < ROWS; rr += 256) {
+        const int64_t b = row0 + rr;
+        if (b >= f.B) continue;
+        const uint16_t* h = act[cur] + rr * kActPitch;
+        float y0 = f.b5[0], y1 = f.b5[1];
+        for (int i = 0; i < kHidden; i += 8) {
+            const uint4 qv = *reinterpret_cast<const uint4*>(h + i);
+            const uint32_t wv[4] = {qv.x, qv.y, qv.z, qv.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float lo = __uint_as_float(wv[e] << 16), hi = __uint_as_float(wv[e] & 0xFFFF0000u);
+                y0 += lo * w5[0][i + 2 * e] + hi * w5[0][i + 2 * e + 1];
+                y1 += lo * w5[1][i + 2 * e] + hi * w5[1][i + 2 * e + 1];
+            }
+        }
+        f.Y[2 * b] = y0;
+        f.Y[2 * b + 1] = y1;
+    }
+}
+
 /* db[n] = sum over the batch of dZt[n][.] (one wave per n). */
 __global__ __launch_bounds__(64) void k_bias_grad(const uint16_t* dZt, int ldt, int B, float* gb) {
     const int n = blockIdx.x, lane = threadIdx.x;
@@ -242,6 +327,17 @@ void launch_out(const OutArgs& o, bool train, hipStream_t s) {
     const unsigned g = (unsigned)((o.B + 255) / 256);
     if (train) hipLaunchKernelGGL(k_out<true>, dim3(g), dim3(256), 0, s, o);
     else hipLaunchKernelGGL(k_out<false>, dim3(g), dim3(256), 0, s, o);
+    SGPU_HIP(hipGetLastError());
+}
+
+void launch_infer(const InferArgs& f, int rows, hipStream_t s) {
+    if (f.B <= 0) return;
+    if (rows == 32)
+        hipLaunchKernelGGL(k_mlp_infer<32>, dim3((unsigned)((f.B + 31) / 32)), dim3(256), 0, s, f);
+    else if (rows == 64)
+        hipLaunchKernelGGL(k_mlp_infer<64>, dim3((unsigned)((f.B + 63) / 64)), dim3(256), 0, s, f);
+    else
+        fail(SGPU_ERR_ARG, "inference tiles of 32 or 64 rows");
     SGPU_HIP(hipGetLastError());
 }
 

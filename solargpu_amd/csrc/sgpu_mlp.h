@@ -55,6 +55,19 @@ struct OutArgs {
 void launch_out(const OutArgs& o, bool train, hipStream_t s);
 void launch_bias_grad(const uint16_t* dZt, int ldt, int B, int n, float* gb, hipStream_t s);
 
+/* Fused inference of B rows: X [B][64] bf16 -> Y [B][2] fp32 (k_mlp_infer, `rows` voxels per
+   workgroup: 32 or 64). */
+struct InferArgs {
+    const uint16_t* X;
+    const uint16_t* W[4];   // hidden layer l: [256][in_l] bf16
+    const float* b[4];      // [256]
+    const uint16_t* W5;     // [2][256] bf16
+    const float* b5;        // [2]
+    int64_t B;
+    float* Y;
+};
+void launch_infer(const InferArgs& f, int rows, hipStream_t s);
+
 struct AdamArgs {
     float* w;               // fp32 master
     float* m;

@@ -71,6 +71,7 @@ struct sgpu_mlp {
     float* tb = nullptr;
     double* loss = nullptr;
     float out_wt[2] = {1.0f, 1.0f};
+    int infer_rows = 64;   // inference: fused kernel with tiles of 32 or 64 rows; 0 = layer by layer
 
     void free_work() {
         dfree(xb); dfree(xbt); dfree(tb); dfree(loss);
@@ -138,8 +139,26 @@ void upload_layer(sgpu_mlp* m, int li, const float* w, const float* b) {
     SGPU_HIP(hipStreamSynchronize(m->stream));
 }
 
+InferArgs infer_args(const sgpu_mlp* m, const uint16_t* x, int64_t B, float* y_dev) {
+    InferArgs f{};
+    f.X = x;
+    for (int l = 0; l < 4; ++l) {
+        f.W[l] = m->L[l].wb;
+        f.b[l] = m->L[l].b;
+    }
+    f.W5 = m->L[4].wb;
+    f.b5 = m->L[4].b;
+    f.B = B;
+    f.Y = y_dev;
+    return f;
+}
+
 /* Forward over B rows of x (device, [B][64]).  Training also keeps Z and H^T. */
 void forward(sgpu_mlp* m, const uint16_t* x, int B, bool train, float* y_dev) {
+    if (!train && m->infer_rows) {
+        launch_infer(infer_args(m, x, B, y_dev), m->infer_rows, m->stream);
+        return;
+    }
     const uint16_t* a = x;
     int lda = kInPad;
     for (int l = 0; l < 4; ++l) {
@@ -353,13 +372,21 @@ int sgpu_mlp_set_loss_weights(sgpu_mlp* m, float w0, float w1) {
     });
 }
 
+int sgpu_mlp_set_inference(sgpu_mlp* m, int rows) {
+    return mlp_guard([&] {
+        if (!m) fail(SGPU_ERR_ARG, "null argument");
+        if (rows != 0 && rows != 32 && rows != 64) fail(SGPU_ERR_ARG, "rows must be 0, 32 or 64");
+        m->infer_rows = rows;
+    });
+}
+
 /* Inference: X [n][64] bf16 (host), Y [n][2] (host). */
 int sgpu_mlp_forward(sgpu_mlp* m, const uint16_t* X, int64_t n, float* Y) {
     return mlp_guard([&] {
         if (!m || !X || !Y || n < 0) fail(SGPU_ERR_ARG, "bad argument");
         SGPU_HIP(hipSetDevice(m->device));
         const int chunk = 65536;
-        m->ensure_work(chunk);
+        if (!m->infer_rows) m->ensure_work(chunk);
         uint16_t* xd = dnew<uint16_t>((size_t)chunk * kInPad);
         float* yd = dnew<float>((size_t)chunk * 2);
         for (int64_t r = 0; r < n; r += chunk) {
@@ -380,14 +407,18 @@ int sgpu_mlp_forward_device(sgpu_mlp* m, const uint16_t* X, int64_t n, float* Y,
         if (!m || !X || !Y) fail(SGPU_ERR_ARG, "bad argument");
         SGPU_HIP(hipSetDevice(m->device));
         const int chunk = 1 << 18;
-        m->ensure_work((int)std::min<int64_t>(chunk, std::max<int64_t>(n, 64)));
+        if (!m->infer_rows) m->ensure_work((int)std::min<int64_t>(chunk, std::max<int64_t>(n, 64)));
         hipEvent_t a, b;
         SGPU_HIP(hipEventCreate(&a));
         SGPU_HIP(hipEventCreate(&b));
         SGPU_HIP(hipEventRecord(a, m->stream));
-        for (int64_t r = 0; r < n; r += m->cap) {
-            const int B = (int)std::min<int64_t>(m->cap, n - r);
-            forward(m, X + r * kInPad, B, false, Y + 2 * r);
+        if (m->infer_rows) {
+            launch_infer(infer_args(m, X, n, Y), m->infer_rows, m->stream);   // every row in one launch
+        } else {
+            for (int64_t r = 0; r < n; r += m->cap) {
+                const int B = (int)std::min<int64_t>(m->cap, n - r);
+                forward(m, X + r * kInPad, B, false, Y + 2 * r);
+            }
         }
         SGPU_HIP(hipEventRecord(b, m->stream));
         SGPU_HIP(hipEventSynchronize(b));

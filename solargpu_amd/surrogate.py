@@ -90,6 +90,10 @@ class MLP:
     def set_loss_weights(self, w0: float, w1: float) -> None:
         _mcheck(_lib.sgpu_mlp_set_loss_weights(self._h, ctypes.c_float(w0), ctypes.c_float(w1)))
 
+    def set_inference(self, rows: int) -> None:
+        """Inference kernel: 32 or 64 voxels per workgroup of the fused kernel, 0 = layer by layer."""
+        _mcheck(_lib.sgpu_mlp_set_inference(self._h, rows))
+
     def forward(self, X_bf16: np.ndarray) -> np.ndarray:
         X = _c(X_bf16, np.uint16)
         n = X.shape[0]

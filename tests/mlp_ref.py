@@ -30,6 +30,23 @@ def forward(layers, X):
     return y, (cache, zs)
 
 
+def _bf16(x):
+    """Round to bfloat16 (nearest even) and back, as the GPU stores activations."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(layers, X):
+    """forward() with every hidden activation rounded to bfloat16, as the library keeps them
+    between layers; products and sums stay float64."""
+    a = X.astype(np.float64)
+    for W, b in layers[:-1]:
+        a = _bf16(gelu(a @ W.astype(np.float64).T + b))
+    W, b = layers[-1]
+    return a @ W.astype(np.float64).T + b
+
+
 def loss_and_grad(layers, X, T, wt=(1.0, 1.0)):
     B = X.shape[0]
     y, (acts, zs) = forward(layers, X)

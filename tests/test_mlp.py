@@ -55,7 +55,28 @@ def test_gpu_forward_matches_fp64_reference():
     y_gpu = mlp.forward(Xb)
     y_ref, _ = mlp_ref.forward(layers, sm.from_bf16(Xb))
     scale = np.abs(y_ref).max()
+    # the library keeps hidden activations in bf16: against exact float64 activations the outputs
+    # differ by that rounding (a few 2^-8 relative); against a float64 forward that rounds them the
+    # same way, by fp32 accumulation, erff vs erf and the odd activation rounded the other way
     assert np.abs(y_gpu - y_ref).max() <= 2e-2 * scale + 1e-4
+    y_b = mlp_ref.forward_bf16(layers, sm.from_bf16(Xb))
+    assert np.abs(y_gpu - y_b).max() <= 2e-3 * np.abs(y_b).max()
+
+
+@pytest.mark.gpu
+def test_fused_inference_equals_layer_by_layer():
+    """The fused inference kernel (activations kept on chip, tiles of 32 or 64 voxels) and the
+    layer-by-layer GEMMs give the same outputs bit for bit, including a partial last tile."""
+    rng = np.random.default_rng(7)
+    mlp = sm.MLP(seed=8)
+    Xb = _inputs(rng, 3001)
+    mlp.set_inference(0)
+    y_layers = mlp.forward(Xb)
+    for rows in (32, 64):
+        mlp.set_inference(rows)
+        np.testing.assert_array_equal(mlp.forward(Xb), y_layers)
+    with pytest.raises(Exception):
+        mlp.set_inference(48)
 
 
 @pytest.mark.gpu
