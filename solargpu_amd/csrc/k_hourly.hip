@@ -607,7 +607,8 @@ __device__ __forceinline__ int64_t queue_position(uint64_t item, unsigned x, int
     return (k * kXcds + (int64_t)x) * band + o;
 }
 
-/* Queued assignment (full-scene order mode): a persistent grid whose waves each take the next
+/* Queued assignment (the full scene in processing order, or a subset of voxels given in that
+ * order): a persistent grid whose waves each take the next
  * receiver from a queue of their XCD (block b is on XCD b % 8), so that at any moment the
  * resident waves of an XCD work on a compact window of the processing order and share its
  * neighbours' rows in the L2.  An XCD's positions are split over kQueuesPerXcd interleaved
@@ -633,7 +634,9 @@ void k_reflect_queue(HourArgs a) {
     const unsigned x = blockIdx.x % kXcds;
     const unsigned c = ((blockIdx.x / kXcds) * WAVES + (unsigned)wave) % kQueuesPerXcd;
     unsigned* counter = a.queue + (x * kQueuesPerXcd + c) * kQueueStride;
-    const int64_t n = a.n_total;
+    /* positions: the full scene in processing order (voxel order[p]; lists and sums by voxel), or a
+       subset (voxel subset[p]; lists and sums by position) */
+    const int64_t n = a.order ? a.n_total : a.n_sub;
     const float inv_w = 1.0f / (float)a.W;
     const int cells = a.W * a.H;
     const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);
@@ -648,15 +651,16 @@ void k_reflect_queue(HourArgs a) {
         const int64_t p = queue_position((uint64_t)item * kQueuesPerXcd + c, x, n, a.queue_band);
         if (p >= n) break;
         pending = atomicAdd(counter, lane == 0 ? 1u : 0u);   // the next item, while this one is worked
-        const int64_t v = a.order[p];
+        const int64_t v = a.order ? (int64_t)a.order[p] : (int64_t)a.subset[p];
+        const int64_t q = a.order ? v : p;
         const Recv r = receiver_of(a, v, inv_w);
-        const int64_t s0 = uniform64(a.nb_off[v]), s1 = uniform64(a.nb_off[v + 1]);
+        const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
         win.end = s1;
         float own, refl;
         receiver_parts<K, FMT, WAVES, kQueueStageBytes>(a, hrs, r, v, s0, s1, win, L, wave, lane, off32, inv_w, cells,
                                                         own, refl);
         const float t = wave_sum(own + refl);
-        if (lane == 0) kahan_add(a.sum, a.comp, v, t);
+        if (lane == 0) kahan_add(a.sum, a.comp, q, t);
     }
 }
 
@@ -790,7 +794,10 @@ void reflect_queue_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
 
 template <int K>
 void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
-    const bool queued = a.queue != nullptr && a.order != nullptr && a.subset == nullptr && a.n_sub == 0;
+    /* queued: the full scene in processing order, or a subset given by voxel ids */
+    const bool queued = a.queue != nullptr && a.dbg_hourly == nullptr &&
+                        ((a.order != nullptr && a.subset == nullptr && a.n_sub == 0) ||
+                         (a.order == nullptr && a.subset != nullptr && a.n_sub > 0));
     if (queued) {
         if (a.waves == 2) reflect_queue_kw<K, 2>(a, fmt, s);
         else reflect_queue_kw<K, 4>(a, fmt, s);

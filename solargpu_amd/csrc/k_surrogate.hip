@@ -405,9 +405,18 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const int32_t* ids, const f
     }
 }
 
-__global__ __launch_bounds__(256) void k_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids) {
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (v < n && flags[v]) ids[off[v]] = (int32_t)v;
+/* flags by position p: flags of voxel order[p] (order NULL: p itself) */
+__global__ __launch_bounds__(256) void k_permute_flags(const int32_t* flags, const int32_t* order, int64_t n,
+                                                       int32_t* pflags) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) pflags[p] = flags[order ? (int64_t)order[p] : p];
+}
+
+/* ids[off[p]] = the voxel at position p for every flagged position */
+__global__ __launch_bounds__(256) void k_compact(const int32_t* pflags, const int32_t* order, const int64_t* off,
+                                                 int64_t n, int32_t* ids) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n && pflags[p]) ids[off[p]] = order ? order[p] : (int32_t)p;
 }
 
 }  // namespace
@@ -453,9 +462,15 @@ void launch_scatter_sum(const int32_t* ids, const float* vals, const float* base
     hipLaunchKernelGGL(k_scatter_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ids, vals, base, n, out);
     SGPU_HIP(hipGetLastError());
 }
-void launch_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids, hipStream_t s) {
+void launch_permute_flags(const int32_t* flags, const int32_t* order, int64_t n, int32_t* pflags, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, off, n, ids);
+    hipLaunchKernelGGL(k_permute_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, order, n, pflags);
+    SGPU_HIP(hipGetLastError());
+}
+void launch_compact(const int32_t* pflags, const int32_t* order, const int64_t* off, int64_t n, int32_t* ids,
+                    hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pflags, order, off, n, ids);
     SGPU_HIP(hipGetLastError());
 }
 

@@ -53,6 +53,10 @@ void launch_scatter(const int32_t* ids, const float* vals, int64_t n, float* out
 /* ids[off[v]] = v for every flagged v (off = exclusive scan of the flags). */
 void launch_scatter_sum(const int32_t* ids, const float* vals, const float* base, int64_t n, float* out,
                         hipStream_t s);
-void launch_compact(const int32_t* flags, const int64_t* off, int64_t n, int32_t* ids, hipStream_t s);
+/* The flagged voxels in processing order (order NULL: voxel order): pflags[p] = flags[order[p]];
+   after the scan of pflags into off, ids[off[p]] = order[p] for every flagged position. */
+void launch_permute_flags(const int32_t* flags, const int32_t* order, int64_t n, int32_t* pflags, hipStream_t s);
+void launch_compact(const int32_t* pflags, const int32_t* order, const int64_t* off, int64_t n, int32_t* ids,
+                    hipStream_t s);
 
 }  // namespace sgpu

@@ -156,6 +156,7 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, sgpu_mlp* error_mlp, const sg
         float* Yerr = nullptr;
         float* out = nullptr;
         int32_t* flags = nullptr;
+        int32_t* pflags = nullptr;
         int64_t* off = nullptr;
         int64_t* scr = nullptr;
         int32_t* ids = nullptr;
@@ -165,6 +166,7 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, sgpu_mlp* error_mlp, const sg
             if (error_mlp) Yerr = dalloc<float>((size_t)N * 2);
             out = dalloc<float>(N);
             flags = dalloc<int32_t>(N);
+            pflags = dalloc<int32_t>(N);
             off = dalloc<int64_t>(N + 1);
             scr = dalloc<int64_t>(scan_scratch_elems(N));
             SGPU_HIP(hipStreamSynchronize(c->stream));
@@ -187,7 +189,10 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, sgpu_mlp* error_mlp, const sg
             a.diff_miss_weight = gate->diff_miss_weight;
             a.share_abs_err = gate->share_abs_err;
             launch_gate(a, c->stream);
-            launch_scan_counts(flags, N, off, scr, c->stream);
+            /* the gated voxels in the reflection pass's processing order, so consecutive receivers of
+               the fallback are neighbours and share their gathered rows in the L2 */
+            launch_permute_flags(flags, c->order, N, pflags, c->stream);
+            launch_scan_counts(pflags, N, off, scr, c->stream);
             int64_t nf = 0;
             SGPU_HIP(hipMemcpyAsync(&nf, off + N, sizeof nf, hipMemcpyDeviceToHost, c->stream));
             SGPU_HIP(hipStreamSynchronize(c->stream));
@@ -196,7 +201,7 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, sgpu_mlp* error_mlp, const sg
             if (nf > 0) {
                 ids = dalloc<int32_t>(nf);
                 exact = dalloc<float>(nf);
-                launch_compact(flags, off, N, ids, c->stream);
+                launch_compact(pflags, c->order, off, N, ids, c->stream);
                 L = subset_viewshed(c, ids, nf, true);
                 if (annual) {
                     subset_reflected_linear(c, ids, nf, L, SGPU_FMT_BARLE, c->sur.dd, exact);
@@ -216,11 +221,11 @@ int sgpu_surrogate_run(sgpu_ctx* c, sgpu_mlp* mlp, sgpu_mlp* error_mlp, const sg
             s->t_total_ms = all.ms();
         } catch (...) {
             L.release();
-            dfree(Y); dfree(Yerr); dfree(out); dfree(flags); dfree(off); dfree(scr); dfree(ids); dfree(exact);
+            dfree(Y); dfree(Yerr); dfree(out); dfree(flags); dfree(pflags); dfree(off); dfree(scr); dfree(ids); dfree(exact);
             throw;
         }
         L.release();
-        dfree(Y); dfree(Yerr); dfree(out); dfree(flags); dfree(off); dfree(scr); dfree(ids); dfree(exact);
+        dfree(Y); dfree(Yerr); dfree(out); dfree(flags); dfree(pflags); dfree(off); dfree(scr); dfree(ids); dfree(exact);
     });
 }
 
