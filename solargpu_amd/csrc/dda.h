@@ -85,18 +85,22 @@ __device__ __forceinline__ bool pool_ray_steps(PoolRay& r, const float* __restri
         off[u] = inside[u] ? r.cx4 + r.ry4 : 0u;
         h[u] = *(const float*)((const char*)grid + off[u]);
     }
-    unsigned out = 0;
-    bool done = false;
+    /* the stopping step's cell offset and whether it is a hit (inside, below the exits); its id is
+       formed once, after the steps */
+    unsigned stop_off = 0;
+    bool hit = false, done = false;
 #pragma unroll
     for (int u = 0; u < kDdaUnroll; ++u) {
         if (COUNT) steps += done ? 0 : 1;
         const float zr = r.oz + r.cz * t[u];
         const bool zout = (zr > r.zhi) | (zr < r.zlo);
         const bool stop = !inside[u] | zout | (h[u] - zr > 0.01f);
-        out = (stop & !done) ? ((inside[u] & !zout) ? (off[u] >> 2) + 1u : 0u) : out;
+        const bool first = stop & !done;   // at most one step of the ray's pass is its first stop
+        stop_off = first ? off[u] : stop_off;
+        hit = hit | (first & inside[u] & !zout);
         done |= stop;
     }
-    res = out;
+    res = hit ? (stop_off >> 2) + 1u : 0u;
     return done;
 }
 
