@@ -348,6 +348,10 @@ int sgpu_batch_viewshed(sgpu_batch* b);
 /* Stage 2 for every tile (format CSR or BARLE, K hours per pass, 2 or 4 waves per reflection
    workgroup); graph != 0 records the loop as one hipGraph and launches that. */
 int sgpu_batch_hour_loop(sgpu_batch* b, unsigned format, int hours_per_pass, int waves, int graph);
+/* Reflection-pass receiver assignment of the batched hour loop (new): 1 = waves take receivers one
+   by one from per-XCD queues over every tile's positions; 0 (the default) = static blocks of the
+   concatenated tiles.  Both give bit-identical annual sums. */
+int sgpu_batch_set_receiver_queue(sgpu_batch* b, int on);
 int sgpu_batch_get_info(sgpu_batch* b, sgpu_batch_info* out);
 /* Per tile: sizes, then the reference-layout products (offsets rebased to the tile). */
 int sgpu_batch_tile_sizes(sgpu_batch* b, int32_t tile, int64_t* num_total, int64_t* nnz, int64_t* barle_bytes,

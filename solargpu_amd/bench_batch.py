@@ -4,7 +4,8 @@ Workload: 64 seeded tiles, side log-uniform in [128, 1024] cells of 1 m,
 building density U(10 %, 60 %), latitude U(30, 60), longitude U(-10, 30),
 synthetic weather per tile.  Batched: one persistent viewshed launch per voxel
 chunk over all tiles and a hipGraph hour loop with one solar and one reflection
-launch per pass.  Sequential: a context per tile (viewshed + hour loop), the
+launch per pass (static blocks of the concatenated tiles; the assignment from
+per-XCD receiver queues is timed too).  Sequential: a context per tile (viewshed + hour loop), the
 same launch shapes.  Both on the GPU (HIP events); prints one JSON line.
 
     python -m solargpu_amd.bench_batch [--tiles 64] [--fmt barle]
@@ -45,7 +46,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--tiles", type=int, default=64)
     ap.add_argument("--fmt", choices=("barle", "csr"), default="barle")
-    ap.add_argument("--hours-per-pass", type=int, default=128)
+    ap.add_argument("--hours-per-pass", type=int, default=256)
     ap.add_argument("--waves", type=int, default=4)
     ap.add_argument("--no-sequential", action="store_true")
     args = ap.parse_args(argv)
@@ -61,6 +62,10 @@ def main(argv=None):
             b.add_tile(sc, az, dni, dhi)
         b.viewshed()
         b.hour_loop(fmt, K, waves, graph=True)   # warm-up (graph build, first touch)
+        b.set_receiver_queue(True)                # receivers from per-XCD queues, for comparison
+        b.hour_loop(fmt, K, waves, graph=True)
+        queued_ms = b.info()["t_hours_ms"]
+        b.set_receiver_queue(False)
         b.viewshed()
         b.hour_loop(fmt, K, waves, graph=True)
         bi = b.info()
@@ -73,14 +78,15 @@ def main(argv=None):
            "batched": {"viewshed_ms": bi["t_viewshed_ms"], "trace_ms": bi["t_trace_ms"],
                        "lists_ms": bi["t_lists_ms"], "encode_ms": bi["t_encode_ms"], "hours_ms": bi["t_hours_ms"],
                        "dda_rays_per_s": n_vox * 2592 / (bi["t_viewshed_ms"] / 1e3),
-                       "voxel_hours_per_s": bi["voxel_hours"] / (bi["t_hours_ms"] / 1e3)}}
+                       "voxel_hours_per_s": bi["voxel_hours"] / (bi["t_hours_ms"] / 1e3),
+                       "hours_ms_queued_assignment": queued_ms}}
     if not args.no_sequential:
         tv = th = 0.0
         for k, (sc, az, dni, dhi) in enumerate(tiles):
             with sg.Context(0) as ctx:
                 ctx.set_scene(sc)
                 ctx.set_sky(az, dni, dhi)
-                ctx.set_tuning(32, K, waves, 256, 32)   # the batched launches' shapes
+                ctx.set_tuning(32, K, waves, 256, 0, 1)   # the batched launches' shapes (queued receivers)
                 ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
                 tv += ctx.info()["t_viewshed_ms"]
                 ctx.hour_loop(fmt, K)

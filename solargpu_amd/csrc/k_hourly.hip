@@ -715,6 +715,78 @@ __global__ __launch_bounds__(64 * WAVES) void k_reflect_batch(const BatchHour* d
     reflect_block<K, FMT, false, WAVES>(a, (unsigned)((int64_t)b - d[t].block0));
 }
 
+/* The tile owning batch position p: the last tile whose first position is <= p (wave-uniform). */
+__device__ __forceinline__ int tile_of_position(const BatchHour* d, int ntiles, int64_t p) {
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].pos0 <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+/* Batched tiles, queued: k_reflect_queue over the concatenated positions of every tile (tile t owns
+ * [pos0, pos0 + n_total), its voxel order[p - pos0]).  A wave takes a position from its XCD's
+ * queue, finds the tile and runs the shared per-receiver routine on that tile's pass (the tile's
+ * hour constants read from global memory: tiles have their own hours); tiles whose hours are done
+ * are passed over.  The stream window stays valid across tiles: every tile's offsets index the
+ * batch's one stream buffer. */
+template <int K, int FMT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(7)))
+void k_reflect_batch_queue(const BatchHour* d, int ntiles, int pass, unsigned* queue, int64_t n, int64_t band) {
+    __shared__ WaveLds<FMT, WAVES, kQueueStageBytes> L;
+    __shared__ HourArgs targs[WAVES];   // each wave's current tile and pass, read field by field on use
+    static_assert(sizeof(HourArgs) % 4 == 0, "copied as words");
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const unsigned x = blockIdx.x % kXcds;
+    const unsigned c = ((blockIdx.x / kXcds) * WAVES + (unsigned)wave) % kQueuesPerXcd;
+    unsigned* counter = queue + (x * kQueuesPerXcd + c) * kQueueStride;
+    StreamWindow win{L.stage[wave], d[0].a.nb_bytes, 0, 0, 0, WaveLds<FMT, WAVES, kQueueStageBytes>::kCap};
+    unsigned pending = atomicAdd(counter, lane == 0 ? 1u : 0u);   // every lane joins (see k_reflect_queue)
+    /* the current tile and its positions [t_lo, t_hi), searched again only when a position falls
+       outside them (consecutive items of a wave mostly stay in one tile); its arguments are re-read
+       per receiver (wave-uniform loads that hit the scalar cache) */
+    int64_t t_lo = 0, t_hi = 0;
+    bool live = false;   // the loaded tile has hours in this pass
+    const HourArgs& a = targs[wave];
+    for (int64_t turn = 0; turn <= n; ++turn) {
+        const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)pending);
+        const int64_t p = queue_position((uint64_t)item * kQueuesPerXcd + c, x, n, band);
+        if (p >= n) break;
+        pending = atomicAdd(counter, lane == 0 ? 1u : 0u);
+        if (p < t_lo || p >= t_hi) {
+            const int t = tile_of_position(d, ntiles, p);
+            t_lo = d[t].pos0;
+            t_hi = t_lo + d[t].a.n_total;
+            live = pass * K < d[t].nsel;
+            wave_lds_sync();   // the wave's reads of the previous tile's arguments are done
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(&d[t].a);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(&targs[wave]);
+            for (int i = lane; i < (int)(sizeof(HourArgs) / 4); i += 64) dst[i] = src[i];
+            wave_lds_sync();
+            if (live && lane == 0) {   // tile_pass_args: this pass's hours
+                targs[wave].hours = d[t].a.hours + pass * K;
+                targs[wave].nh = min(K, d[t].nsel - pass * K);
+            }
+            wave_lds_sync();
+        }
+        if (!live) continue;
+        const float inv_w = 1.0f / (float)a.W;
+        const int cells = a.W * a.H;
+        const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);
+        const int64_t v = a.order[p - t_lo];
+        const Recv r = receiver_of(a, v, inv_w);
+        const int64_t s0 = uniform64(a.nb_off[v]), s1 = uniform64(a.nb_off[v + 1]);
+        win.end = s1;
+        float own, refl;
+        receiver_parts<K, FMT, WAVES, kQueueStageBytes>(a, a.hours, r, v, s0, s1, win, L, wave, lane, off32, inv_w,
+                                                        cells, own, refl);
+        const float tt = wave_sum(own + refl);
+        if (lane == 0) kahan_add(a.sum, a.comp, v, tt);
+    }
+}
+
 template <int K>
 void solar_top_k(const HourArgs& a, hipStream_t s) {
     const unsigned g = (unsigned)((a.n_top + kSolarBlock - 1) / kSolarBlock);
@@ -733,10 +805,18 @@ void reflect_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
     else hipLaunchKernelGGL((k_reflect<K, SGPU_FMT_BARLE, DBG, WAVES>), dim3(g), blk, 0, s, a);
 }
 
+template <int K, int WAVES>
+void batch_queue_kw(const BatchHour* d, int ntiles, unsigned fmt, int pass, unsigned* queue, int64_t n, hipStream_t s);
+
 template <int K>
 void batch_k(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, unsigned fmt, int waves, int pass,
-             hipStream_t s) {
+             unsigned* queue, int64_t n_all, hipStream_t s) {
     hipLaunchKernelGGL(k_solar_top_batch<K>, dim3((unsigned)top_blocks), dim3(kSolarBlock), 0, s, d, ntiles, pass);
+    if (queue) {
+        if (waves == 2) batch_queue_kw<K, 2>(d, ntiles, fmt, pass, queue, n_all, s);
+        else batch_queue_kw<K, 4>(d, ntiles, fmt, pass, queue, n_all, s);
+        return;
+    }
     blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // whole XCD sets for xcd_block
     if (fmt == SGPU_FMT_CSR) {
         if (waves == 2)
@@ -789,6 +869,24 @@ void reflect_queue_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
         static const GridShape g = occupancy(k_reflect_queue<K, SGPU_FMT_BARLE, WAVES>, 64 * WAVES);
         hipLaunchKernelGGL((k_reflect_queue<K, SGPU_FMT_BARLE, WAVES>), dim3(queue_grid(g, a.queue_blocks_per_cu)),
                            dim3(64 * WAVES), 0, s, a);
+    }
+}
+
+/* XCD bands of the queued batched reflection: 65536 positions (what the single-scene tuning picks
+   on the 2048^2 scene: bands of 256 workgroups of 4 waves x 64) */
+constexpr int64_t kBatchQueueBand = 65536;
+
+template <int K, int WAVES>
+void batch_queue_kw(const BatchHour* d, int ntiles, unsigned fmt, int pass, unsigned* queue, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(kXcds * kQueuesPerXcd), 0, s, queue);
+    if (fmt == SGPU_FMT_CSR) {
+        static const GridShape g = occupancy(k_reflect_batch_queue<K, SGPU_FMT_CSR, WAVES>, 64 * WAVES);
+        hipLaunchKernelGGL((k_reflect_batch_queue<K, SGPU_FMT_CSR, WAVES>), dim3(queue_grid(g, 0)), dim3(64 * WAVES),
+                           0, s, d, ntiles, pass, queue, n, kBatchQueueBand);
+    } else {
+        static const GridShape g = occupancy(k_reflect_batch_queue<K, SGPU_FMT_BARLE, WAVES>, 64 * WAVES);
+        hipLaunchKernelGGL((k_reflect_batch_queue<K, SGPU_FMT_BARLE, WAVES>), dim3(queue_grid(g, 0)),
+                           dim3(64 * WAVES), 0, s, d, ntiles, pass, queue, n, kBatchQueueBand);
     }
 }
 
@@ -869,20 +967,20 @@ void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s)
 }
 
 void launch_hour_pass_batch(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, int K,
-                            unsigned fmt, int waves, int pass, hipStream_t s) {
+                            unsigned fmt, int waves, int pass, unsigned* queue, int64_t n_all, hipStream_t s) {
     if (ntiles <= 0 || blocks <= 0) return;
     if (fmt != SGPU_FMT_CSR && fmt != SGPU_FMT_BARLE) fail(SGPU_ERR_ARG, "format must be CSR or BARLE");
     if (waves != 2 && waves != 4) fail(SGPU_ERR_ARG, "waves per group must be 2 or 4");
     switch (K) {
-        case 1: batch_k<1>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
-        case 2: batch_k<2>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
-        case 4: batch_k<4>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
-        case 8: batch_k<8>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
-        case 16: batch_k<16>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
-        case 32: batch_k<32>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
-        case 64: batch_k<64>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
-        case 128: batch_k<128>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
-        case 256: batch_k<256>(d, ntiles, top_blocks, blocks, fmt, waves, pass, s); break;
+        case 1: batch_k<1>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 2: batch_k<2>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 4: batch_k<4>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 8: batch_k<8>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 16: batch_k<16>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 32: batch_k<32>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 64: batch_k<64>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 128: batch_k<128>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 256: batch_k<256>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());

@@ -277,9 +277,12 @@ struct BatchHour {
     HourArgs a;          // hours: the tile's first sunlit-hour constant; nh unused
     int nsel;            // the tile's sunlit hours
     int64_t block0, top_block0;
+    int64_t pos0;        // the tile's first position in the batch's voxel space (queued reflection)
 };
+/* queue == NULL: static assignment (blocks of the concatenated tiles); else the queued kernel over
+   the n_all positions of the batch (kQueueWords counters, zeroed by the launcher). */
 void launch_hour_pass_batch(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, int K,
-                            unsigned fmt, int waves, int pass, hipStream_t s);
+                            unsigned fmt, int waves, int pass, unsigned* queue, int64_t n_all, hipStream_t s);
 void launch_reflect_accum(const HourArgs& a, int K, unsigned format, hipStream_t s);
 bool supported_hours_per_pass(int K);
 

@@ -30,13 +30,17 @@ def _tiles():
     return out
 
 
-@pytest.mark.parametrize("fmt,K,waves", [(sg.FMT_BARLE, 64, 4), (sg.FMT_CSR, 16, 2), (sg.FMT_BARLE, 128, 2)])
-def test_batch_matches_single_tile_contexts(fmt, K, waves):
+@pytest.mark.parametrize("fmt,K,waves,queued", [(sg.FMT_BARLE, 64, 4, True), (sg.FMT_CSR, 16, 2, True),
+                                                (sg.FMT_BARLE, 128, 2, False), (sg.FMT_BARLE, 256, 4, False)])
+def test_batch_matches_single_tile_contexts(fmt, K, waves, queued):
+    """Queued (receivers one by one from per-XCD queues over every tile) and static receiver
+    assignments both give each tile's own annual sums bit for bit."""
     tiles = _tiles()
     with sg.Batch(0) as b:
         for sc, sky in tiles:
             b.add_tile(sc, *sky)
         b.viewshed()
+        b.set_receiver_queue(queued)
         b.hour_loop(fmt, K, waves, graph=True)
         inf = b.info()
         assert inf["tiles"] == len(tiles) and inf["passes"] >= 1
