@@ -68,3 +68,32 @@ def loss_and_grad(layers, X, T, wt=(1.0, 1.0)):
 
 def flat(grads):
     return np.concatenate([np.concatenate([gW.ravel(), gb.ravel()]) for gW, gb in grads])
+
+
+def loss_and_grad_bf16(layers, X, T, wt=(1.0, 1.0)):
+    """loss_and_grad() with bfloat16 rounding where the library's trainer stores values: the
+    hidden pre-activations Z and activations H after each layer, and every dZ of the backward
+    pass (dZ = dH * gelu'(Z) from the stored Z); weights enter as given (bf16 copies), products
+    and sums stay float64.  The weight gradient of layer l is dZ_l^T H_(l-1) with both rounded."""
+    B = X.shape[0]
+    a = X.astype(np.float64)
+    acts, zbs = [a], []
+    for W, b in layers[:-1]:
+        z = a @ W.astype(np.float64).T + b
+        zbs.append(_bf16(z))
+        a = _bf16(gelu(z))
+        acts.append(a)
+    W5, b5 = layers[-1]
+    y = a @ W5.astype(np.float64).T + b5
+    w = np.asarray(wt, np.float64)
+    e = y - T
+    loss = float((w * e * e).sum() / B)
+    g = 2.0 * e * w / B
+    grads = [None] * len(layers)
+    grads[-1] = (g.T @ acts[-1], g.sum(0))
+    dz = _bf16((g @ W5.astype(np.float64)) * gelu_grad(zbs[-1]))
+    for l in range(len(layers) - 2, -1, -1):
+        grads[l] = (dz.T @ acts[l], dz.sum(0))
+        if l > 0:
+            dz = _bf16((dz @ layers[l][0].astype(np.float64)) * gelu_grad(zbs[l - 1]))
+    return loss, grads

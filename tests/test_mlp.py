@@ -33,6 +33,23 @@ def test_reference_gradient_matches_finite_differences():
     np.testing.assert_allclose(g, np.array(num), rtol=1e-6, atol=1e-9)
 
 
+def test_bf16_aware_reference_reduces_to_the_exact_one(monkeypatch):
+    """With bf16 rounding replaced by the identity, the rounding-aware forward and gradient are
+    the float64 reference (pinned by finite differences above) exactly: the same terms in the
+    same places, only the storage rounding differs."""
+    rng = np.random.default_rng(3)
+    layers = _small_layers(rng, [6, 5, 5, 5, 5, 2])
+    X = rng.normal(size=(9, 6))
+    T = rng.normal(size=(9, 2))
+    monkeypatch.setattr(mlp_ref, "_bf16", lambda x: np.asarray(x, np.float64))
+    loss_b, grads_b = mlp_ref.loss_and_grad_bf16(layers, X, T, (1.0, 0.5))
+    loss, grads = mlp_ref.loss_and_grad(layers, X, T, (1.0, 0.5))
+    assert loss_b == loss
+    np.testing.assert_array_equal(mlp_ref.flat(grads_b), mlp_ref.flat(grads))
+    y, _ = mlp_ref.forward(layers, X)
+    np.testing.assert_array_equal(mlp_ref.forward_bf16(layers, X), y)
+
+
 def test_bf16_round_trip():
     x = np.array([1.0, -2.5, 3.14159265, 1e-3, 65504.0, 0.0], np.float32)
     y = sm.from_bf16(sm.to_bf16(x))
@@ -81,6 +98,10 @@ def test_fused_inference_equals_layer_by_layer():
 
 @pytest.mark.gpu
 def test_gpu_gradient_matches_fp64_reference():
+    """The HIP backward (MFMA GEMMs, fused GELU' epilogue, split-batch partial sums) against two
+    float64 references: one that rounds Z, H and dZ to bfloat16 where the trainer stores them
+    (measured: 2e-5 .. 4e-4 relative per tensor, loss 2e-6), and the exact one (measured up to
+    5e-3: the bf16 storage itself)."""
     rng = np.random.default_rng(2)
     mlp = sm.MLP(seed=4)
     mlp.set_loss_weights(1.0, 0.5)
@@ -89,18 +110,20 @@ def test_gpu_gradient_matches_fp64_reference():
     Xb = _inputs(rng, 512)
     T = rng.normal(0, 0.5, (512, 2)).astype(np.float32)
     g_gpu, loss_gpu = mlp.gradient(Xb, T)
+    loss_b, grads_b = mlp_ref.loss_and_grad_bf16(layers, sm.from_bf16(Xb), T, (1.0, 0.5))
     loss_ref, grads = mlp_ref.loss_and_grad(layers, sm.from_bf16(Xb), T, (1.0, 0.5))
-    assert abs(loss_gpu - loss_ref) <= 2e-2 * loss_ref
+    assert abs(loss_gpu - loss_b) <= 1e-5 * loss_b
+    assert abs(loss_gpu - loss_ref) <= 1e-4 * loss_ref
     pos = 0
-    for li, ((gW, gb), (o, i)) in enumerate(zip(grads, sm.layer_shapes())):
+    for li, ((gW, gb), (bW, bb), (o, i)) in enumerate(zip(grads, grads_b, sm.layer_shapes())):
         got_W = g_gpu[pos:pos + o * i].reshape(o, i)
         pos += o * i
         got_b = g_gpu[pos:pos + o]
         pos += o
-        for name, got, ref in (("W", got_W, gW), ("b", got_b, gb)):
-            cos = float((got.ravel() @ ref.ravel()) / (np.linalg.norm(got) * np.linalg.norm(ref) + 1e-30))
+        for name, got, ref, ref_b in (("W", got_W, gW, bW), ("b", got_b, gb, bb)):
+            rel_b = np.linalg.norm(got - ref_b) / (np.linalg.norm(ref_b) + 1e-30)
             rel = np.linalg.norm(got - ref) / (np.linalg.norm(ref) + 1e-30)
-            assert cos > 0.999 and rel < 5e-2, (li, name, cos, rel)
+            assert rel_b < 2e-3 and rel < 2e-2, (li, name, rel_b, rel)
     assert pos == g_gpu.size
 
 
