@@ -745,8 +745,9 @@ void k_reflect_batch_queue(const BatchHour* d, int ntiles, int pass, unsigned* q
     StreamWindow win{L.stage[wave], d[0].a.nb_bytes, 0, 0, 0, WaveLds<FMT, WAVES, kQueueStageBytes>::kCap};
     unsigned pending = atomicAdd(counter, lane == 0 ? 1u : 0u);   // every lane joins (see k_reflect_queue)
     /* the current tile and its positions [t_lo, t_hi), searched again only when a position falls
-       outside them (consecutive items of a wave mostly stay in one tile); its arguments are re-read
-       per receiver (wave-uniform loads that hit the scalar cache) */
+       outside them (consecutive items of a wave mostly stay in one tile); its pass arguments are
+       copied into the wave's LDS slot then and read from there field by field (held in registers
+       across the receiver routine they spilled to scratch) */
     int64_t t_lo = 0, t_hi = 0;
     bool live = false;   // the loaded tile has hours in this pass
     const HourArgs& a = targs[wave];
