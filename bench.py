@@ -194,6 +194,8 @@ def main():
                     help="G,K,waves,band,group[,queue]: use this launch shape instead of autotuning (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer drop-in timing")
+    ap.add_argument("--trace-tables", action="store_true",
+                    help="trace the viewsheds by step tables (sgpu_set_trace_tables) instead of the pooled walk")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -229,6 +231,8 @@ def main():
         return a, b
 
     tuning = None
+    if args.trace_tables:
+        ctx.set_trace_tables(True)
     if args.tuning:
         vals = [int(x) for x in args.tuning.split(",")]
         ctx.set_tuning(*vals)
@@ -301,7 +305,8 @@ def main():
     # same workload; the launch time live (HIP events around each chunk's trace launch).
     chunks = recs[-1][0]["viewshed_chunks"]
     avg_trace = t_tr / (chunks * args.steps)
-    valu_per_launch = read_profile("trace_pool_c2_valu.txt", "valu_insts_per_launch")
+    # (the committed summary is the pooled walk's; with --trace-tables the profile figures do not apply)
+    valu_per_launch = None if args.trace_tables else read_profile("trace_pool_c2_valu.txt", "valu_insts_per_launch")
     dda_roof = {"bound": "valu", "unit": "wave64 VALU instructions/s", "peak": valu_peak,
                 "peak_probes": {"fp32_fma": valu_fma, "int_add_compare_select": valu_mix,
                                 "int_add_compare_to_sgpr": valu_cmp},
@@ -315,6 +320,17 @@ def main():
                 "hbm_traffic_gbps_profiled": read_profile("trace_pool_c2_valu.txt", "hbm_traffic_gbps")}
     dda_roof["frac"] = dda_roof["achieved"] / valu_peak if dda_roof["achieved"] else None
     dda_roof["frac_nominal"] = dda_roof["achieved"] / VALU_NOMINAL_PEAK if dda_roof["achieved"] else None
+    # the DDA's second bound: L1 tag lookups of its scattered height gathers (one per lane whose cell
+    # shares no line with its quad's), per launch from the same PMC summary, against the probe's rate
+    # for lanes on distinct lines of an L1-resident buffer
+    l1_rate = ctx.l1_gather_rate(16 << 10, 5)
+    l1_per_launch = None if args.trace_tables else read_profile("trace_pool_c2_valu.txt", "l1_accesses_per_launch")
+    dda_roof["l1"] = {"bound": "l1 tag lookups", "unit": "lookups/s", "peak": l1_rate,
+                      "peak_source": "measured: every lane gathers 4-byte values from its own 128-byte line of a "
+                                     "16 KiB (L1-resident) buffer",
+                      "lookups_per_launch": l1_per_launch,
+                      "achieved": (l1_per_launch / avg_trace) if l1_per_launch else None}
+    dda_roof["l1"]["frac"] = dda_roof["l1"]["achieved"] / l1_rate if dda_roof["l1"]["achieved"] else None
     if not args.no_dropin:
         # the drop-in entry point end to end: host arrays in (PCIe), viewshed, hour loop, host out
         t0 = time.perf_counter()
@@ -347,7 +363,8 @@ def main():
         "config": {"workload": f"plain method at scale: {W}x{W} DSM at {cell} m, {args.density:.0%} buildings, "
                                f"{N} voxels ({ntop} top), {H} sunlit hours, ba-RLE neighbour streams",
                    "grid": W, "cell_m": cell, "voxels": N, "top_voxels": ntop, "sunlit_hours": H,
-                   "hours_per_pass": Kp, "nnz": inf["nnz"], "barle_bytes": inf["barle_bytes"]},
+                   "hours_per_pass": Kp, "nnz": inf["nnz"], "barle_bytes": inf["barle_bytes"],
+                   "trace": "step tables" if args.trace_tables else "pooled walk"},
         "dda_rays_per_s": rays,
         "dda_rays_per_s_trace_kernel": N * 2592 * args.steps / t_tr,
         "dda_cell_steps": dda_steps,
@@ -366,6 +383,7 @@ def main():
         "roofline_dda": dda_roof,
         "measured_copy_gbps": copy_gbps,
         "measured_l2_read_gbps": l2_gbps,
+        "measured_l1_gather_lookups_per_s": l1_rate,
         "measured_valu_wave_insts_per_s": {"fp32_fma": valu_fma, "int_add_compare_select": valu_mix,
                                            "int_add_compare_to_sgpr": valu_cmp},
     }

@@ -73,6 +73,14 @@ typedef struct sgpu_info {
     double t_refl_kernel_ms;    /* sum of those launch times */
     double t_solar_kernel_ms;   /* sum of solar-top launch times */
     int64_t dda_steps;          /* only when step counting is enabled */
+    /* the step-table trace of the last sgpu_viewshed (new): rays left to the pooled walk after
+       their table's steps, voxels whose cells are near a binade end (all their rays by the pooled
+       walk), chunks traced by tables, and chunks traced again by the pooled walk alone because the
+       pending rays overflowed their buffer */
+    int64_t table_pending_rays;
+    int64_t table_irregular_voxels;
+    int32_t table_chunks;
+    int32_t table_fallbacks;
 } sgpu_info;
 
 const char* sgpu_version(void);
@@ -210,6 +218,16 @@ int sgpu_set_kernel_timing(sgpu_ctx* ctx, int enable);
 int sgpu_set_hourly_limit(sgpu_ctx* ctx, int64_t floats);
 /* Count DDA steps in the trace kernel (slower; new). */
 int sgpu_set_step_counting(sgpu_ctx* ctx, int enable);
+/* Trace whole-scene viewsheds by step tables (enable = 1) or by the pooled walk alone (0, the
+   default: on the 2048^2 bench scene the tables are within 4% of it, not faster); the lists are
+   the same either way.  pending_cap > 0 caps the pending rays per chunk below the buffer (tests
+   of the overflow path); 0 = the buffer (chunk x 2592 / 8 rays).  (new; the reference traces
+   every ray on its own, viewshed_kernels.cu:7-59) */
+int sgpu_set_trace_tables(sgpu_ctx* ctx, int enable, int64_t pending_cap);
+/* Shape of the step-table trace (new): directions per workgroup x steps per table, one of
+   4 x 255 (the default, the fastest on the bench scene), 8 x 255, 8 x 126, 4 x 126, 16 x 126,
+   8 x 63.  Larger tables leave fewer rays to the pooled walk; smaller ones take less LDS. */
+int sgpu_set_trace_table_shape(sgpu_ctx* ctx, int dirs, int steps);
 /* Synchronise the context stream. */
 int sgpu_sync(sgpu_ctx* ctx);
 
@@ -252,6 +270,11 @@ int sgpu_valu_rate(sgpu_ctx* ctx, int mix, double* wave_insts_per_s);
    L2, beyond a CU's L1) as 1 KB rows of 16-byte lane loads, `reps` launches (new; the peak of
    the reflection pass's row gathers).  footprint: a multiple of 4 KiB in 64 KiB .. 64 MiB. */
 int sgpu_l2_bandwidth(sgpu_ctx* ctx, int64_t footprint, int reps, double* gbps);
+/* L1 gather rate in tag lookups/s (the unit of the TCP_TOTAL_CACHE_ACCESSES counter): every
+   lane of every wave loads 4-byte values from its own 128-byte line of a `footprint`-byte buffer,
+   `reps` launches (new; the second peak of the DDA, whose pooled lanes gather one height each
+   from scattered cells).  footprint: a power of two in 4 KiB .. 1 GiB (16 KiB stays in L1). */
+int sgpu_l1_gather_rate(sgpu_ctx* ctx, int64_t footprint, int reps, double* accesses_per_s);
 
 /* ---- per-voxel surrogate MLP (new; no reference counterpart) ----
    64 inputs (53 features + zero padding) -> 4 x 256 GELU -> 2 outputs; BF16 weights and

@@ -62,6 +62,8 @@ class Info(ctypes.Structure):
         ("hours_per_pass", ctypes.c_int32), ("timed_launches", ctypes.c_int32),
         ("t_refl_kernel_ms", ctypes.c_double), ("t_solar_kernel_ms", ctypes.c_double),
         ("dda_steps", ctypes.c_int64),
+        ("table_pending_rays", ctypes.c_int64), ("table_irregular_voxels", ctypes.c_int64),
+        ("table_chunks", ctypes.c_int32), ("table_fallbacks", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -351,6 +353,14 @@ class Context:
     def set_step_counting(self, on: bool) -> None:
         _check(_lib.sgpu_set_step_counting(self._h, int(on)))
 
+    def set_trace_tables(self, on: bool, pending_cap: int = 0) -> None:
+        """Whole-scene viewsheds by step tables or by the pooled walk alone (default); pending_cap
+        > 0 caps the pending rays per chunk (forces the overflow path in tests)."""
+        _check(_lib.sgpu_set_trace_tables(self._h, int(on), _I64(pending_cap)))
+
+    def set_trace_table_shape(self, dirs: int = 4, steps: int = 255) -> None:
+        _check(_lib.sgpu_set_trace_table_shape(self._h, int(dirs), int(steps)))
+
     def viewshed(self, formats: int = FMT_CSR | FMT_BARLE) -> None:
         _check(_lib.sgpu_viewshed(self._h, ctypes.c_uint(formats)))
 
@@ -465,6 +475,12 @@ class Context:
         """GB/s of L2-resident reads: every wave streams a footprint-byte buffer as 1 KB rows."""
         g = ctypes.c_double()
         _check(_lib.sgpu_l2_bandwidth(self._h, _I64(footprint), reps, ctypes.byref(g)))
+        return g.value
+
+    def l1_gather_rate(self, footprint: int = 16 << 10, reps: int = 5) -> float:
+        """L1 tag lookups/s of scattered 4-byte gathers, every lane on its own 128-byte line."""
+        g = ctypes.c_double()
+        _check(_lib.sgpu_l1_gather_rate(self._h, _I64(footprint), reps, ctypes.byref(g)))
         return g.value
 
     def valu_rate(self, mix: int = 0) -> float:

@@ -41,16 +41,24 @@ struct PoolRay {
     float zhi, zlo;                // leaves above zhi (upper) or below zlo (lower)
 };
 
+/* Start cell (ix, iy) and first-edge distances (tmx, tmy) of the ray from integer cell (gx, gy)
+   along dc (viewshed_kernels.cu:27-35); the walk and the step tables both set up through it. */
+__device__ __forceinline__ void ray_start(const DirConst& dc, float gxf, float gyf, int& ix, int& iy, float& tmx,
+                                          float& tmy) {
+    const float ox = gxf + dc.cx * 2.0f;
+    const float oy = gyf + dc.cy * 2.0f;
+    ix = (int)floorf(ox);
+    iy = (int)floorf(oy);
+    tmx = div_by((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix), dc.ax, dc.rx);
+    tmy = div_by((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy), dc.ay, dc.ry);
+}
+
 /* Set-up of the ray from integer cell (gx, gy) at base_z along dc (viewshed_kernels.cu:27-35). */
 __device__ __forceinline__ void pool_ray_setup(PoolRay& r, const DirConst& dc, float gxf, float gyf, float base_z,
                                                unsigned uW, float maxz) {
-    const float ox = gxf + dc.cx * 2.0f;
-    const float oy = gyf + dc.cy * 2.0f;
+    int ix, iy;
+    ray_start(dc, gxf, gyf, ix, iy, r.tmx, r.tmy);
     r.oz = base_z + dc.cz * 2.0f;
-    const int ix = (int)floorf(ox);
-    const int iy = (int)floorf(oy);
-    r.tmx = div_by((dc.cx >= 0.0f) ? ((float)ix + 1.0f - ox) : (ox - (float)ix), dc.ax, dc.rx);
-    r.tmy = div_by((dc.cy >= 0.0f) ? ((float)iy + 1.0f - oy) : (oy - (float)iy), dc.ay, dc.ry);
     r.sx4 = (dc.cx >= 0.0f) ? 4u : (unsigned)-4;
     r.sy4 = (dc.cy >= 0.0f) ? 4u * uW : (unsigned)-(int)(4u * uW);
     r.cx4 = 4u * (unsigned)ix;

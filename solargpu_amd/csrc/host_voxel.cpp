@@ -95,6 +95,29 @@ std::vector<int32_t> trace_order(int64_t n_total, const float* grid, const int32
     return perm;
 }
 
+void class_order(int64_t n_total, int W, const int32_t* pp, std::vector<int32_t>& perm, std::vector<ClassRun>& runs) {
+    perm.assign((size_t)n_total, 0);
+    runs.clear();
+    std::vector<uint8_t> key((size_t)std::min<int64_t>(n_total, kClassBlock));
+    for (int64_t b = 0; b < n_total; b += kClassBlock) {
+        const int64_t e = std::min<int64_t>(n_total, b + kClassBlock);
+        int64_t count[256] = {0};
+        for (int64_t v = b; v < e; ++v) {
+            const int k = class_key(pp[v] % W, pp[v] / W);
+            key[(size_t)(v - b)] = (uint8_t)k;
+            ++count[k];
+        }
+        int64_t start[256];
+        int64_t at = b;
+        for (int k = 0; k < 256; ++k) {   // a counting sort: stable, so raster order within a class
+            start[k] = at;
+            if (count[k]) runs.push_back(ClassRun{at, (int32_t)count[k], k});
+            at += count[k];
+        }
+        for (int64_t v = b; v < e; ++v) perm[(size_t)start[key[(size_t)(v - b)]]++] = (int32_t)v;
+    }
+}
+
 std::vector<std::pair<int, int>> hilbert_tiles(int tw, int th) {
     int side = 1;
     while (side < std::max(tw, th)) side <<= 1;

@@ -73,6 +73,22 @@ __global__ __launch_bounds__(256) void k_l2_read(const float4* __restrict__ src,
     out[(int64_t)blockIdx.x * 256 + threadIdx.x] = acc.x + acc.y + acc.z + acc.w;
 }
 
+/* L1 gather-rate probe: every lane loads 4-byte values from its own 128-byte line of an
+   L1-resident buffer (a power-of-two float count `mask` + 1), 8 independent loads per loop turn,
+   addresses independent of the data: each wave instruction is 64 L1 tag lookups (the unit of
+   TCP_TOTAL_CACHE_ACCESSES), the access pattern of the DDA's pooled height gathers. */
+constexpr int kGatherTurns = 64;
+__global__ __launch_bounds__(256) void k_gather_probe(const float* __restrict__ buf, unsigned mask, float* out) {
+    const unsigned lane = threadIdx.x & 63, wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const unsigned base = ((wave * 2654435761u + lane * 40503u * 32u) & mask) & ~31u;
+    float acc = 0.0f;
+    for (int i = 0; i < kGatherTurns; ++i) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += buf[(base + (unsigned)(i * 8 + u) * 7919u * 32u) & mask];
+    }
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
 /* VALU issue-rate probe: eight independent FMA chains per lane, 16 x 8 = 128 FMAs per loop turn
  * (the loop's own counter and branch are scalar instructions), so every wave issues 128 VALU
  * instructions per turn: the calibration kernel for the VALU-bound DDA. */
@@ -161,10 +177,13 @@ int64_t choose_chunk(const sgpu_ctx* c, int64_t n) {
     size_t free_b = 0, total_b = 0;
     SGPU_HIP(hipMemGetInfo(&free_b, &total_b));
     free_b += c->ws_bytes;   // the kept workspace is reused or replaced
-    const double per_voxel = (double)kDirs * 4.0 + 32.0;
+    /* + the step-table trace's pending rays (up to one in eight, 8 bytes each) and voxel records */
+    const double per_voxel = (double)kDirs * 5.0 + 48.0;
     int64_t chunk = (int64_t)(0.4 * (double)free_b / per_voxel);
     chunk = std::min<int64_t>(chunk, (int64_t)1 << 20 << 2);
-    if (chunk > kTraceOrderBlock) chunk -= chunk % kTraceOrderBlock;   // chunks start on lane-order blocks
+    /* chunks start on lane-order blocks (kClassBlock is a multiple of kTraceOrderBlock) */
+    if (chunk > kClassBlock) chunk -= chunk % kClassBlock;
+    else if (chunk > kTraceOrderBlock) chunk -= chunk % kTraceOrderBlock;
     chunk = std::max<int64_t>(chunk, 1);
     return std::min<int64_t>(chunk, n);
 }
@@ -439,6 +458,74 @@ void host_take(sgpu_ctx* c, bool csr, bool barle, int64_t v0, int64_t cnt, const
     SGPU_HIP(hipStreamSynchronize(c->stream));
 }
 
+/* The first-hit table of chunk [v0, v0 + cnt) by the step-table trace: the regular classes' items,
+   then the pooled walk over the irregular voxels and the rays that outlived their table.  If the
+   pending rays overflow their buffer, the chunk is traced again by the pooled walk alone (the
+   step count restored first).  The chunk starts on a class block. */
+void trace_chunk_tables(sgpu_ctx* c, int64_t v0, int64_t cnt, int32_t* raw, unsigned long long* steps) {
+    std::vector<TableItem> items;
+    std::vector<int32_t> irr;
+    const auto lo = std::lower_bound(c->class_runs.begin(), c->class_runs.end(), v0,
+                                     [](const ClassRun& r, int64_t p) { return r.pos < p; });
+    for (auto r = lo; r != c->class_runs.end() && r->pos < v0 + cnt; ++r) {
+        const int32_t p0 = (int32_t)(r->pos - v0);
+        if (r->key == kClassIrregular) {
+            for (int32_t i = 0; i < r->count; ++i) irr.push_back(p0 + i);
+        } else {
+            for (int32_t o = 0; o < r->count; o += kTableItemVoxels)
+                items.push_back(TableItem{p0 + o, std::min(kTableItemVoxels, r->count - o), r->key, 0});
+        }
+    }
+    need(items.size() <= 65535, SGPU_ERR_ARG, "too many step-table items in one chunk");
+    TableItem* items_d = dalloc<TableItem>(items.size());
+    int32_t* irr_d = dalloc<int32_t>(irr.size());
+    try {
+        if (!items.empty())
+            SGPU_HIP(hipMemcpyAsync(items_d, items.data(), items.size() * sizeof(TableItem), hipMemcpyHostToDevice,
+                                    c->stream));
+        if (!irr.empty())
+            SGPU_HIP(hipMemcpyAsync(irr_d, irr.data(), irr.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        SGPU_HIP(hipMemsetAsync(c->ws_pend_count, 0, sizeof(unsigned long long), c->stream));
+        if (steps)
+            SGPU_HIP(hipMemcpyAsync(c->ws_pend_count + 1, steps, sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                                    c->stream));
+        const int64_t cap = c->pend_cap_override > 0 ? std::min(c->pend_cap_override, c->ws_pend_cap) : c->ws_pend_cap;
+        launch_table_records(c->class_perm, v0, cnt, c->pix_parent, c->grid, c->wall_z, c->n_top, c->W, c->ws_rec,
+                             c->stream);
+        TableArgs ta{c->grid, c->W,     c->H,   c->n_top,           c->pix_parent, c->wall_z, c->maxz, c->dirs,
+                     v0,      c->ws_rec, items_d, raw, steps, c->ws_pend_count, c->ws_pend, cap, c->ws_resume};
+        launch_trace_table(ta, (int)items.size(), c->table_dirs, c->table_steps, c->stream);
+        unsigned long long npend = 0;
+        SGPU_HIP(hipMemcpyAsync(&npend, c->ws_pend_count, sizeof npend, hipMemcpyDeviceToHost, c->stream));
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+        c->info.table_chunks += 1;
+        c->info.table_irregular_voxels += (int64_t)irr.size();
+        if (npend > (unsigned long long)cap) {
+            c->info.table_fallbacks += 1;
+            if (steps)
+                SGPU_HIP(hipMemcpyAsync(steps, c->ws_pend_count + 1, sizeof(unsigned long long),
+                                        hipMemcpyDeviceToDevice, c->stream));
+            TraceArgs t{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, cnt, raw,
+                        steps, nullptr, c->tune.dirs_per_group, c->trace_perm};
+            launch_trace(t, c->stream);
+        } else {
+            c->info.table_pending_rays += (int64_t)npend;
+            PendingArgs pa{c->grid, c->W,       c->H,       c->n_top,        c->pix_parent,
+                           c->wall_z, c->maxz,  c->dirs,    v0,              c->class_perm,
+                           irr_d,   (int64_t)irr.size(), c->ws_pend, (int64_t)npend, c->ws_resume,
+                           raw,     steps};
+            launch_trace_pending(pa, c->stream);
+        }
+        SGPU_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        dfree(items_d);
+        dfree(irr_d);
+        throw;
+    }
+    dfree(items_d);
+    dfree(irr_d);
+}
+
 /* host_csr / host_barle: send every chunk's lists to pinned host memory instead of keeping
    them on the device (then out.off / out.ids stay null).  barle_only: encode every chunk's
    lists to ba-RLE at once and keep only the streams (out.br_*), so the whole CSR never
@@ -479,12 +566,38 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
         /* the height-sorted lane order permutes within blocks of kTraceOrderBlock ids from 0:
            usable when every chunk starts on a block (whole scene, chunk a multiple of it) */
         const int32_t* perm = (!subset && (chunk >= N || chunk % kTraceOrderBlock == 0)) ? c->trace_perm : nullptr;
+        /* the step-table trace likewise needs chunks on class blocks */
+        const bool tables = c->trace_tables && !subset && c->class_perm && N == c->n_total &&
+                            (chunk >= N || chunk % kClassBlock == 0) && c->W < 65536 && c->H < 65536;
+        if (tables) {
+            const int64_t need_cap = std::max<int64_t>(chunk * kDirs / 8, 1);
+            if (c->ws_pend_cap < need_cap) {
+                dfree(c->ws_pend);
+                c->ws_pend_cap = 0;
+                c->ws_pend = dalloc<uint2>((size_t)need_cap);
+                c->ws_pend_cap = need_cap;
+                c->ws_bytes += (size_t)need_cap * sizeof(uint2);
+            }
+            if (c->ws_rec_cap < chunk) {
+                dfree(c->ws_rec);
+                c->ws_rec_cap = 0;
+                c->ws_rec = dalloc<int4>((size_t)chunk);
+                c->ws_rec_cap = chunk;
+                c->ws_bytes += (size_t)chunk * sizeof(int4);
+            }
+            if (!c->ws_pend_count) c->ws_pend_count = dalloc<unsigned long long>(2);
+            if (!c->ws_resume) c->ws_resume = dalloc<TableResume>((size_t)kTableClasses * kDirs);
+        }
         for (int64_t v0 = 0; v0 < N; v0 += chunk) {
             const int64_t cnt = std::min(chunk, N - v0);
-            TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, cnt, raw,
-                         steps, subset, c->tune.dirs_per_group, perm};
             SGPU_HIP(hipEventRecord(ev_trace.a, c->stream));
-            launch_trace(ta, c->stream);
+            if (tables) {
+                trace_chunk_tables(c, v0, cnt, raw, steps);
+            } else {
+                TraceArgs ta{c->grid, c->W, c->H, c->n_top, c->pix_parent, c->wall_z, c->maxz, c->dirs, v0, cnt, raw,
+                             steps, subset, c->tune.dirs_per_group, perm};
+                launch_trace(ta, c->stream);
+            }
             SGPU_HIP(hipEventRecord(ev_trace.b, c->stream));
             ListArgs la{raw, v0, cnt, c->n_total, c->dirs, c->sky.svf_bytes, svf, counts, off, nullptr};
             SGPU_HIP(hipEventRecord(ev_lists.a, c->stream));
@@ -686,6 +799,8 @@ void do_viewshed(sgpu_ctx* c, unsigned formats) {
     const int64_t N = c->n_total;
     c->info.t_trace_ms = c->info.t_lists_ms = 0.0;
     c->info.dda_steps = 0;
+    c->info.table_pending_rays = c->info.table_irregular_voxels = 0;
+    c->info.table_chunks = c->info.table_fallbacks = 0;
     EventPair all, ev_enc;
     SGPU_HIP(hipEventRecord(all.a, c->stream));
     c->svf = dalloc<uint8_t>((size_t)std::max(c->sky.svf_bytes, 1) * N);
@@ -1096,6 +1211,16 @@ int sgpu_get_info(sgpu_ctx* c, sgpu_info* out) {
 
 namespace {
 
+/* The step-table trace's class-sorted lane order and its runs, from the voxels' parent cells. */
+void upload_class_order(sgpu_ctx* c, const int32_t* pp) {
+    std::vector<int32_t> perm;
+    class_order(c->n_total, c->W, pp, perm, c->class_runs);
+    c->class_perm = dalloc<int32_t>(c->n_total);
+    SGPU_HIP(hipMemcpyAsync(c->class_perm, perm.data(), c->n_total * sizeof(int32_t), hipMemcpyHostToDevice,
+                            c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+}
+
 /* The reflection-pass processing order and the DDA lane order of the context's scene, built on
    the host from its heightmap, parents and wall heights, then uploaded. */
 void upload_orders(sgpu_ctx* c, const float* grid, const int32_t* pp, const float* wall_z) {
@@ -1107,6 +1232,7 @@ void upload_orders(sgpu_ctx* c, const float* grid, const int32_t* pp, const floa
     SGPU_HIP(hipMemcpyAsync(c->trace_perm, tperm.data(), c->n_total * sizeof(int32_t), hipMemcpyHostToDevice,
                             c->stream));
     SGPU_HIP(hipStreamSynchronize(c->stream));
+    upload_class_order(c, pp);
 }
 
 }  // namespace
@@ -1195,6 +1321,13 @@ int sgpu_set_scene_heights(sgpu_ctx* c, const float* heights, int w, int h, doub
             throw;
         }
         dfree(dv.wall_off);
+        {   /* the class order from the parents, on the host */
+            std::vector<int32_t> pp((size_t)n_total);
+            SGPU_HIP(hipMemcpyAsync(pp.data(), c->pix_parent, (size_t)n_total * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                    c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            upload_class_order(c, pp.data());
+        }
         c->scene = true;
     });
 }
@@ -1576,6 +1709,26 @@ int sgpu_set_step_counting(sgpu_ctx* c, int enable) {
     });
 }
 
+int sgpu_set_trace_tables(sgpu_ctx* c, int enable, int64_t pending_cap) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        need(pending_cap >= 0, SGPU_ERR_ARG, "pending_cap must be >= 0");
+        c->trace_tables = enable != 0;
+        c->pend_cap_override = pending_cap;
+    });
+}
+
+int sgpu_set_trace_table_shape(sgpu_ctx* c, int dirs, int steps) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        const bool built = (dirs == 8 && (steps == 255 || steps == 126 || steps == 63)) ||
+                           (dirs == 4 && (steps == 255 || steps == 126)) || (dirs == 16 && steps == 126);
+        need(built, SGPU_ERR_ARG, "step-table shape not built (8x255, 4x255, 8x126, 4x126, 16x126, 8x63)");
+        c->table_dirs = dirs;
+        c->table_steps = steps;
+    });
+}
+
 int sgpu_sync(sgpu_ctx* c) {
     return guarded([&] {
         need(c != nullptr, SGPU_ERR_ARG, "null context");
@@ -1643,6 +1796,42 @@ int sgpu_l2_bandwidth(sgpu_ctx* c, int64_t footprint, int reps, double* gbps) {
             throw;
         }
         dfree(src);
+        dfree(out);
+    });
+}
+
+int sgpu_l1_gather_rate(sgpu_ctx* c, int64_t footprint, int reps, double* accesses_per_s) {
+    return guarded([&] {
+        need(c && accesses_per_s && reps >= 1, SGPU_ERR_ARG, "bad arguments");
+        need(footprint >= 4096 && footprint <= ((int64_t)1 << 30) && (footprint & (footprint - 1)) == 0, SGPU_ERR_ARG,
+             "footprint must be a power of two in 4 KiB .. 1 GiB");
+        c->set_device();
+        int cus = 0;
+        SGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        const int blocks = std::max(cus, 1) * 8;   // 32 waves per CU
+        const int64_t n = footprint / 4;
+        float* buf = dalloc<float>((size_t)n);
+        float* out = nullptr;
+        try {
+            out = dalloc<float>((size_t)blocks * 256);
+            SGPU_HIP(hipMemsetAsync(buf, 0, (size_t)footprint, c->stream));
+            const unsigned mask = (unsigned)(n - 1);
+            hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, c->stream, buf, mask, out);   // warm-up
+            SGPU_HIP(hipGetLastError());
+            EventPair ev;
+            SGPU_HIP(hipEventRecord(ev.a, c->stream));
+            for (int r = 0; r < reps; ++r)
+                hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, c->stream, buf, mask, out);
+            SGPU_HIP(hipEventRecord(ev.b, c->stream));
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+            const double accesses = (double)blocks * 4.0 * kGatherTurns * 8.0 * 64.0 * reps;
+            *accesses_per_s = accesses / (ev.ms() * 1e-3);
+        } catch (...) {
+            dfree(buf);
+            dfree(out);
+            throw;
+        }
+        dfree(buf);
         dfree(out);
     });
 }

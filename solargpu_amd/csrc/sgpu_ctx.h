@@ -132,6 +132,11 @@ struct sgpu_ctx {
     float* aspects = nullptr;
     int32_t* order = nullptr;    // reflection-pass processing order (Hilbert-ordered cell tiles of kOrderTile, walls after their cell)
     int32_t* trace_perm = nullptr;   // DDA lane order (blocks of kTraceOrderBlock ids sorted by height)
+    int32_t* class_perm = nullptr;   // step-table lane order (blocks of kClassBlock ids sorted by traversal class)
+    std::vector<sgpu::ClassRun> class_runs;   // its runs of one class
+    bool trace_tables = false;       // sgpu_set_trace_tables: whole-scene viewsheds by the step-table trace
+    int64_t pend_cap_override = 0;   // pending-ray capacity per chunk (0: chunk x kDirs / 8)
+    int table_dirs = 4, table_steps = 255;   // step-table shape (sgpu_set_trace_table_shape)
     bool scene = false;
 
     bool sky_set = false;
@@ -171,6 +176,14 @@ struct sgpu_ctx {
     int64_t* ws_scratch = nullptr;
     int64_t ws_chunk = 0;
     size_t ws_bytes = 0;
+    /* the step-table trace's pending rays (8 bytes each), their counter and a saved step count,
+       and the classes' resume states */
+    uint2* ws_pend = nullptr;
+    int64_t ws_pend_cap = 0;
+    unsigned long long* ws_pend_count = nullptr;   // [0] pending rays, [1] saved step count
+    sgpu::TableResume* ws_resume = nullptr;
+    int4* ws_rec = nullptr;   // voxel records of a chunk in the class order
+    int64_t ws_rec_cap = 0;
 
     int64_t chunk_override = 0;
     bool kernel_timing = false;
@@ -205,6 +218,12 @@ struct sgpu_ctx {
         sgpu::dfree(ws_counts);
         sgpu::dfree(ws_off);
         sgpu::dfree(ws_scratch);
+        sgpu::dfree(ws_pend);
+        sgpu::dfree(ws_pend_count);
+        sgpu::dfree(ws_resume);
+        sgpu::dfree(ws_rec);
+        ws_rec_cap = 0;
+        ws_pend_cap = 0;
         ws_chunk = 0;
         ws_bytes = 0;
     }
@@ -217,6 +236,8 @@ struct sgpu_ctx {
         sgpu::dfree(aspects);
         sgpu::dfree(order);
         sgpu::dfree(trace_perm);
+        sgpu::dfree(class_perm);
+        class_runs.clear();
         sgpu::dfree(sum);
         sgpu::dfree(comp);
         sgpu::dfree(hourly_top);

@@ -125,6 +125,92 @@ struct TraceArgs {
 };
 void launch_trace(const TraceArgs& a, hipStream_t s);
 
+/* ---- the step-table trace (k_trace_table.hip) ----
+ * A cell coordinate g is in binade e when 2^e <= g < 2^(e+1).  When g also lies two or more
+ * from both ends of its binade (and e >= 3), g + 2c rounds to g + (2c rounded to the binade's
+ * spacing) for every |c| <= 1, so the ray set-up from g (start cell - g, first-edge distances)
+ * is the same for every such g: every cell of one (x binade, y binade) class walks the same
+ * step sequence along a direction.  Class key (ex - 3) * 16 + (ey - 3); kClassIrregular for
+ * the cells near a binade end or below 8. */
+constexpr int kClassIrregular = 255;
+constexpr int kClassBlock = 65536;   // the class-sorted lane order permutes within blocks of this many ids
+inline int class_of_coord(unsigned g) {
+    if (g < 8) return -1;
+    const int e = 31 - __builtin_clz(g);
+    const unsigned lo = 1u << e, hi = (2u << e) - 1u;
+    return (g - lo >= 2 && hi - g >= 2 && e <= 17) ? e : -1;
+}
+inline int class_key(int gx, int gy) {
+    const int ex = class_of_coord((unsigned)gx), ey = class_of_coord((unsigned)gy);
+    return (ex < 0 || ey < 0) ? kClassIrregular : (ex - 3) * 16 + (ey - 3);
+}
+/* A run of one class in the class-sorted lane order: positions [pos, pos + count). */
+struct ClassRun {
+    int64_t pos;
+    int32_t count, key;
+};
+/* Lane order sorted by class within blocks of kClassBlock ids (stable: raster order inside a
+   class), and its runs. */
+void class_order(int64_t n_total, int W, const int32_t* pix_parent, std::vector<int32_t>& perm,
+                 std::vector<ClassRun>& runs);
+/* An item of the table trace: chunk positions [pos, pos + count) of one regular class, at most
+   kTableItemVoxels (a multiple of the table workgroup's 256 lanes). */
+constexpr int kTableItemVoxels = 2048;
+struct TableItem {
+    int32_t pos, count, key, pad;
+};
+/* The resume state of a class and direction after its table's steps. */
+struct TableResume {
+    float tmx, tmy;
+    int32_t mx, my;   // x and y steps taken
+};
+/* table shapes built: directions per workgroup x steps per table (steps a multiple of the walk's
+   3-step unroll, below 2^16): 8 x 255, 4 x 255, 8 x 126, 4 x 126, 16 x 126, 8 x 63 */
+constexpr int kTableClasses = 256;
+struct TableArgs {
+    const float* grid;
+    int W, H;
+    int64_t n_top;
+    const int32_t* pix_parent;
+    const float* wall_z;
+    float maxz;
+    const DirConst* dirs;
+    int64_t v0;                  // chunk start (a multiple of kClassBlock)
+    const int4* rec;             // per chunk position: (gx | gy << 16, bits of the base z, raw row)
+    const TableItem* items;
+    int32_t* raw;                // the chunk's first-hit table
+    unsigned long long* steps;   // optional step counter
+    unsigned long long* pend_count;
+    uint2* pend;                 // (raw row, direction | key << 12) of the rays that outlive their table
+    int64_t pend_cap;
+    TableResume* resume;         // kTableClasses x kDirs
+};
+void launch_trace_table(const TableArgs& a, int n_items, int dirs, int steps, hipStream_t s);
+/* The records of chunk positions [0, n) of the class-sorted order (voxel perm[v0 + p]). */
+void launch_table_records(const int32_t* perm, int64_t v0, int64_t n, const int32_t* pix_parent, const float* grid,
+                          const float* wall_z, int64_t n_top, int W, int4* rec, hipStream_t s);
+/* The rest of the chunk's rays by the pooled walk: every direction of the irregular positions,
+   then the pending rays from their resume states. */
+struct PendingArgs {
+    const float* grid;
+    int W, H;
+    int64_t n_top;
+    const int32_t* pix_parent;
+    const float* wall_z;
+    float maxz;
+    const DirConst* dirs;
+    int64_t v0;
+    const int32_t* perm;
+    const int32_t* irregular;    // chunk positions of the irregular voxels
+    int64_t n_irregular;
+    const uint2* pend;
+    int64_t n_pend;
+    const TableResume* resume;
+    int32_t* raw;
+    unsigned long long* steps;
+};
+void launch_trace_pending(const PendingArgs& a, hipStream_t s);
+
 struct ListArgs {
     int32_t* raw;           // pass 1 leaves each voxel's kept ids at the front of its row
     int64_t v0, count, n_total;
