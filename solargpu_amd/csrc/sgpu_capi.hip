@@ -458,6 +458,21 @@ void host_take(sgpu_ctx* c, bool csr, bool barle, int64_t v0, int64_t cnt, const
     SGPU_HIP(hipStreamSynchronize(c->stream));
 }
 
+/* The step-table trace's class-sorted lane order and its runs, built on the host from the
+   scene's parent cells the first time a viewshed uses them. */
+void ensure_class_order(sgpu_ctx* c) {
+    if (c->class_perm) return;
+    std::vector<int32_t> pp((size_t)c->n_total), perm;
+    SGPU_HIP(hipMemcpyAsync(pp.data(), c->pix_parent, (size_t)c->n_total * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+    class_order(c->n_total, c->W, pp.data(), perm, c->class_runs);
+    c->class_perm = dalloc<int32_t>(c->n_total);
+    SGPU_HIP(hipMemcpyAsync(c->class_perm, perm.data(), c->n_total * sizeof(int32_t), hipMemcpyHostToDevice,
+                            c->stream));
+    SGPU_HIP(hipStreamSynchronize(c->stream));
+}
+
 /* The first-hit table of chunk [v0, v0 + cnt) by the step-table trace: the regular classes' items,
    then the pooled walk over the irregular voxels and the rays that outlived their table.  If the
    pending rays overflow their buffer, the chunk is traced again by the pooled walk alone (the
@@ -567,9 +582,10 @@ ListsOut build_lists(sgpu_ctx* c, const int32_t* subset, int64_t N, uint8_t* svf
            usable when every chunk starts on a block (whole scene, chunk a multiple of it) */
         const int32_t* perm = (!subset && (chunk >= N || chunk % kTraceOrderBlock == 0)) ? c->trace_perm : nullptr;
         /* the step-table trace likewise needs chunks on class blocks */
-        const bool tables = c->trace_tables && !subset && c->class_perm && N == c->n_total &&
+        const bool tables = c->trace_tables && !subset && N == c->n_total &&
                             (chunk >= N || chunk % kClassBlock == 0) && c->W < 65536 && c->H < 65536;
         if (tables) {
+            ensure_class_order(c);
             const int64_t need_cap = std::max<int64_t>(chunk * kDirs / 8, 1);
             if (c->ws_pend_cap < need_cap) {
                 dfree(c->ws_pend);
@@ -1211,16 +1227,6 @@ int sgpu_get_info(sgpu_ctx* c, sgpu_info* out) {
 
 namespace {
 
-/* The step-table trace's class-sorted lane order and its runs, from the voxels' parent cells. */
-void upload_class_order(sgpu_ctx* c, const int32_t* pp) {
-    std::vector<int32_t> perm;
-    class_order(c->n_total, c->W, pp, perm, c->class_runs);
-    c->class_perm = dalloc<int32_t>(c->n_total);
-    SGPU_HIP(hipMemcpyAsync(c->class_perm, perm.data(), c->n_total * sizeof(int32_t), hipMemcpyHostToDevice,
-                            c->stream));
-    SGPU_HIP(hipStreamSynchronize(c->stream));
-}
-
 /* The reflection-pass processing order and the DDA lane order of the context's scene, built on
    the host from its heightmap, parents and wall heights, then uploaded. */
 void upload_orders(sgpu_ctx* c, const float* grid, const int32_t* pp, const float* wall_z) {
@@ -1232,7 +1238,6 @@ void upload_orders(sgpu_ctx* c, const float* grid, const int32_t* pp, const floa
     SGPU_HIP(hipMemcpyAsync(c->trace_perm, tperm.data(), c->n_total * sizeof(int32_t), hipMemcpyHostToDevice,
                             c->stream));
     SGPU_HIP(hipStreamSynchronize(c->stream));
-    upload_class_order(c, pp);
 }
 
 }  // namespace
@@ -1321,13 +1326,6 @@ int sgpu_set_scene_heights(sgpu_ctx* c, const float* heights, int w, int h, doub
             throw;
         }
         dfree(dv.wall_off);
-        {   /* the class order from the parents, on the host */
-            std::vector<int32_t> pp((size_t)n_total);
-            SGPU_HIP(hipMemcpyAsync(pp.data(), c->pix_parent, (size_t)n_total * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                    c->stream));
-            SGPU_HIP(hipStreamSynchronize(c->stream));
-            upload_class_order(c, pp.data());
-        }
         c->scene = true;
     });
 }
