@@ -4,7 +4,8 @@
  * per launch; K <= 256).  Per pass:
  *   k_solar_top  one lane per top cell: direct (SVF-gated) + Perez diffuse for
  *                the K hours -> hourly_top[cell][K] (cell-major: one cell's K
- *                hours are one contiguous row).
+ *                hours are one contiguous row; written through LDS as whole
+ *                128-byte pieces of rows when K is a multiple of 32).
  *   k_reflect    one wave per 64 consecutive voxels (receivers).
  *                  phase 1, lane = voxel: receiver geometry and the voxel's own
  *                           direct + diffuse summed over the K hours;
@@ -91,9 +92,71 @@ __device__ __forceinline__ void solar_top_cell(const HourArgs& a, int64_t c) {
     }
 }
 
+/* Hours per LDS round of the wave form below: 32 hours of a cell are one 128-byte piece of its row. */
+constexpr int kSolarHours = 32;
+struct SolarLds {
+    float t[kSolarBlock / 64][64][kSolarHours + 4];   // per wave: 64 cells x 32 hours (rows padded by 16 B)
+};
+
+/* solar_top_cell for the 64 cells c0 .. c0 + 63 of a wave (K a multiple of kSolarHours): each lane
+   computes its cell's hours 32 at a time into LDS, then the wave stores them as whole 128-byte
+   pieces of the rows, 8 cells per store instruction (a lane's own 16-byte pieces of its row land in
+   64 rows 1 KB apart, which the memory system writes back as twice the bytes).  Every lane of the
+   wave takes part. */
+template <int K>
+__device__ __forceinline__ void solar_top_wave(const HourArgs& a, int64_t c0, float (*tile)[kSolarHours + 4]) {
+    static_assert(K % kSolarHours == 0, "whole 32-hour rounds");
+    const int lane = threadIdx.x & 63;
+    const int64_t c = c0 + lane;
+    const bool valid = c < a.n_top;
+    const int ncells = (int)min<int64_t>(64, a.n_top - c0);
+    float ss = 0.0f, cs = 1.0f, sa = 0.0f, ca = 1.0f;
+    if (valid) {
+        sincosf(a.slopes[c], &ss, &cs);
+        sincosf(a.aspects[c], &sa, &ca);
+    }
+    for (int h0 = 0; h0 < K; h0 += kSolarHours) {
+#pragma unroll
+        for (int u = 0; u < kSolarHours; u += 4) {
+            float out[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int h = h0 + u + w;
+                out[w] = 0.0f;
+                if (valid && h < a.nh) {
+                    const HourConst& hc = a.hours[h];
+                    out[w] = hour_irradiance(hc, cs, ss, ca, sa, svf_blocked(a.svf, c, hc.dense, a.n_total));
+                }
+            }
+            *reinterpret_cast<float4*>(&tile[lane][u]) = make_float4(out[0], out[1], out[2], out[3]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {   // 64 cells x 8 pieces of 16 bytes
+            const int j = r * 64 + lane, cell = j >> 3, part = j & 7;
+            if (cell < ncells) {
+                float* dst = a.hourly_top + (int64_t)row_slot((unsigned)(c0 + cell), a.row_swz) * K + h0 + part * 4;
+                *reinterpret_cast<float4*>(dst) = *reinterpret_cast<const float4*>(&tile[cell][part * 4]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // the tile is rewritten next round
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(kSolarBlock) void k_solar_top(HourArgs a) {
-    solar_top_cell<K>(a, (int64_t)blockIdx.x * kSolarBlock + threadIdx.x);
+    if constexpr (K % kSolarHours == 0) {
+        __shared__ SolarLds L;
+        const int wave = threadIdx.x >> 6;
+        const int64_t c0 = (int64_t)blockIdx.x * kSolarBlock + wave * 64;
+        if (c0 < a.n_top) solar_top_wave<K>(a, c0, L.t[wave]);
+    } else {
+        solar_top_cell<K>(a, (int64_t)blockIdx.x * kSolarBlock + threadIdx.x);
+    }
 }
 
 struct Recv {
@@ -697,7 +760,14 @@ __global__ __launch_bounds__(kSolarBlock) void k_solar_top_batch(const BatchHour
     const int t = tile_of_block(d, ntiles, blockIdx.x, true);
     HourArgs a;
     if (!tile_pass_args<K>(d[t], pass, a)) return;
-    solar_top_cell<K>(a, ((int64_t)blockIdx.x - d[t].top_block0) * kSolarBlock + threadIdx.x);
+    if constexpr (K % kSolarHours == 0) {
+        __shared__ SolarLds L;
+        const int wave = threadIdx.x >> 6;
+        const int64_t c0 = ((int64_t)blockIdx.x - d[t].top_block0) * kSolarBlock + wave * 64;
+        if (c0 < a.n_top) solar_top_wave<K>(a, c0, L.t[wave]);
+    } else {
+        solar_top_cell<K>(a, ((int64_t)blockIdx.x - d[t].top_block0) * kSolarBlock + threadIdx.x);
+    }
 }
 
 /* XCD band of the batched reflection (xcd_block): the XCDs walk consecutive bands of 256 blocks
