@@ -219,15 +219,17 @@ int sgpu_set_hourly_limit(sgpu_ctx* ctx, int64_t floats);
 /* Count DDA steps in the trace kernel (slower; new). */
 int sgpu_set_step_counting(sgpu_ctx* ctx, int enable);
 /* Trace whole-scene viewsheds by step tables (enable = 1) or by the pooled walk alone (0, the
-   default: on the 2048^2 bench scene the tables are within 4% of it, not faster); the lists are
-   the same either way.  pending_cap > 0 caps the pending rays per chunk below the buffer (tests
+   default: on the 2048^2 bench scene the tables run 3-7% behind it); the lists are the same
+   either way.  pending_cap > 0 caps the pending rays per chunk below the buffer (tests
    of the overflow path); 0 = the buffer (chunk x 2592 / 8 rays).  (new; the reference traces
    every ray on its own, viewshed_kernels.cu:7-59) */
 int sgpu_set_trace_tables(sgpu_ctx* ctx, int enable, int64_t pending_cap);
 /* Shape of the step-table trace (new): directions per workgroup x steps per table, one of
-   4 x 255 (the default, the fastest on the bench scene), 8 x 255, 8 x 126, 4 x 126, 16 x 126,
-   8 x 63.  Larger tables leave fewer rays to the pooled walk; smaller ones take less LDS. */
-int sgpu_set_trace_table_shape(sgpu_ctx* ctx, int dirs, int steps);
+   4 x 255 (the default), 4 x 510, 8 x 255, 8 x 126, 4 x 126, 16 x 126, 8 x 63.  Larger tables leave
+   fewer rays to the pooled walk; smaller ones take less LDS.  skip = 1 (the default) lets a ray
+   jump 9 steps at once when no cell among them can stop it as a hit (quadrant max maps of the
+   heightmap); the results are the same either way.  Step counting walks every step. */
+int sgpu_set_trace_table_shape(sgpu_ctx* ctx, int dirs, int steps, int skip);
 /* Synchronise the context stream. */
 int sgpu_sync(sgpu_ctx* ctx);
 

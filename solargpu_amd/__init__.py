@@ -358,8 +358,8 @@ class Context:
         > 0 caps the pending rays per chunk (forces the overflow path in tests)."""
         _check(_lib.sgpu_set_trace_tables(self._h, int(on), _I64(pending_cap)))
 
-    def set_trace_table_shape(self, dirs: int = 4, steps: int = 255) -> None:
-        _check(_lib.sgpu_set_trace_table_shape(self._h, int(dirs), int(steps)))
+    def set_trace_table_shape(self, dirs: int = 4, steps: int = 255, skip: bool = True) -> None:
+        _check(_lib.sgpu_set_trace_table_shape(self._h, int(dirs), int(steps), int(skip)))
 
     def viewshed(self, formats: int = FMT_CSR | FMT_BARLE) -> None:
         _check(_lib.sgpu_viewshed(self._h, ctypes.c_uint(formats)))

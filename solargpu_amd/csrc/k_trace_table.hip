@@ -27,6 +27,9 @@ namespace {
 
 constexpr int kTableBlock = 256;
 constexpr int kQuads = 16;          // lockstep quads of a wave: 4 lanes, 4 consecutive voxels, one direction
+/* Steps a ray may skip at once: the next kSkipSteps cells lie in the (kSkipSteps + 1)^2 cells ahead of
+   its current cell, whose highest height the quadrant's max map holds. */
+constexpr int kSkipSteps = kMaxMapWindow - 1;
 constexpr int kRefillQuads = 4;     // refill once this many quads are idle (or all of them)
 constexpr int kPendRefill = 16;
 constexpr int kPendPerWave = 2048;  // pending-walk rays per wave
@@ -40,6 +43,9 @@ struct TabDir {
     int cy;
     float c2z;        // RN(2 cz): the origin's z offset
     float zhi, zlo;   // a ray leaves above zhi or below zlo
+    int quad;         // the ray's quadrant: bit 0 toward +x, bit 1 toward +y (its max map)
+    int lowest;       // the step of a skip window whose z is lowest: 0 (rising ray) or kSkipSteps - 1
+    int pad[2];
 };
 
 template <int G, int S>
@@ -68,7 +74,8 @@ __device__ void build_tables(TableLds<G, S>& L, int key, int W, int H, float max
     const bool upper = dc.info & 1;
     /* x moves left inside: W - 1 - (gx + dix) toward +x, gx + dix toward -x; + 1 for the lookup */
     L.td[k] = TabDir{4 * dix + 4 * W * diy, px ? -1 : 1, px ? W - dix : dix + 1, py ? -1 : 1, py ? H - diy : diy + 1,
-                     dc.cz * 2.0f, upper ? maxz : INFINITY, upper ? -INFINITY : 0.0f};
+                     dc.cz * 2.0f, upper ? maxz : INFINITY, upper ? -INFINITY : 0.0f, (px ? 1 : 0) | (py ? 2 : 0),
+                     dc.cz >= 0.0f ? 0 : kSkipSteps - 1, {0, 0}};
     const int sx4 = px ? 4 : -4;
     const int sy4 = py ? 4 * W : -4 * W;
     int mx = 0, my = 0, off = 0;
@@ -98,6 +105,40 @@ __device__ void build_tables(TableLds<G, S>& L, int key, int W, int H, float max
 
 __device__ __forceinline__ int mbcnt64(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+/* The quadrant max maps, in two separable passes: along x, the highest of cells x, x + sx, ...,
+   x + (kMaxMapWindow - 1) sx of the row (sx = -1 for quadrants 0 and 2, +1 for 1 and 3); then along y
+   likewise over those maxima.  Cells off the grid do not count. */
+__global__ __launch_bounds__(256) void k_max_map_x(const float* grid, int W, int H, float* tmp) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t cells = (int64_t)W * H;
+    if (i >= 2 * cells) return;
+    const int sx = (i < cells) ? -1 : 1;
+    const int64_t c = i % cells;
+    const int x = (int)(c % W);
+    float m = -INFINITY;
+    for (int d = 0; d < kMaxMapWindow; ++d) {
+        const int xx = x + sx * d;
+        if (xx >= 0 && xx < W) m = fmaxf(m, grid[c - x + xx]);
+    }
+    tmp[i] = m;
+}
+__global__ __launch_bounds__(256) void k_max_map_y(const float* tmp, int W, int H, float* qmax) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t cells = (int64_t)W * H;
+    if (i >= 4 * cells) return;
+    const int q = (int)(i / cells);
+    const int64_t c = i % cells;
+    const float* src = tmp + ((q & 1) ? cells : 0);
+    const int sy = (q & 2) ? 1 : -1;
+    const int y = (int)(c / W);
+    float m = -INFINITY;
+    for (int d = 0; d < kMaxMapWindow; ++d) {
+        const int yy = y + sy * d;
+        if (yy >= 0 && yy < H) m = fmaxf(m, src[c + (int64_t)(yy - y) * W]);
+    }
+    qmax[i] = m;
 }
 
 /* One voxel of the class-sorted order: its cell (gx | gy << 16; grids below 2^16 a side), base
@@ -139,6 +180,7 @@ __global__ __launch_bounds__(kTableBlock) void k_trace_table(TableArgs a) {
     bool active = false, push = false;
     int k = 0, ti = 0, rem = 0, prow = 0;   // direction, table index k * S + s, steps left inside the grid - 1
     unsigned base = 0;                      // byte offset of the start cell (unsigned: grids reach 2^32 bytes)
+    unsigned pos = 0;                       // byte offset of the cell of the ray's last step
     float oz = 0.0f, zhi = 0.0f, zlo = 0.0f;
     int32_t* out = nullptr;
     unsigned long long steps = 0;
@@ -187,6 +229,7 @@ __global__ __launch_bounds__(kTableBlock) void k_trace_table(TableArgs a) {
                     prow = r.z;
                     out = a.raw + (int64_t)r.z * kDirs + rowbase + k;
                     ti = k * S;
+                    pos = base;
                     active = true;
                 }
             }
@@ -197,6 +240,12 @@ __global__ __launch_bounds__(kTableBlock) void k_trace_table(TableArgs a) {
             continue;
         }
         if (active) {
+            /* the next three steps' heights, and with them (same round trip) the max map of the cells
+               ahead: the next kSkipSteps cells cannot stop the ray as a hit when the highest of them is
+               no more than 0.01 above its lowest z in those steps (z is monotone along the ray, and so
+               is rounding); an exit among them is seen by the next step, which is then off the grid or
+               out of z too.  A ray whose last cell is off the grid (rem < 0) or out of z has left: no
+               skip, so the next step stops it. */
             int2 e[3];
             float h[3];
             bool ins[3];
@@ -207,6 +256,25 @@ __global__ __launch_bounds__(kTableBlock) void k_trace_table(TableArgs a) {
                 const unsigned off = ins[u] ? base + (unsigned)e[u].x : 0u;
                 h[u] = *(const float*)(gridb + off);
             }
+            bool skip = false;
+            if (!COUNT && a.qmax && ti != k * S && ti + kSkipSteps <= k * S + S && rem >= 0) {
+                const TabDir& t = L.td[k];
+                const float m = *(const float*)((const char*)(a.qmax + (int64_t)t.quad * a.map_cells) + pos);
+                const float zpos = oz + __int_as_float(L.ent[ti - 1].y);
+                const float zref = oz + __int_as_float(L.ent[ti + t.lowest].y);
+                skip = !(zpos > zhi) && !(zpos < zlo) && !(m - zref > 0.01f);
+            }
+            if (skip) {
+                pos = base + (unsigned)L.ent[ti + kSkipSteps - 1].x;
+                ti += kSkipSteps;
+                rem -= kSkipSteps;
+                if (ti - k * S >= S) {
+                    push = true;
+                    active = false;
+                }
+                continue;
+            }
+            pos = base + (unsigned)e[2].x;   // the last step's cell (inside the grid unless the ray stops)
             /* the first stopping step: its cell's byte offset when it is a hit, else ~0 (never a
                multiple of 4) */
             bool st[3];
@@ -318,12 +386,20 @@ void launch_trace_table(const TableArgs& a, int n_items, int dirs, int steps, hi
     }
     SGPU_TABLE_CASE(8, 255)
     SGPU_TABLE_CASE(4, 255)
+    SGPU_TABLE_CASE(4, 510)
     SGPU_TABLE_CASE(8, 126)
     SGPU_TABLE_CASE(4, 126)
     SGPU_TABLE_CASE(16, 126)
     SGPU_TABLE_CASE(8, 63)
 #undef SGPU_TABLE_CASE
     fail(SGPU_ERR_ARG, "step-table shape %d directions x %d steps is not built", dirs, steps);
+}
+
+void launch_max_maps(const float* grid, int W, int H, float* tmp, float* qmax, hipStream_t s) {
+    const int64_t cells = (int64_t)W * H;
+    hipLaunchKernelGGL(k_max_map_x, dim3((unsigned)((2 * cells + 255) / 256)), dim3(256), 0, s, grid, W, H, tmp);
+    hipLaunchKernelGGL(k_max_map_y, dim3((unsigned)((4 * cells + 255) / 256)), dim3(256), 0, s, tmp, W, H, qmax);
+    SGPU_HIP(hipGetLastError());
 }
 
 void launch_table_records(const int32_t* perm, int64_t v0, int64_t n, const int32_t* pix_parent, const float* grid,

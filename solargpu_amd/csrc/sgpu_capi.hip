@@ -461,6 +461,19 @@ void host_take(sgpu_ctx* c, bool csr, bool barle, int64_t v0, int64_t cnt, const
 /* The step-table trace's class-sorted lane order and its runs, built on the host from the
    scene's parent cells the first time a viewshed uses them. */
 void ensure_class_order(sgpu_ctx* c) {
+    if (!c->max_maps) {   // the quadrant max maps of the heightmap (skip tests)
+        const int64_t cells = (int64_t)c->W * c->H;
+        float* tmp = dalloc<float>((size_t)(2 * cells));
+        try {
+            c->max_maps = dalloc<float>((size_t)(4 * cells));
+            launch_max_maps(c->grid, c->W, c->H, tmp, c->max_maps, c->stream);
+            SGPU_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            dfree(tmp);
+            throw;
+        }
+        dfree(tmp);
+    }
     if (c->class_perm) return;
     std::vector<int32_t> pp((size_t)c->n_total), perm;
     SGPU_HIP(hipMemcpyAsync(pp.data(), c->pix_parent, (size_t)c->n_total * sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -508,7 +521,8 @@ void trace_chunk_tables(sgpu_ctx* c, int64_t v0, int64_t cnt, int32_t* raw, unsi
         launch_table_records(c->class_perm, v0, cnt, c->pix_parent, c->grid, c->wall_z, c->n_top, c->W, c->ws_rec,
                              c->stream);
         TableArgs ta{c->grid, c->W,     c->H,   c->n_top,           c->pix_parent, c->wall_z, c->maxz, c->dirs,
-                     v0,      c->ws_rec, items_d, raw, steps, c->ws_pend_count, c->ws_pend, cap, c->ws_resume};
+                     v0,      c->ws_rec, items_d, raw, steps, c->ws_pend_count, c->ws_pend, cap, c->ws_resume,
+                     c->table_skip ? c->max_maps : nullptr, (int64_t)c->W * c->H};
         launch_trace_table(ta, (int)items.size(), c->table_dirs, c->table_steps, c->stream);
         unsigned long long npend = 0;
         SGPU_HIP(hipMemcpyAsync(&npend, c->ws_pend_count, sizeof npend, hipMemcpyDeviceToHost, c->stream));
@@ -1716,14 +1730,15 @@ int sgpu_set_trace_tables(sgpu_ctx* c, int enable, int64_t pending_cap) {
     });
 }
 
-int sgpu_set_trace_table_shape(sgpu_ctx* c, int dirs, int steps) {
+int sgpu_set_trace_table_shape(sgpu_ctx* c, int dirs, int steps, int skip) {
     return guarded([&] {
         need(c != nullptr, SGPU_ERR_ARG, "null context");
         const bool built = (dirs == 8 && (steps == 255 || steps == 126 || steps == 63)) ||
-                           (dirs == 4 && (steps == 255 || steps == 126)) || (dirs == 16 && steps == 126);
-        need(built, SGPU_ERR_ARG, "step-table shape not built (8x255, 4x255, 8x126, 4x126, 16x126, 8x63)");
+                           (dirs == 4 && (steps == 510 || steps == 255 || steps == 126)) || (dirs == 16 && steps == 126);
+        need(built, SGPU_ERR_ARG, "step-table shape not built (8x255, 4x255, 4x510, 8x126, 4x126, 16x126, 8x63)");
         c->table_dirs = dirs;
         c->table_steps = steps;
+        c->table_skip = skip != 0;
     });
 }
 

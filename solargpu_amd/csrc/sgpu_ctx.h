@@ -137,6 +137,8 @@ struct sgpu_ctx {
     bool trace_tables = false;       // sgpu_set_trace_tables: whole-scene viewsheds by the step-table trace
     int64_t pend_cap_override = 0;   // pending-ray capacity per chunk (0: chunk x kDirs / 8)
     int table_dirs = 4, table_steps = 255;   // step-table shape (sgpu_set_trace_table_shape)
+    bool table_skip = true;                  // skip windows no cell of which can be hit (max maps)
+    float* max_maps = nullptr;               // the four quadrant max maps of the heightmap
     bool scene = false;
 
     bool sky_set = false;
@@ -237,6 +239,7 @@ struct sgpu_ctx {
         sgpu::dfree(order);
         sgpu::dfree(trace_perm);
         sgpu::dfree(class_perm);
+        sgpu::dfree(max_maps);
         class_runs.clear();
         sgpu::dfree(sum);
         sgpu::dfree(comp);

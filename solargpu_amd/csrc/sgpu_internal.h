@@ -165,7 +165,7 @@ struct TableResume {
     int32_t mx, my;   // x and y steps taken
 };
 /* table shapes built: directions per workgroup x steps per table (steps a multiple of the walk's
-   3-step unroll, below 2^16): 8 x 255, 4 x 255, 8 x 126, 4 x 126, 16 x 126, 8 x 63 */
+   3-step unroll, below 2^16): 8 x 255, 4 x 255, 4 x 510, 8 x 126, 4 x 126, 16 x 126, 8 x 63 */
 constexpr int kTableClasses = 256;
 struct TableArgs {
     const float* grid;
@@ -184,8 +184,16 @@ struct TableArgs {
     uint2* pend;                 // (raw row, direction | key << 12) of the rays that outlive their table
     int64_t pend_cap;
     TableResume* resume;         // kTableClasses x kDirs
+    const float* qmax;           // the four quadrant max maps (map_cells each), or null: no skipping
+    int64_t map_cells;
 };
 void launch_trace_table(const TableArgs& a, int n_items, int dirs, int steps, hipStream_t s);
+/* Cells of a quadrant max-map window a side: map q at cell (x, y) holds the highest height of the
+   cells (x + sx a, y + sy b), a, b in [0, kMaxMapWindow), sx = +1 when bit 0 of q is set (else -1),
+   sy likewise with bit 1. */
+constexpr int kMaxMapWindow = 10;
+/* The four maps (4 W H floats) from the heightmap; tmp holds 2 W H floats. */
+void launch_max_maps(const float* grid, int W, int H, float* tmp, float* qmax, hipStream_t s);
 /* The records of chunk positions [0, n) of the class-sorted order (voxel perm[v0 + p]). */
 void launch_table_records(const int32_t* perm, int64_t v0, int64_t n, const int32_t* pix_parent, const float* grid,
                           const float* wall_z, int64_t n_top, int W, int4* rec, hipStream_t s);

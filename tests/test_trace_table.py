@@ -5,7 +5,8 @@ ends of its binade [2^e, 2^(e+1)), e >= 3, the ray origin g + 2c rounds to g + (
 binade's spacing), so every such g has the same set-up (start cell - g, first-edge distances) along
 every direction.  The CPU test checks that over all 2592 directions and every such g up to 2^12;
 the GPU tests check that the viewshed by tables equals the pooled walk's bit for bit (lists, SVF,
-ba-RLE streams, step counts), over one and several chunks, and through the overflow path.
+ba-RLE streams with the skips of the max maps; step counts, which walk every step), over one and
+several chunks, and through the overflow path.
 """
 import numpy as np
 import pytest
@@ -39,12 +40,13 @@ def test_ray_setup_is_shared_within_a_binade_class():
         assert np.any(dx != dref)
 
 
-def _run(sc, tables, chunk=0, cap=0, steps=False):
+def _run(sc, tables, chunk=0, cap=0, steps=False, shape=(4, 255, True)):
     az_alt, dni, dhi, _ = sky()
     ctx = sg.Context(0)
     ctx.set_scene(sc)
     ctx.set_sky(az_alt, dni, dhi)
     ctx.set_trace_tables(tables, cap)
+    ctx.set_trace_table_shape(*shape)
     ctx.set_step_counting(steps)
     if chunk:
         ctx.set_chunk_voxels(chunk)
@@ -69,16 +71,20 @@ def _same(a, b):
     (33, 1100, 70, 0.5, 0.45, 20),     # columns across 1024, long rays at 0.5 m
 ])
 def test_tables_equal_pooled_walk(seed, w, h, cell, density, trees):
+    """With skipping (the lists) and without (the step counts: counting walks every step)."""
     sc = synth_scene(seed, w, h, cell, density, trees)
     print(f"scene seed {seed} {w}x{h}: {sc.num_total} voxels", flush=True)
-    t = _run(sc, True, steps=True)
     p = _run(sc, False, steps=True)
+    for shape in [(4, 255, True), (4, 510, True), (8, 126, False)]:
+        t = _run(sc, True, shape=shape)
+        _same(t, p)
+        ti = t["info"]
+        assert ti["table_chunks"] == ti["viewshed_chunks"] >= 1 and p["info"]["table_chunks"] == 0
+        assert ti["table_fallbacks"] == 0
+        assert ti["table_irregular_voxels"] > 0
+    t = _run(sc, True, steps=True)
     _same(t, p)
-    ti, pi = t["info"], p["info"]
-    assert ti["table_chunks"] == ti["viewshed_chunks"] >= 1 and pi["table_chunks"] == 0
-    assert ti["table_fallbacks"] == 0
-    assert ti["table_irregular_voxels"] > 0
-    assert ti["dda_steps"] == pi["dda_steps"] > 0
+    assert t["info"]["dda_steps"] == p["info"]["dda_steps"] > 0
 
 
 @pytest.mark.gpu
@@ -87,13 +93,15 @@ def test_tables_over_chunks_with_pending_rays():
     sc = synth_scene(34, 400, 400, 0.5, 0.45, 30)
     print(f"scene 400x400 @ 0.5 m: {sc.num_total} voxels", flush=True)
     assert sc.num_total > 3 * 65536
-    t = _run(sc, True, chunk=65536, steps=True)
     p = _run(sc, False, chunk=65536, steps=True)
+    t = _run(sc, True, chunk=65536)
     _same(t, p)
     ti = t["info"]
     assert ti["viewshed_chunks"] == ti["table_chunks"] == -(-sc.num_total // 65536)
     assert ti["table_pending_rays"] > 0 and ti["table_fallbacks"] == 0
-    assert ti["dda_steps"] == p["info"]["dda_steps"]
+    t = _run(sc, True, chunk=65536, steps=True)
+    _same(t, p)
+    assert t["info"]["dda_steps"] == p["info"]["dda_steps"]
 
 
 @pytest.mark.gpu
@@ -101,12 +109,14 @@ def test_pending_overflow_traces_the_chunk_again():
     """A pending capacity of one ray overflows on every chunk: each is traced again by the pooled
     walk (with the step count restored), and the lists are unchanged."""
     sc = synth_scene(34, 400, 400, 0.5, 0.45, 30)
-    t = _run(sc, True, chunk=65536, cap=1, steps=True)
     p = _run(sc, False, chunk=65536, steps=True)
-    _same(t, p)
-    ti = t["info"]
-    assert ti["table_fallbacks"] == ti["table_chunks"] == ti["viewshed_chunks"]
-    assert ti["dda_steps"] == p["info"]["dda_steps"]
+    for steps in (False, True):
+        t = _run(sc, True, chunk=65536, cap=1, steps=steps)
+        _same(t, p)
+        ti = t["info"]
+        assert ti["table_fallbacks"] == ti["table_chunks"] == ti["viewshed_chunks"]
+        if steps:
+            assert ti["dda_steps"] == p["info"]["dda_steps"]
 
 
 @pytest.mark.gpu
