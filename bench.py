@@ -299,6 +299,14 @@ def main():
               "l2_hit_rate": read_profile("reflect_barle_c2_traffic.txt", "l2_hit_rate"),
               "achieved": (l2_req * 128 / avg_launch / 1e9) if l2_req else None}
     gather["frac"] = gather["achieved"] / l2_gbps if gather["achieved"] else None
+    # the rate its hit rate allows: hits at the L2-resident rate, misses at the rate of the same 1 KB
+    # row stream over 32 MiB (beyond the L2, in the MALL / HBM), blended by the profiled hit rate
+    beyond_gbps = ctx.l2_bandwidth(32 << 20, 3)
+    hit = gather["l2_hit_rate"]
+    if hit is not None:
+        gather["peak_at_hit_rate"] = 1.0 / (hit / l2_gbps + (1.0 - hit) / beyond_gbps)
+        gather["peak_beyond_l2"] = beyond_gbps
+        gather["frac_at_hit_rate"] = (gather["achieved"] / gather["peak_at_hit_rate"]) if gather["achieved"] else None
     valu_fma, valu_mix, valu_cmp = ctx.valu_rate(0), ctx.valu_rate(1), ctx.valu_rate(2)
     valu_peak = max(valu_fma, valu_mix, valu_cmp)
     # DDA roofline: VALU issue.  Instructions per trace launch from the committed PMC summary of the

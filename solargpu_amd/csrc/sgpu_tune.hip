@@ -6,7 +6,7 @@
  *   stage-2 pass   hours per pass K in {64, 128, 256} x waves per reflection
  *                  workgroup in {2, 4} (LDS footprint and occupancy) x XCD
  *                  band in {0, 64, 256} (contiguous eighths or side-by-side
- *                  bands), and the queued receivers with bands {32, 256}, then
+ *                  bands), and the queued receivers with bands {32, 256, 512}, then
  *                  for the best K: waves {2, 4} x receiver groups {8, 32, 64,
  *                  128} x band {0, 256}: three passes each on the live lists,
  *                  scratch accumulators.
@@ -166,7 +166,7 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
         for (int K : {64, 128, 256})
             for (int waves : {2, 4}) {
                 for (int band : {0, 64, 256}) trial(K, waves, band, 0, 0);
-                for (int band : {32, 256}) trial(K, waves, band, 0, 1);   // queued receivers
+                for (int band : {32, 256, 512}) trial(K, waves, band, 0, 1);   // queued receivers
             }
         const int K = t.hours_per_pass;
         for (int waves : {2, 4})   // the best waves count with receiver groups can differ
