@@ -269,6 +269,9 @@ def main():
 
     # roofline of the dominant kernel: reflection + Kahan accumulate (ba-RLE decode)
     Kp = inf["hours_per_pass"]
+    rq = ctx.tuning()["receiver_queue"]
+    refl_kernel = (f"k_reflect_group<K={Kp}, {rq} receivers per wave, ba-RLE>" if rq >= 2 and Kp >= 16
+                   else f"k_reflect<K={Kp}, ba-RLE>")
     passes = inf["hour_passes"]
     nh_list = [min(Kp, H - p * Kp) for p in range(passes)]
     bytes_all = sum(reflect_pass_bytes(N, ntop, inf["barle_bytes"], nh) for nh in nh_list)
@@ -293,7 +296,7 @@ def main():
     # the reflection pass's row gathers: L1 -> L2 line requests per launch (committed PMC summary of the
     # same workload and shape) over the live launch time, against the measured L2 read bandwidth
     l2_req = read_profile("reflect_barle_c2_traffic.txt", "l1_to_l2_read_requests_per_launch")
-    gather = {"bound": "l2", "unit": "GB/s", "kernel": f"k_reflect<K={Kp}, ba-RLE>", "peak": l2_gbps,
+    gather = {"bound": "l2", "unit": "GB/s", "kernel": refl_kernel, "peak": l2_gbps,
               "peak_source": "measured: every wave streams 2 MiB (L2-resident) as 1 KB rows of 16-byte lane loads",
               "l1_to_l2_read_bytes_per_launch": (l2_req * 128) if l2_req else None,
               "l2_hit_rate": read_profile("reflect_barle_c2_traffic.txt", "l2_hit_rate"),
@@ -381,7 +384,7 @@ def main():
         "stage2_ms": t_hr * 1e3 / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": f"k_reflect<K={Kp}, ba-RLE>", "avg_launch_ms": avg_launch * 1e3,
+                     "kernel": refl_kernel, "avg_launch_ms": avg_launch * 1e3,
                      "algorithmic_bytes_per_launch": bytes_all / passes,
                      # what binds the batched gather: its L2-miss traffic (the PMC figure above) per
                      # launch time, against the copy bandwidth measured in this run

@@ -461,20 +461,60 @@ struct WaveLds {
     int pend_id[WAVES][FMT == SGPU_FMT_BARLE ? 128 : 1];
 };
 
-/* One receiver's pass, by the whole wave: lane l's part of the receiver's own irradiance (hours
- * l, l + 64, ... of the pass) and of the reflected irradiance summed over the pass's hours.
- * Shared by the static and the queued kernels, so every receiver's total does not depend on
- * which wave takes it (bit-identical annual sums under every receiver assignment). */
-template <int K, int FMT, int WAVES, int STAGE>
-__device__ __forceinline__ void receiver_parts(const HourArgs& a, const HourConst* hours, const Recv& r, int64_t v,
-                                               int64_t s0, int64_t s1, StreamWindow& win, WaveLds<FMT, WAVES, STAGE>& L,
-                                               int wave, int lane, bool off32, float inv_w, int cells, float& own,
-                                               float& refl) {
-    own = 0.0f;
+/* Lane l's part of a receiver's own irradiance over the pass: hours l, l + 64, ... */
+template <int K>
+__device__ __forceinline__ float own_part(const HourArgs& a, const HourConst* hours, const Recv& r, int64_t v,
+                                          int lane) {
+    float own = 0.0f;
     for (int h = lane; h < K && h < a.nh; h += 64) {
         const HourConst& hc = hours[h];
         own += hour_irradiance(hc, r.cs, r.ss, r.ca, r.sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
     }
+    return own;
+}
+
+/* The ba-RLE stream [s0, s1) decoded by the wave, its ids handed to take(valid, id) 64 at a time
+ * (a block holds ~30 ids: decoded ids wait in pend_id, the wave's 128 slots, so that the weights
+ * are computed on full 64-id chunks; the last call may have fewer valid lanes). */
+template <typename F>
+__device__ __forceinline__ void decode_chunks(StreamWindow& win, int64_t s0, int64_t s1, int* pend_id, int lane,
+                                              F&& take) {
+    int pend = 0;
+    auto gather = [&](bool valid, int id) __attribute__((always_inline)) {
+        const uint64_t m = __ballot(valid);
+        if (valid) pend_id[pend + lanes_below(m)] = id;
+        pend += __popcll(m);
+        if (pend >= 64) {
+            wave_lds_sync();
+            const int x = pend_id[lane];
+            const bool more = 64 + lane < pend;
+            const int y = more ? pend_id[64 + lane] : 0;
+            wave_lds_sync();
+            if (more) pend_id[lane] = y;
+            pend -= 64;
+            take(true, x);
+        }
+    };
+    wave_decode(win, s0, s1, lane, gather);
+    if (pend > 0) {
+        wave_lds_sync();
+        const bool valid = lane < pend;
+        take(valid, valid ? pend_id[lane] : -1);
+    }
+}
+
+/* One receiver's pass, by the whole wave: lane l's part of the receiver's own irradiance (hours
+ * l, l + 64, ... of the pass) and of the reflected irradiance summed over the pass's hours.
+ * Shared by the static and the queued kernels, so every receiver's total does not depend on
+ * which wave takes it (bit-identical annual sums under every receiver assignment).
+ * list_id / list_w: the wave's kListCap compacted entries; pend_id: its 128 decoded ids not yet
+ * weighed (ba-RLE). */
+template <int K, int FMT>
+__device__ __forceinline__ void receiver_parts(const HourArgs& a, const HourConst* hours, const Recv& r, int64_t v,
+                                               int64_t s0, int64_t s1, StreamWindow& win, int* list_id, float* list_w,
+                                               int* pend_id, int lane, bool off32, float inv_w, int cells, float& own,
+                                               float& refl) {
+    own = own_part<K>(a, hours, r, v, lane);
     float acc = 0.0f;
     int cnt = 0;
     auto take = [&](bool valid, int id) __attribute__((always_inline)) {
@@ -483,13 +523,13 @@ __device__ __forceinline__ void receiver_parts(const HourArgs& a, const HourCons
         const uint64_t m = __ballot(keep);
         if (keep) {
             const int at = cnt + lanes_below(m);
-            L.list_id[wave][at] = id;
-            L.list_w[wave][at] = w;
+            list_id[at] = id;
+            list_w[at] = w;
         }
         cnt += __popcll(m);
         if (cnt > kListCap - 64) {
             wave_lds_sync();
-            acc += apply_list<K>(L.list_id[wave], L.list_w[wave], cnt, a.hourly_top, lane, off32, a.row_swz);
+            acc += apply_list<K>(list_id, list_w, cnt, a.hourly_top, lane, off32, a.row_swz);
             wave_lds_sync();
             cnt = 0;
         }
@@ -501,34 +541,11 @@ __device__ __forceinline__ void receiver_parts(const HourArgs& a, const HourCons
             take(valid, valid ? a.nb_ids[k] - 1 : -1);
         }
     } else {
-        /* a ba-RLE block holds ~30 ids: decoded ids wait in pend_id so that the weights are
-           computed on full 64-id chunks */
-        int pend = 0;
-        auto gather = [&](bool valid, int id) __attribute__((always_inline)) {
-            const uint64_t m = __ballot(valid);
-            if (valid) L.pend_id[wave][pend + lanes_below(m)] = id;
-            pend += __popcll(m);
-            if (pend >= 64) {
-                wave_lds_sync();
-                const int x = L.pend_id[wave][lane];
-                const bool more = 64 + lane < pend;
-                const int y = more ? L.pend_id[wave][64 + lane] : 0;
-                wave_lds_sync();
-                if (more) L.pend_id[wave][lane] = y;
-                pend -= 64;
-                take(true, x);
-            }
-        };
-        wave_decode(win, s0, s1, lane, gather);
-        if (pend > 0) {
-            wave_lds_sync();
-            const bool valid = lane < pend;
-            take(valid, valid ? L.pend_id[wave][lane] : -1);
-        }
+        decode_chunks(win, s0, s1, pend_id, lane, take);
     }
     if (cnt > 0) {
         wave_lds_sync();
-        acc += apply_list<K>(L.list_id[wave], L.list_w[wave], cnt, a.hourly_top, lane, off32, a.row_swz);
+        acc += apply_list<K>(list_id, list_w, cnt, a.hourly_top, lane, off32, a.row_swz);
         wave_lds_sync();
     }
     refl = acc;
@@ -631,8 +648,8 @@ __device__ __forceinline__ void reflect_block(const HourArgs& a, unsigned lb) {
         const int64_t s0 = uniform64(beg[wave][j]), s1 = uniform64(fin[wave][j]);
         win.end = wend >= 0 ? wend : s1;
         float own, refl;
-        receiver_parts<K, FMT, WAVES, kStageBytes>(a, hrs, r, vj, s0, s1, win, L, wave, lane, off32, inv_w, cells, own,
-                                                   refl);
+        receiver_parts<K, FMT>(a, hrs, r, vj, s0, s1, win, L.list_id[wave], L.list_w[wave], L.pend_id[wave], lane, off32,
+                               inv_w, cells, own, refl);
         if constexpr (DBG) {
             own = wave_sum(own);
             refl = wave_sum(refl);
@@ -720,8 +737,8 @@ void k_reflect_queue(HourArgs a) {
         const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
         win.end = s1;
         float own, refl;
-        receiver_parts<K, FMT, WAVES, kQueueStageBytes>(a, hrs, r, v, s0, s1, win, L, wave, lane, off32, inv_w, cells,
-                                                        own, refl);
+        receiver_parts<K, FMT>(a, hrs, r, v, s0, s1, win, L.list_id[wave], L.list_w[wave], L.pend_id[wave], lane, off32,
+                               inv_w, cells, own, refl);
         const float t = wave_sum(own + refl);
         if (lane == 0) kahan_add(a.sum, a.comp, q, t);
     }
@@ -729,6 +746,283 @@ void k_reflect_queue(HourArgs a) {
 
 __global__ void k_clear_queue(unsigned* q) {
     if (threadIdx.x < kXcds * kQueuesPerXcd) q[threadIdx.x * kQueueStride] = 0u;
+}
+
+/* ---- grouped receivers: one wave walks R consecutive positions and gathers each row once ---- */
+
+/* The group's merged list: entry i is a neighbour row and its R weights ws[i][j] (zero where the
+ * row is not in receiver j's kept list).  Each row is gathered once and added into the R sums;
+ * lanes hold hours as in apply_list_at. */
+template <int K, int R, bool OFF32>
+__device__ __forceinline__ void apply_group_at(const int* ids, const float (*ws)[R], int cnt,
+                                               const float* __restrict__ top, int lane, int swz, float (&acc)[R]) {
+    constexpr int HPL = (K >= 16) ? 4 : 1;
+    constexpr int LPE = K / HPL;
+    constexpr int G = 64 / LPE;
+    typedef float vec __attribute__((ext_vector_type(HPL)));
+    constexpr int B = 16 / HPL;   // 16 VGPRs of rows in flight
+    const unsigned lane_off = (unsigned)((lane % LPE) * HPL) * 4u;
+    const char* lane_top = reinterpret_cast<const char*>(top) + lane_off;
+    constexpr int kRowShift = __builtin_ctz(K * 4);
+    auto load = [&](int cell) __attribute__((always_inline)) {
+        const unsigned id = row_slot((unsigned)cell, swz);
+        if constexpr (OFF32) {
+            const unsigned off = (unsigned)id * (unsigned)(K * 4) + lane_off;
+            return *reinterpret_cast<const vec*>(reinterpret_cast<const char*>(top) + off);
+        } else {
+            return *reinterpret_cast<const vec*>(lane_top + ((uint64_t)id << kRowShift));
+        }
+    };
+    vec s[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) s[j] = vec{};
+    int i = lane / LPE;
+    for (; i + (B - 1) * G < cnt; i += B * G) {
+        vec x[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) x[u] = load(ids[i + u * G]);
+        __builtin_amdgcn_sched_group_barrier(0x0020, B, 0);   // the VMEM reads first
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const float w = ws[i + u * G][j];
+#pragma unroll
+                for (int h = 0; h < HPL; ++h) s[j][h] = fmaf(w, x[u][h], s[j][h]);
+            }
+    }
+    for (; i < cnt; i += G) {
+        const vec x = load(ids[i]);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const float w = ws[i][j];
+#pragma unroll
+            for (int h = 0; h < HPL; ++h) s[j][h] = fmaf(w, x[h], s[j][h]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        float t = 0.0f;
+#pragma unroll
+        for (int h = 0; h < HPL; ++h) t += s[j][h];
+        acc[j] += t;
+    }
+}
+
+/* Per-wave stream window of the grouped kernel. */
+constexpr int kGroupStageBytes = 1024;
+
+/* One wave's LDS in the grouped kernel.  dec: while the group's streams are decoded, the stream
+ * window and the decoded ids not yet weighed (and, for a group walked receiver by receiver, its
+ * compacted list); mrg: while the lists are merged, the 64-entry heads of the R lists and the
+ * merged list with R weights per entry. */
+template <int R>
+union GroupLds {
+    struct {
+        int list_id[kListCap];
+        float list_w[kListCap];
+        __align__(16) uint8_t stage[kGroupStageBytes];
+        int pend_id[128];
+    } dec;
+    struct {
+        int cid[R][64];
+        float cw[R][64];
+        int gid[kListCap];
+        float gw[kListCap][R];
+    } mrg;
+};
+
+/* Position p of the full scene in processing order (voxel order[p], lists and sums by voxel) or
+   of a subset (voxel subset[p], lists and sums by position): its voxel and its list / sum index. */
+__device__ __forceinline__ void queue_voxel(const HourArgs& a, int64_t p, int64_t& v, int64_t& q) {
+    v = a.order ? (int64_t)a.order[p] : (int64_t)a.subset[p];
+    q = a.order ? v : p;
+}
+
+/* Grouped receivers, queued as k_reflect_queue but by groups of R consecutive positions (XCD bands
+ * of queue_band / R groups).  Receivers next to each other in the processing order share most of
+ * their kept neighbours (a top cell and its walls, walls stacked on one cell, adjacent cells): a
+ * wave that walks the group's lists together gathers each shared row once.
+ * Phase 1: each receiver's ba-RLE stream is decoded and weighed; its kept (id, weight) entries,
+ *   in stream order (descending ids), go to the wave's scratch, list j at [lo_j, lo_(j+1)).
+ * Phase 2: the R descending lists are merged 64-entry heads at a time.  With t the largest of the
+ *   64th head entries among the lists that have more than 64 left, every entry >= t of every list
+ *   is in its head (later entries are smaller than their list's 64th); those are taken, each id
+ *   once, by the first list holding it, with the weights the other lists give it (a binary search
+ *   of their heads), into the merged list, which is applied 64 to 128 entries at a time.  The list
+ *   that set t gives all 64 of its head, so every round makes progress.
+ * A group whose entries overflow the scratch is walked receiver by receiver (receiver_parts).
+ * Each receiver's own irradiance and the Kahan add are as in k_reflect_queue; its reflected sum
+ * differs from the receiver-by-receiver walk only in the order of the additions. */
+template <int K, int WAVES, int R>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(R == 2 ? 7 : 5)))
+void k_reflect_group(HourArgs a) {
+    static_assert(R == 2 || R == 4, "groups of 2 or 4 receivers");
+    __shared__ GroupLds<R> GL[WAVES];
+    __shared__ HourConst hrs[K];
+    stage_hours<K>(a, hrs);
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const unsigned x = blockIdx.x % kXcds;
+    const unsigned c = ((blockIdx.x / kXcds) * WAVES + (unsigned)wave) % kQueuesPerXcd;
+    unsigned* counter = a.queue + (x * kQueuesPerXcd + c) * kQueueStride;
+    const int64_t n = a.order ? a.n_total : a.n_sub;
+    const int64_t ng = (n + R - 1) / R;
+    const int64_t band = a.queue_band > 0 ? max<int64_t>(a.queue_band / R, 1) : 0;
+    const float inv_w = 1.0f / (float)a.W;
+    const int cells = a.W * a.H;
+    const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);
+    const int cap = a.grp_cap;
+    int2* scr = a.grp_scratch + ((int64_t)blockIdx.x * WAVES + wave) * cap;
+    GroupLds<R>& L = GL[wave];
+    StreamWindow win{L.dec.stage, a.nb_bytes, 0, 0, 0, kGroupStageBytes};
+    unsigned pending = atomicAdd(counter, lane == 0 ? 1u : 0u);   // every lane joins (see k_reflect_queue)
+    for (int64_t turn = 0; turn <= ng; ++turn) {
+        const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)pending);
+        const int64_t g = queue_position((uint64_t)item * kQueuesPerXcd + c, x, ng, band);
+        if (g >= ng) break;
+        pending = atomicAdd(counter, lane == 0 ? 1u : 0u);
+        const int64_t p0 = g * R;
+        const int nr = (int)min<int64_t>(R, n - p0);
+        win.lo = win.hi = 0;   // the merge overwrote the window's bytes
+
+        /* phase 1: lane k (1 <= k <= R) ends with the end of list k - 1 (lists past nr are empty) */
+        int lo_v = 0, used = 0;
+        for (int j = 0; j < nr; ++j) {
+            int64_t v, q;
+            queue_voxel(a, p0 + j, v, q);
+            const Recv r = receiver_of(a, v, inv_w);
+            const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
+            win.end = s1;
+            auto take = [&](bool valid, int id) __attribute__((always_inline)) {
+                const float w = valid ? neighbour_weight(id, r, a.grid, a.W, cells, inv_w) : 0.0f;
+                const bool keep = w > 0.0f;
+                const uint64_t m = __ballot(keep);
+                const int at = used + lanes_below(m);
+                if (keep && at < cap) scr[at] = make_int2(id, __float_as_int(w));
+                used += __popcll(m);
+            };
+            decode_chunks(win, s0, s1, L.dec.pend_id, lane, take);
+            if (lane > j) lo_v = used;
+        }
+        if (used > cap) {   // walked receiver by receiver
+            for (int j = 0; j < nr; ++j) {
+                int64_t v, q;
+                queue_voxel(a, p0 + j, v, q);
+                const Recv r = receiver_of(a, v, inv_w);
+                const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
+                win.end = s1;
+                float own, refl;
+                receiver_parts<K, SGPU_FMT_BARLE>(a, hrs, r, v, s0, s1, win, L.dec.list_id, L.dec.list_w,
+                                                  L.dec.pend_id, lane, off32, inv_w, cells, own, refl);
+                const float t = wave_sum(own + refl);
+                if (lane == 0) kahan_add(a.sum, a.comp, q, t);
+            }
+            continue;
+        }
+
+        /* phase 2: merge (the scratch entries written by other lanes are visible after the fence
+           of wave_lds_sync) */
+        wave_lds_sync();
+        int h[R], e[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            h[j] = __builtin_amdgcn_readlane(lo_v, j);
+            e[j] = __builtin_amdgcn_readlane(lo_v, j + 1);
+        }
+        float acc[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = 0.0f;
+        int gcnt = 0;
+        for (;;) {
+            bool more = false;
+#pragma unroll
+            for (int j = 0; j < R; ++j) more |= h[j] < e[j];
+            if (!more) break;
+            int idv[R];
+            float wv[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int k = h[j] + lane;
+                idv[j] = -1;
+                wv[j] = 0.0f;
+                if (k < e[j]) {
+                    const int2 ent = scr[k];
+                    idv[j] = ent.x;
+                    wv[j] = __int_as_float(ent.y);
+                }
+            }
+            int t = -1;
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                if (e[j] - h[j] > 64) t = max(t, __builtin_amdgcn_readlane(idv[j], 63));
+            const int tmin = max(t, 0);   // t < 0: every list ends within its head
+            int nj[R];
+            wave_lds_sync();   // the previous round's reads of the heads are done
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                L.mrg.cid[j][lane] = idv[j];
+                L.mrg.cw[j][lane] = wv[j];
+                nj[j] = __popcll(__ballot(idv[j] >= tmin));   // a prefix of the descending head
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int xid = idv[j];
+                bool first = lane < nj[j];
+                float w[R];
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    w[i] = 0.0f;
+                    if (i == j) {
+                        w[i] = wv[j];
+                        continue;
+                    }
+                    /* pos: the entries of head i (prefix of nj[i]) greater than xid */
+                    int pos = 0;
+#pragma unroll
+                    for (int st = 64; st >= 1; st >>= 1)
+                        if (pos + st <= nj[i] && L.mrg.cid[i][pos + st - 1] > xid) pos += st;
+                    const bool found = pos < nj[i] && L.mrg.cid[i][pos] == xid;
+                    if (i < j) first = first && !found;
+                    else if (found) w[i] = L.mrg.cw[i][pos];
+                }
+                const uint64_t m = __ballot(first);
+                if (first) {
+                    const int at = gcnt + lanes_below(m);
+                    L.mrg.gid[at] = xid;
+#pragma unroll
+                    for (int i = 0; i < R; ++i) L.mrg.gw[at][i] = w[i];
+                }
+                gcnt += __popcll(m);
+                if (gcnt > kListCap - 64) {
+                    wave_lds_sync();
+                    if (off32) apply_group_at<K, R, true>(L.mrg.gid, L.mrg.gw, gcnt, a.hourly_top, lane, a.row_swz, acc);
+                    else apply_group_at<K, R, false>(L.mrg.gid, L.mrg.gw, gcnt, a.hourly_top, lane, a.row_swz, acc);
+                    wave_lds_sync();
+                    gcnt = 0;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < R; ++j) h[j] += nj[j];
+        }
+        if (gcnt > 0) {
+            wave_lds_sync();
+            if (off32) apply_group_at<K, R, true>(L.mrg.gid, L.mrg.gw, gcnt, a.hourly_top, lane, a.row_swz, acc);
+            else apply_group_at<K, R, false>(L.mrg.gid, L.mrg.gw, gcnt, a.hourly_top, lane, a.row_swz, acc);
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (j < nr) {
+                int64_t v, q;
+                queue_voxel(a, p0 + j, v, q);
+                const Recv r = receiver_of(a, v, inv_w);
+                const float tot = wave_sum(own_part<K>(a, hrs, r, v, lane) + acc[j]);
+                if (lane == 0) kahan_add(a.sum, a.comp, q, tot);
+            }
+        }
+    }
 }
 
 /* ---- batched tiles: one launch per pass covers every tile of a batch ---- */
@@ -851,8 +1145,8 @@ void k_reflect_batch_queue(const BatchHour* d, int ntiles, int pass, unsigned* q
         const int64_t s0 = uniform64(a.nb_off[v]), s1 = uniform64(a.nb_off[v + 1]);
         win.end = s1;
         float own, refl;
-        receiver_parts<K, FMT, WAVES, kQueueStageBytes>(a, a.hours, r, v, s0, s1, win, L, wave, lane, off32, inv_w,
-                                                        cells, own, refl);
+        receiver_parts<K, FMT>(a, a.hours, r, v, s0, s1, win, L.list_id[wave], L.list_w[wave], L.pend_id[wave], lane,
+                               off32, inv_w, cells, own, refl);
         const float tt = wave_sum(own + refl);
         if (lane == 0) kahan_add(a.sum, a.comp, v, tt);
     }
@@ -943,6 +1237,28 @@ void reflect_queue_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
     }
 }
 
+template <int K, int WAVES, int R>
+void reflect_group_kwr(const HourArgs& a, hipStream_t s) {
+    static const GridShape g = occupancy(k_reflect_group<K, WAVES, R>, 64 * WAVES);
+    const unsigned grid = queue_grid(g, a.queue_blocks_per_cu);
+    if ((int64_t)grid * WAVES > a.grp_waves || a.grp_cap < 64)
+        fail(SGPU_ERR_STATE, "group scratch holds %lld waves of %d entries; the grid has %lld waves",
+             (long long)a.grp_waves, a.grp_cap, (long long)grid * WAVES);
+    hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(kXcds * kQueuesPerXcd), 0, s, a.queue);
+    hipLaunchKernelGGL((k_reflect_group<K, WAVES, R>), dim3(grid), dim3(64 * WAVES), 0, s, a);
+}
+
+template <int K>
+void reflect_group_k(const HourArgs& a, hipStream_t s) {
+    if (a.grp_size == 4) {
+        if (a.waves == 2) reflect_group_kwr<K, 2, 4>(a, s);
+        else reflect_group_kwr<K, 4, 4>(a, s);
+    } else {
+        if (a.waves == 2) reflect_group_kwr<K, 2, 2>(a, s);
+        else reflect_group_kwr<K, 4, 2>(a, s);
+    }
+}
+
 /* XCD bands of the queued batched reflection: 65536 positions (what the single-scene tuning picks
    on the 2048^2 scene: bands of 256 workgroups of 4 waves x 64) */
 constexpr int64_t kBatchQueueBand = 65536;
@@ -968,6 +1284,12 @@ void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
                         ((a.order != nullptr && a.subset == nullptr && a.n_sub == 0) ||
                          (a.order == nullptr && a.subset != nullptr && a.n_sub > 0));
     if (queued) {
+        if constexpr (K >= 16) {   // grouped receivers: ba-RLE lists, 4 hours per lane
+            if (fmt == SGPU_FMT_BARLE && a.grp_size >= 2 && a.grp_scratch != nullptr) {
+                reflect_group_k<K>(a, s);
+                return;
+            }
+        }
         if (a.waves == 2) reflect_queue_kw<K, 2>(a, fmt, s);
         else reflect_queue_kw<K, 4>(a, fmt, s);
         return;

@@ -219,10 +219,34 @@ int row_swizzle(const sgpu_ctx* c) {
     return s;
 }
 
+/* The grouped reflection's per-wave scratch: kGroupCap entries for each wave the device can hold
+   at once (kMaxWavesPerCu per CU; 256 CUs: 268 MB). */
+void ensure_group_scratch(sgpu_ctx* c) {
+    if (c->grp_scratch) return;
+    int dev = 0, cus = 0;
+    SGPU_HIP(hipGetDevice(&dev));
+    SGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t waves = (int64_t)std::max(cus, 1) * kMaxWavesPerCu;
+    c->grp_scratch = dalloc<int2>((size_t)waves * kGroupCap);
+    c->grp_waves = waves;
+}
+
+void set_group_args(const sgpu_ctx* c, HourArgs& a, int queue) {
+    a.grp_size = 0;
+    a.grp_scratch = nullptr;
+    if (queue >= 2 && c->grp_scratch) {
+        a.grp_size = queue;
+        a.grp_scratch = c->grp_scratch;
+        a.grp_cap = c->group_cap > 0 ? c->group_cap : kGroupCap;
+        a.grp_waves = c->grp_waves;
+    }
+}
+
 void ensure_accumulators(sgpu_ctx* c, int K) {
     if (!c->sum) c->sum = dalloc<float>(c->n_total);
     if (!c->comp) c->comp = dalloc<float>(c->n_total);
     if (!c->reflect_queue) c->reflect_queue = dalloc<unsigned>(kQueueWords);
+    if (c->tune.receiver_queue >= 2) ensure_group_scratch(c);
     const size_t need_elems = (size_t)hourly_rows(c->n_top, row_swizzle(c)) * K;
     if (need_elems > c->hourly_top_elems) {
         dfree(c->hourly_top);
@@ -261,6 +285,7 @@ HourArgs hour_args(sgpu_ctx* c, unsigned fmt) {
         a.queue = c->reflect_queue;
         a.queue_band = (int64_t)c->tune.xcd_band * 64 * (c->tune.waves_per_group == 2 ? 2 : 4);
         a.queue_blocks_per_cu = c->tune.group_blocks;   // group_blocks in queue mode: workgroups per CU
+        set_group_args(c, a, c->tune.receiver_queue);
     }
     if (fmt == SGPU_FMT_CSR) {
         a.nb_off = c->csr_off;
@@ -1727,6 +1752,16 @@ int sgpu_set_trace_tables(sgpu_ctx* c, int enable, int64_t pending_cap) {
         need(pending_cap >= 0, SGPU_ERR_ARG, "pending_cap must be >= 0");
         c->trace_tables = enable != 0;
         c->pend_cap_override = pending_cap;
+    });
+}
+
+int sgpu_set_group_scratch_cap(sgpu_ctx* c, int entries) {
+    return guarded([&] {
+        need(c != nullptr, SGPU_ERR_ARG, "null context");
+        need(entries == 0 || (entries >= 64 && entries <= kGroupCap), SGPU_ERR_ARG,
+             "group scratch cap must be 0 or 64..4096 entries");
+        static_assert(kGroupCap == 4096, "the message above states the cap");
+        c->group_cap = entries;
     });
 }
 

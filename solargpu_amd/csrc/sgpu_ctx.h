@@ -196,6 +196,9 @@ struct sgpu_ctx {
        shape the autotuner picks on the 2048^2 scene, until a scene of its own is tuned */
     sgpu_tuning tune{32, 128, 4, 32, 0, 0, 0.0, 0.0, 1};
     unsigned* reflect_queue = nullptr;                   // kXcds counters of the queued reflection
+    int2* grp_scratch = nullptr;                         // kGroupCap entries per wave, grouped reflection
+    int64_t grp_waves = 0;                               // waves grp_scratch covers
+    int group_cap = 0;                                   // entries per wave used (0: kGroupCap)
     int residency = SGPU_LISTS_DEVICE;          // where sgpu_viewshed leaves the lists
     sgpu::HostLists host;                       // the lists when host-resident
     hipStream_t copy_stream = nullptr;          // H2D tile copies of the streamed hour loop
@@ -245,6 +248,8 @@ struct sgpu_ctx {
         sgpu::dfree(comp);
         sgpu::dfree(hourly_top);
         sgpu::dfree(reflect_queue);
+        sgpu::dfree(grp_scratch);
+        grp_waves = 0;
         hourly_top_elems = 0;
         scene = false;
     }
@@ -266,6 +271,8 @@ namespace sgpu {
    the autotuner. */
 HourArgs stage2_args(sgpu_ctx* c, unsigned fmt);
 void ensure_accumulators(sgpu_ctx* c, int K);
+void ensure_group_scratch(sgpu_ctx* c);
+void set_group_args(const sgpu_ctx* c, HourArgs& a, int queue);
 void ensure_hours_d(sgpu_ctx* c, const std::vector<HourConst>& hs);
 void zero_floats(float* p, int64_t n, hipStream_t s);
 /* The current device's address of the allocation-free hour loop's scratch array, looked up

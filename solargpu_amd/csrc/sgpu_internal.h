@@ -355,7 +355,19 @@ struct HourArgs {
     unsigned* queue;
     int64_t queue_band;
     int queue_blocks_per_cu;   // resident workgroups per CU of the queued grid (0 = as many as fit)
+    /* Grouped receivers (queued, ba-RLE, K >= 16): the queue hands out groups of grp_size (2 or 4)
+       consecutive positions, each walked by one wave that gathers every kept neighbour row of the
+       group once.  Wave w of the grid keeps the group's (id, weight) entries in its grp_cap entries
+       of grp_scratch, from w * grp_cap; the buffer covers grp_waves waves.  grp_size <= 1 or no
+       scratch: receivers one by one. */
+    int2* grp_scratch;
+    int grp_cap;
+    int grp_size;
+    int64_t grp_waves;
 };
+/* Entries per wave of the group scratch (8 bytes each), and the most waves a CU holds at once. */
+constexpr int kGroupCap = 4096;
+constexpr int kMaxWavesPerCu = 32;
 /* Rows of the hourly buffer for n_top cells: swizzled rows stay inside aligned blocks of 1024. */
 inline int64_t hourly_rows(int64_t n_top, int row_swz) { return row_swz > 0 ? (n_top + 1023) / 1024 * 1024 : n_top; }
 constexpr int kQueuesPerXcd = 16;

@@ -97,6 +97,8 @@ int sgpu_hour_loop_noalloc(sgpu_ctx* c, unsigned fmt, int K, float* out) {
         a.comp = reinterpret_cast<float*>(base + round256((size_t)c->n_top * K * sizeof(float)));
         a.sum = out;
         a.queue = nullptr;
+        a.grp_size = 0;   // receivers one by one: the grouped walk's scratch is a context buffer
+        a.grp_scratch = nullptr;
         if (c->tune.receiver_queue) {   // the launcher zeroes the counters with a kernel
             a.queue = reinterpret_cast<unsigned*>(base + want - kQueueBytes);
             a.queue_band = (int64_t)c->tune.xcd_band * 64 * (c->tune.waves_per_group == 2 ? 2 : 4);

@@ -90,6 +90,7 @@ double time_trace(sgpu_ctx* c, int G, int64_t v0, int64_t n, int32_t* raw) {
 
 double time_passes(sgpu_ctx* c, unsigned fmt, int K, int waves, int band, int group, int queue, int passes) {
     ensure_accumulators(c, K);   // the hourly buffer grows to this trial's n_top x K
+    if (queue >= 2) ensure_group_scratch(c);
     HourArgs a = stage2_args(c, fmt);
     a.waves = waves;
     a.band = band;
@@ -97,6 +98,7 @@ double time_passes(sgpu_ctx* c, unsigned fmt, int K, int waves, int band, int gr
     a.queue = queue ? c->reflect_queue : nullptr;
     a.queue_band = (int64_t)band * 64 * waves;
     a.queue_blocks_per_cu = queue ? group : 0;
+    set_group_args(c, a, queue);
     const int nsel = (int)c->hours.size();
     auto run = [&](int np) {
         for (int p = 0, h0 = 0; p < np && h0 < nsel; ++p, h0 += K) {
@@ -167,6 +169,8 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
             for (int waves : {2, 4}) {
                 for (int band : {0, 64, 256}) trial(K, waves, band, 0, 0);
                 for (int band : {32, 256, 512}) trial(K, waves, band, 0, 1);   // queued receivers
+                if (fmt == SGPU_FMT_BARLE && waves == 4)   // queued groups of 2 or 4 receivers
+                    for (int grp : {2, 4}) trial(K, waves, 512, 0, grp);
             }
         const int K = t.hours_per_pass;
         for (int waves : {2, 4})   // the best waves count with receiver groups can differ
@@ -211,7 +215,8 @@ int sgpu_set_tuning(sgpu_ctx* c, const sgpu_tuning* t) {
         need(t->waves_per_group == 2 || t->waves_per_group == 4, SGPU_ERR_ARG, "waves_per_group must be 2 or 4");
         need(t->xcd_band == 0 || (t->xcd_band >= 16 && t->xcd_band <= 4096), SGPU_ERR_ARG,
              "xcd_band must be 0 or 16..4096");
-        need(t->receiver_queue == 0 || t->receiver_queue == 1, SGPU_ERR_ARG, "receiver_queue must be 0 or 1");
+        need(t->receiver_queue == 0 || t->receiver_queue == 1 || t->receiver_queue == 2 || t->receiver_queue == 4,
+             SGPU_ERR_ARG, "receiver_queue must be 0, 1, 2 or 4");
         if (t->receiver_queue)
             need(t->group_blocks >= 0 && t->group_blocks <= 64, SGPU_ERR_ARG,
                  "group_blocks (workgroups per CU of the receiver queue) must be 0..64");
