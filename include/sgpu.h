@@ -230,10 +230,11 @@ int sgpu_set_trace_tables(sgpu_ctx* ctx, int enable, int64_t pending_cap);
    jump 9 steps at once when no cell among them can stop it as a hit (quadrant max maps of the
    heightmap); the results are the same either way.  Step counting walks every step. */
 int sgpu_set_trace_table_shape(sgpu_ctx* ctx, int dirs, int steps, int skip);
-/* Diagnostic (new; tests): kept-neighbour entries per wave of the grouped reflection's scratch
-   (receiver_queue 2 or 4): 64..4096, 0 = 4096.  A group whose lists hold more is walked receiver
-   by receiver; a small cap forces that path. */
-int sgpu_set_group_scratch_cap(sgpu_ctx* ctx, int entries);
+/* Diagnostic (new; tests): receiver pairs (receiver_queue 2) whose kept neighbour lists hold more
+   than `entries` entries together are walked receiver by receiver (0, the default: only pairs
+   whose lists overflow the pair scratch, 2 x 256 entries in LDS + 4096 per wave); a small limit
+   forces that path. */
+int sgpu_set_pair_limit(sgpu_ctx* ctx, int entries);
 /* Synchronise the context stream. */
 int sgpu_sync(sgpu_ctx* ctx);
 
@@ -260,11 +261,11 @@ typedef struct sgpu_tuning {
                                     XCD in the tile-major order (XCD bands of xcd_band workgroups'
                                     worth of receivers; group_blocks then caps the resident
                                     workgroups per CU, 0 = as many as fit); 0: static assignment.
-                                    Both give bit-identical annual sums.  2 or 4: queued groups of
-                                    2 or 4 consecutive receivers, each walked by one wave that
-                                    gathers every neighbour row of the group once (ba-RLE lists,
-                                    16 or more hours per pass; otherwise as 1); the sums differ
-                                    from 1 only in the order of the reflection additions. */
+                                    Both give bit-identical annual sums.  2: queued pairs of
+                                    consecutive receivers, each walked by one wave that gathers
+                                    every neighbour row of the pair once (ba-RLE lists, 16 or more
+                                    hours per pass; otherwise as 1); the sums differ from 1 only in
+                                    the order of the additions. */
 } sgpu_tuning;
 int sgpu_autotune(sgpu_ctx* ctx, unsigned format, sgpu_tuning* out);
 int sgpu_set_tuning(sgpu_ctx* ctx, const sgpu_tuning* t);

@@ -361,10 +361,10 @@ class Context:
     def set_trace_table_shape(self, dirs: int = 4, steps: int = 255, skip: bool = True) -> None:
         _check(_lib.sgpu_set_trace_table_shape(self._h, int(dirs), int(steps), int(skip)))
 
-    def set_group_scratch_cap(self, entries: int = 0) -> None:
-        """Diagnostic: kept-neighbour entries per wave of the grouped reflection (0 = 4096); a
-        group whose lists hold more is walked receiver by receiver."""
-        _check(_lib.sgpu_set_group_scratch_cap(self._h, int(entries)))
+    def set_pair_limit(self, entries: int = 0) -> None:
+        """Diagnostic: receiver pairs whose kept lists hold more than `entries` entries together are
+        walked receiver by receiver (0 = only pairs that overflow the pair scratch)."""
+        _check(_lib.sgpu_set_pair_limit(self._h, int(entries)))
 
     def viewshed(self, formats: int = FMT_CSR | FMT_BARLE) -> None:
         _check(_lib.sgpu_viewshed(self._h, ctypes.c_uint(formats)))

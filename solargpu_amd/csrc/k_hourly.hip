@@ -748,20 +748,20 @@ __global__ void k_clear_queue(unsigned* q) {
     if (threadIdx.x < kXcds * kQueuesPerXcd) q[threadIdx.x * kQueueStride] = 0u;
 }
 
-/* ---- grouped receivers: one wave walks R consecutive positions and gathers each row once ---- */
+/* ---- receiver pairs: one wave walks two consecutive positions and gathers each shared row once ---- */
 
-/* The group's merged list: entry i is a neighbour row and its R weights ws[i][j] (zero where the
- * row is not in receiver j's kept list).  Each row is gathered once and added into the R sums;
- * lanes hold hours as in apply_list_at. */
-template <int K, int R, bool OFF32>
-__device__ __forceinline__ void apply_group_at(const int* ids, const float (*ws)[R], int cnt,
-                                               const float* __restrict__ top, int lane, int swz, float (&acc)[R]) {
-    constexpr int HPL = (K >= 16) ? 4 : 1;
-    constexpr int LPE = K / HPL;
+/* Rows ids[i] of a merged list added with two weights into two sums (lanes hold hours as in
+ * apply_list_at, K >= 16: four hours per lane). */
+template <int K, bool OFF32>
+__device__ __forceinline__ void apply_pair_at(const int* ids, const float* w0, const float* w1, int cnt,
+                                              const float* __restrict__ top, int lane, int swz, float& acc0,
+                                              float& acc1) {
+    static_assert(K >= 16, "four hours per lane");
+    constexpr int LPE = K / 4;
     constexpr int G = 64 / LPE;
-    typedef float vec __attribute__((ext_vector_type(HPL)));
-    constexpr int B = 16 / HPL;   // 16 VGPRs of rows in flight
-    const unsigned lane_off = (unsigned)((lane % LPE) * HPL) * 4u;
+    typedef float vec __attribute__((ext_vector_type(4)));
+    constexpr int B = 4;   // 16 VGPRs of rows in flight
+    const unsigned lane_off = (unsigned)((lane % LPE) * 4) * 4u;
     const char* lane_top = reinterpret_cast<const char*>(top) + lane_off;
     constexpr int kRowShift = __builtin_ctz(K * 4);
     auto load = [&](int cell) __attribute__((always_inline)) {
@@ -773,9 +773,7 @@ __device__ __forceinline__ void apply_group_at(const int* ids, const float (*ws)
             return *reinterpret_cast<const vec*>(lane_top + ((uint64_t)id << kRowShift));
         }
     };
-    vec s[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) s[j] = vec{};
+    vec s0 = {}, s1 = {};
     int i = lane / LPE;
     for (; i + (B - 1) * G < cnt; i += B * G) {
         vec x[B];
@@ -783,54 +781,75 @@ __device__ __forceinline__ void apply_group_at(const int* ids, const float (*ws)
         for (int u = 0; u < B; ++u) x[u] = load(ids[i + u * G]);
         __builtin_amdgcn_sched_group_barrier(0x0020, B, 0);   // the VMEM reads first
 #pragma unroll
-        for (int u = 0; u < B; ++u)
+        for (int u = 0; u < B; ++u) {
+            const float a = w0[i + u * G], b = w1[i + u * G];
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const float w = ws[i + u * G][j];
-#pragma unroll
-                for (int h = 0; h < HPL; ++h) s[j][h] = fmaf(w, x[u][h], s[j][h]);
+            for (int h = 0; h < 4; ++h) {
+                s0[h] = fmaf(a, x[u][h], s0[h]);
+                s1[h] = fmaf(b, x[u][h], s1[h]);
             }
+        }
     }
     for (; i < cnt; i += G) {
         const vec x = load(ids[i]);
+        const float a = w0[i], b = w1[i];
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const float w = ws[i][j];
-#pragma unroll
-            for (int h = 0; h < HPL; ++h) s[j][h] = fmaf(w, x[h], s[j][h]);
+        for (int h = 0; h < 4; ++h) {
+            s0[h] = fmaf(a, x[h], s0[h]);
+            s1[h] = fmaf(b, x[h], s1[h]);
         }
     }
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-        float t = 0.0f;
-#pragma unroll
-        for (int h = 0; h < HPL; ++h) t += s[j][h];
-        acc[j] += t;
+    acc0 += (s0[0] + s0[1]) + (s0[2] + s0[3]);
+    acc1 += (s1[0] + s1[1]) + (s1[2] + s1[3]);
+}
+
+/* Each receiver's own irradiance over the pass, for the paired walk: own[q] = the sum over the pass's
+ * hours, one thread per position (voxel v = p in full-scene order, else subset[p]; q = p). */
+template <int K>
+__global__ __launch_bounds__(256) void k_own_pass(HourArgs a, float* __restrict__ own) {
+    __shared__ HourConst hrs[K];
+    stage_hours<K>(a, hrs);
+    __syncthreads();
+    const int64_t n = a.order ? a.n_total : a.n_sub;
+    const float inv_w = 1.0f / (float)a.W;
+    const int nh = min(K, a.nh);
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
+        const int64_t v = a.order ? p : (int64_t)a.subset[p];
+        const Recv r = receiver_of(a, v, inv_w);
+        float s = 0.0f;
+        for (int h = 0; h < nh; ++h) {
+            const HourConst& hc = hrs[h];
+            s += hour_irradiance(hc, r.cs, r.ss, r.ca, r.sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
+        }
+        own[p] = s;
     }
 }
 
-/* Per-wave stream window of the grouped kernel. */
+/* Per-wave stream window of the paired kernel, and the entries of each kept list held in LDS (the
+   rest go to the wave's scratch). */
 constexpr int kGroupStageBytes = 1024;
+constexpr int kPairLds = 256;
 
-/* One wave's LDS in the grouped kernel.  dec: while the group's streams are decoded, the stream
- * window and the decoded ids not yet weighed (and, for a group walked receiver by receiver, its
- * compacted list); mrg: while the lists are merged, the 64-entry heads of the R lists and the
- * merged list with R weights per entry. */
-template <int R>
-union GroupLds {
-    struct {
-        int list_id[kListCap];
-        float list_w[kListCap];
-        __align__(16) uint8_t stage[kGroupStageBytes];
-        int pend_id[128];
-    } dec;
-    struct {
-        int cid[R][64];
-        float cw[R][64];
-        int gid[kListCap];
-        float gw[kListCap][R];
-    } mrg;
+/* One wave's LDS in the paired kernel: the first kPairLds entries of both kept lists; while the
+ * streams are decoded, the stream window and the decoded ids not yet weighed; while the lists are
+ * merged, their 64-entry heads and, per entry of the first head, the second receiver's weight.  A
+ * pair walked receiver by receiver uses lid[0] / lw[0] as its compacted list. */
+struct PairLds {
+    int lid[2][kPairLds];
+    float lw[2][kPairLds];
+    union {
+        struct {
+            __align__(16) uint8_t stage[kGroupStageBytes];
+            int pend_id[128];
+        } dec;
+        struct {
+            int cid[2][64];
+            float cw[2][64];
+            float wx[64];
+        } mrg;
+    } u;
 };
+static_assert(kPairLds >= kListCap, "the receiver-by-receiver walk's list fits lid[0]");
 
 /* Position p of the full scene in processing order (voxel order[p], lists and sums by voxel) or
    of a subset (voxel subset[p], lists and sums by position): its voxel and its list / sum index. */
@@ -839,187 +858,200 @@ __device__ __forceinline__ void queue_voxel(const HourArgs& a, int64_t p, int64_
     q = a.order ? v : p;
 }
 
-/* Grouped receivers, queued as k_reflect_queue but by groups of R consecutive positions (XCD bands
- * of queue_band / R groups).  Receivers next to each other in the processing order share most of
- * their kept neighbours (a top cell and its walls, walls stacked on one cell, adjacent cells): a
- * wave that walks the group's lists together gathers each shared row once.
- * Phase 1: each receiver's ba-RLE stream is decoded and weighed; its kept (id, weight) entries,
- *   in stream order (descending ids), go to the wave's scratch, list j at [lo_j, lo_(j+1)).
- * Phase 2: the R descending lists are merged 64-entry heads at a time.  With t the largest of the
- *   64th head entries among the lists that have more than 64 left, every entry >= t of every list
- *   is in its head (later entries are smaller than their list's 64th); those are taken, each id
- *   once, by the first list holding it, with the weights the other lists give it (a binary search
- *   of their heads), into the merged list, which is applied 64 to 128 entries at a time.  The list
- *   that set t gives all 64 of its head, so every round makes progress.
- * A group whose entries overflow the scratch is walked receiver by receiver (receiver_parts).
- * Each receiver's own irradiance and the Kahan add are as in k_reflect_queue; its reflected sum
- * differs from the receiver-by-receiver walk only in the order of the additions. */
-template <int K, int WAVES, int R>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(R == 2 ? 7 : 5)))
-void k_reflect_group(HourArgs a) {
-    static_assert(R == 2 || R == 4, "groups of 2 or 4 receivers");
-    __shared__ GroupLds<R> GL[WAVES];
-    __shared__ HourConst hrs[K];
-    stage_hours<K>(a, hrs);
-    __syncthreads();
+/* Index of the first of the n descending entries c[0..n) that is <= x (n <= 64): the number greater. */
+__device__ __forceinline__ int rank_desc(const int* c, int n, int x) {
+    int pos = 0;
+#pragma unroll
+    for (int st = 64; st >= 1; st >>= 1)
+        if (pos + st <= n && c[pos + st - 1] > x) pos += st;
+    return pos;
+}
+
+/* Receiver pairs, queued as k_reflect_queue but by pairs of consecutive positions (XCD bands of
+ * queue_band / 2 pairs).  Receivers next to each other in the processing order share most of their
+ * kept neighbours (a top cell and its walls, walls stacked on one cell, adjacent cells): a wave
+ * that walks the two lists together gathers each shared row once.
+ * Phase 1: each receiver's ba-RLE stream is decoded and weighed; its kept (id, weight) entries, in
+ *   stream order (descending ids), go to list j: entries k < kPairLds in LDS, the rest to the wave's
+ *   scratch (list j's part from sb_j).
+ * Phase 2: the two descending lists are merged 64-entry heads at a time.  With t the larger of the
+ *   64th head entries of the lists that have more than 64 left, every entry >= t of either list is
+ *   in its head (later entries are smaller than their list's 64th); those are taken: the first
+ *   head's with the second receiver's weight where the second head holds the id (binary search),
+ *   the second head's where the first does not hold it.  The list that set t gives all 64 of its
+ *   head, so every round makes progress.  Rows of the first head's entries are added into both sums
+ *   (a zero weight where the second list lacks the id); the second head's remaining rows into the
+ *   second receiver's only.
+ * A pair whose lists overflow the scratch is walked receiver by receiver (receiver_parts).  Each
+ * receiver's own irradiance comes from k_own_pass (own[q]); the annual Kahan add is as in
+ * k_reflect_queue.  A receiver's sum differs from the receiver-by-receiver walk only in the order of
+ * its additions, and does not depend on which wave takes the pair. */
+template <int K, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(7)))
+void k_reflect_pair(HourArgs a) {
+    static_assert(K >= 16, "four hours per lane");
+    __shared__ PairLds PL[WAVES];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     const unsigned x = blockIdx.x % kXcds;
     const unsigned c = ((blockIdx.x / kXcds) * WAVES + (unsigned)wave) % kQueuesPerXcd;
     unsigned* counter = a.queue + (x * kQueuesPerXcd + c) * kQueueStride;
     const int64_t n = a.order ? a.n_total : a.n_sub;
-    const int64_t ng = (n + R - 1) / R;
-    const int64_t band = a.queue_band > 0 ? max<int64_t>(a.queue_band / R, 1) : 0;
+    const int64_t ng = (n + 1) / 2;
+    const int64_t band = a.queue_band > 0 ? max<int64_t>(a.queue_band / 2, 1) : 0;
     const float inv_w = 1.0f / (float)a.W;
     const int cells = a.W * a.H;
     const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);
     const int cap = a.grp_cap;
     int2* scr = a.grp_scratch + ((int64_t)blockIdx.x * WAVES + wave) * cap;
-    GroupLds<R>& L = GL[wave];
-    StreamWindow win{L.dec.stage, a.nb_bytes, 0, 0, 0, kGroupStageBytes};
+    PairLds& L = PL[wave];
+    StreamWindow win{L.u.dec.stage, a.nb_bytes, 0, 0, 0, kGroupStageBytes};
     unsigned pending = atomicAdd(counter, lane == 0 ? 1u : 0u);   // every lane joins (see k_reflect_queue)
     for (int64_t turn = 0; turn <= ng; ++turn) {
         const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)pending);
         const int64_t g = queue_position((uint64_t)item * kQueuesPerXcd + c, x, ng, band);
         if (g >= ng) break;
         pending = atomicAdd(counter, lane == 0 ? 1u : 0u);
-        const int64_t p0 = g * R;
-        const int nr = (int)min<int64_t>(R, n - p0);
+        const int64_t p0 = g * 2;
+        const bool two = p0 + 1 < n;
         win.lo = win.hi = 0;   // the merge overwrote the window's bytes
 
-        /* phase 1: lane k (1 <= k <= R) ends with the end of list k - 1 (lists past nr are empty) */
-        int lo_v = 0, used = 0;
-        for (int j = 0; j < nr; ++j) {
+        /* phase 1 */
+        int nl[2] = {0, 0}, sb[2] = {0, 0};
+        int spill = 0;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j == 1 && !two) continue;
             int64_t v, q;
             queue_voxel(a, p0 + j, v, q);
             const Recv r = receiver_of(a, v, inv_w);
             const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
             win.end = s1;
+            int cnt = 0;
+            sb[j] = spill;
             auto take = [&](bool valid, int id) __attribute__((always_inline)) {
                 const float w = valid ? neighbour_weight(id, r, a.grid, a.W, cells, inv_w) : 0.0f;
                 const bool keep = w > 0.0f;
                 const uint64_t m = __ballot(keep);
-                const int at = used + lanes_below(m);
-                if (keep && at < cap) scr[at] = make_int2(id, __float_as_int(w));
-                used += __popcll(m);
+                const int at = cnt + lanes_below(m);
+                if (keep) {
+                    if (at < kPairLds) {
+                        L.lid[j][at] = id;
+                        L.lw[j][at] = w;
+                    } else if (spill + (at - kPairLds) < cap) {
+                        scr[spill + (at - kPairLds)] = make_int2(id, __float_as_int(w));
+                    }
+                }
+                cnt += __popcll(m);
             };
-            decode_chunks(win, s0, s1, L.dec.pend_id, lane, take);
-            if (lane > j) lo_v = used;
+            decode_chunks(win, s0, s1, L.u.dec.pend_id, lane, take);
+            nl[j] = cnt;
+            spill += max(cnt - kPairLds, 0);
         }
-        if (used > cap) {   // walked receiver by receiver
-            for (int j = 0; j < nr; ++j) {
+        if (spill > cap || (a.grp_limit > 0 && nl[0] + nl[1] > a.grp_limit)) {
+            /* walked receiver by receiver, as k_reflect_queue does (the hour constants read from
+               global memory) */
+            for (int j = 0; j < (two ? 2 : 1); ++j) {
                 int64_t v, q;
                 queue_voxel(a, p0 + j, v, q);
                 const Recv r = receiver_of(a, v, inv_w);
                 const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
                 win.end = s1;
                 float own, refl;
-                receiver_parts<K, SGPU_FMT_BARLE>(a, hrs, r, v, s0, s1, win, L.dec.list_id, L.dec.list_w,
-                                                  L.dec.pend_id, lane, off32, inv_w, cells, own, refl);
+                wave_lds_sync();
+                receiver_parts<K, SGPU_FMT_BARLE>(a, a.hours, r, v, s0, s1, win, L.lid[0], L.lw[0],
+                                                  L.u.dec.pend_id, lane, off32, inv_w, cells, own, refl);
                 const float t = wave_sum(own + refl);
                 if (lane == 0) kahan_add(a.sum, a.comp, q, t);
             }
             continue;
         }
 
-        /* phase 2: merge (the scratch entries written by other lanes are visible after the fence
-           of wave_lds_sync) */
+        /* phase 2 (the scratch entries and LDS lists written by other lanes are visible after the
+           fence of wave_lds_sync) */
         wave_lds_sync();
-        int h[R], e[R];
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            h[j] = __builtin_amdgcn_readlane(lo_v, j);
-            e[j] = __builtin_amdgcn_readlane(lo_v, j + 1);
-        }
-        float acc[R];
-#pragma unroll
-        for (int j = 0; j < R; ++j) acc[j] = 0.0f;
-        int gcnt = 0;
-        for (;;) {
-            bool more = false;
-#pragma unroll
-            for (int j = 0; j < R; ++j) more |= h[j] < e[j];
-            if (!more) break;
-            int idv[R];
-            float wv[R];
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int k = h[j] + lane;
-                idv[j] = -1;
-                wv[j] = 0.0f;
-                if (k < e[j]) {
-                    const int2 ent = scr[k];
-                    idv[j] = ent.x;
-                    wv[j] = __int_as_float(ent.y);
+        float acc0 = 0.0f, acc1 = 0.0f;
+        int h0 = 0, h1 = 0;
+        while (h0 < nl[0] || h1 < nl[1]) {
+            int id0 = -1, id1 = -1;
+            float w0 = 0.0f, w1 = 0.0f;
+            {
+                const int k = h0 + lane;
+                if (k < nl[0]) {
+                    if (k < kPairLds) {
+                        id0 = L.lid[0][k];
+                        w0 = L.lw[0][k];
+                    } else {
+                        const int2 e = scr[sb[0] + (k - kPairLds)];
+                        id0 = e.x;
+                        w0 = __int_as_float(e.y);
+                    }
+                }
+            }
+            {
+                const int k = h1 + lane;
+                if (k < nl[1]) {
+                    if (k < kPairLds) {
+                        id1 = L.lid[1][k];
+                        w1 = L.lw[1][k];
+                    } else {
+                        const int2 e = scr[sb[1] + (k - kPairLds)];
+                        id1 = e.x;
+                        w1 = __int_as_float(e.y);
+                    }
                 }
             }
             int t = -1;
-#pragma unroll
-            for (int j = 0; j < R; ++j)
-                if (e[j] - h[j] > 64) t = max(t, __builtin_amdgcn_readlane(idv[j], 63));
-            const int tmin = max(t, 0);   // t < 0: every list ends within its head
-            int nj[R];
+            if (nl[0] - h0 > 64) t = max(t, __builtin_amdgcn_readlane(id0, 63));
+            if (nl[1] - h1 > 64) t = max(t, __builtin_amdgcn_readlane(id1, 63));
+            const int tmin = max(t, 0);   // t < 0: both lists end within their heads
+            const bool in0 = id0 >= tmin, in1 = id1 >= tmin;   // prefixes of the descending heads
+            const int n0 = __popcll(__ballot(in0)), n1 = __popcll(__ballot(in1));
             wave_lds_sync();   // the previous round's reads of the heads are done
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                L.mrg.cid[j][lane] = idv[j];
-                L.mrg.cw[j][lane] = wv[j];
-                nj[j] = __popcll(__ballot(idv[j] >= tmin));   // a prefix of the descending head
+            L.u.mrg.cid[0][lane] = id0;
+            L.u.mrg.cid[1][lane] = id1;
+            L.u.mrg.cw[0][lane] = w0;
+            L.u.mrg.cw[1][lane] = w1;
+            wave_lds_sync();
+            float wx = 0.0f;
+            if (in0) {
+                const int pos = rank_desc(L.u.mrg.cid[1], n1, id0);
+                if (pos < n1 && L.u.mrg.cid[1][pos] == id0) wx = L.u.mrg.cw[1][pos];
+            }
+            bool rest = false;   // in the second head only
+            if (in1) {
+                const int pos = rank_desc(L.u.mrg.cid[0], n0, id1);
+                rest = !(pos < n0 && L.u.mrg.cid[0][pos] == id1);
+            }
+            const uint64_t mr = __ballot(rest);
+            const int nr = __popcll(mr);
+            wave_lds_sync();   // the searches of both heads are done
+            L.u.mrg.wx[lane] = wx;
+            if (rest) {
+                const int at = lanes_below(mr);
+                L.u.mrg.cid[1][at] = id1;
+                L.u.mrg.cw[1][at] = w1;
             }
             wave_lds_sync();
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int xid = idv[j];
-                bool first = lane < nj[j];
-                float w[R];
-#pragma unroll
-                for (int i = 0; i < R; ++i) {
-                    w[i] = 0.0f;
-                    if (i == j) {
-                        w[i] = wv[j];
-                        continue;
-                    }
-                    /* pos: the entries of head i (prefix of nj[i]) greater than xid */
-                    int pos = 0;
-#pragma unroll
-                    for (int st = 64; st >= 1; st >>= 1)
-                        if (pos + st <= nj[i] && L.mrg.cid[i][pos + st - 1] > xid) pos += st;
-                    const bool found = pos < nj[i] && L.mrg.cid[i][pos] == xid;
-                    if (i < j) first = first && !found;
-                    else if (found) w[i] = L.mrg.cw[i][pos];
-                }
-                const uint64_t m = __ballot(first);
-                if (first) {
-                    const int at = gcnt + lanes_below(m);
-                    L.mrg.gid[at] = xid;
-#pragma unroll
-                    for (int i = 0; i < R; ++i) L.mrg.gw[at][i] = w[i];
-                }
-                gcnt += __popcll(m);
-                if (gcnt > kListCap - 64) {
-                    wave_lds_sync();
-                    if (off32) apply_group_at<K, R, true>(L.mrg.gid, L.mrg.gw, gcnt, a.hourly_top, lane, a.row_swz, acc);
-                    else apply_group_at<K, R, false>(L.mrg.gid, L.mrg.gw, gcnt, a.hourly_top, lane, a.row_swz, acc);
-                    wave_lds_sync();
-                    gcnt = 0;
-                }
+            if (off32) {
+                apply_pair_at<K, true>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, a.row_swz,
+                                       acc0, acc1);
+                acc1 += apply_list_at<K, true>(L.u.mrg.cid[1], L.u.mrg.cw[1], nr, a.hourly_top, lane, a.row_swz);
+            } else {
+                apply_pair_at<K, false>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, a.row_swz,
+                                        acc0, acc1);
+                acc1 += apply_list_at<K, false>(L.u.mrg.cid[1], L.u.mrg.cw[1], nr, a.hourly_top, lane, a.row_swz);
             }
-#pragma unroll
-            for (int j = 0; j < R; ++j) h[j] += nj[j];
+            h0 += n0;
+            h1 += n1;
         }
-        if (gcnt > 0) {
-            wave_lds_sync();
-            if (off32) apply_group_at<K, R, true>(L.mrg.gid, L.mrg.gw, gcnt, a.hourly_top, lane, a.row_swz, acc);
-            else apply_group_at<K, R, false>(L.mrg.gid, L.mrg.gw, gcnt, a.hourly_top, lane, a.row_swz, acc);
-            wave_lds_sync();
-        }
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            if (j < nr) {
-                int64_t v, q;
-                queue_voxel(a, p0 + j, v, q);
-                const Recv r = receiver_of(a, v, inv_w);
-                const float tot = wave_sum(own_part<K>(a, hrs, r, v, lane) + acc[j]);
-                if (lane == 0) kahan_add(a.sum, a.comp, q, tot);
+        const float r0 = wave_sum(acc0), r1 = wave_sum(acc1);
+        if (lane == 0) {
+            int64_t v, q;
+            queue_voxel(a, p0, v, q);
+            kahan_add(a.sum, a.comp, q, a.grp_own[q] + r0);
+            if (two) {
+                queue_voxel(a, p0 + 1, v, q);
+                kahan_add(a.sum, a.comp, q, a.grp_own[q] + r1);
             }
         }
     }
@@ -1237,26 +1269,18 @@ void reflect_queue_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
     }
 }
 
-template <int K, int WAVES, int R>
-void reflect_group_kwr(const HourArgs& a, hipStream_t s) {
-    static const GridShape g = occupancy(k_reflect_group<K, WAVES, R>, 64 * WAVES);
+template <int K, int WAVES>
+void reflect_pair_kw(const HourArgs& a, hipStream_t s) {
+    static const GridShape g = occupancy(k_reflect_pair<K, WAVES>, 64 * WAVES);
     const unsigned grid = queue_grid(g, a.queue_blocks_per_cu);
-    if ((int64_t)grid * WAVES > a.grp_waves || a.grp_cap < 64)
-        fail(SGPU_ERR_STATE, "group scratch holds %lld waves of %d entries; the grid has %lld waves",
+    if ((int64_t)grid * WAVES > a.grp_waves || a.grp_cap < 0 || a.grp_own == nullptr)
+        fail(SGPU_ERR_STATE, "pair scratch holds %lld waves of %d entries; the grid has %lld waves",
              (long long)a.grp_waves, a.grp_cap, (long long)grid * WAVES);
+    const int64_t n = a.order ? a.n_total : a.n_sub;
+    const unsigned own_blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_own_pass<K>, dim3(own_blocks), dim3(256), 0, s, a, a.grp_own);
     hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(kXcds * kQueuesPerXcd), 0, s, a.queue);
-    hipLaunchKernelGGL((k_reflect_group<K, WAVES, R>), dim3(grid), dim3(64 * WAVES), 0, s, a);
-}
-
-template <int K>
-void reflect_group_k(const HourArgs& a, hipStream_t s) {
-    if (a.grp_size == 4) {
-        if (a.waves == 2) reflect_group_kwr<K, 2, 4>(a, s);
-        else reflect_group_kwr<K, 4, 4>(a, s);
-    } else {
-        if (a.waves == 2) reflect_group_kwr<K, 2, 2>(a, s);
-        else reflect_group_kwr<K, 4, 2>(a, s);
-    }
+    hipLaunchKernelGGL((k_reflect_pair<K, WAVES>), dim3(grid), dim3(64 * WAVES), 0, s, a);
 }
 
 /* XCD bands of the queued batched reflection: 65536 positions (what the single-scene tuning picks
@@ -1284,9 +1308,10 @@ void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
                         ((a.order != nullptr && a.subset == nullptr && a.n_sub == 0) ||
                          (a.order == nullptr && a.subset != nullptr && a.n_sub > 0));
     if (queued) {
-        if constexpr (K >= 16) {   // grouped receivers: ba-RLE lists, 4 hours per lane
-            if (fmt == SGPU_FMT_BARLE && a.grp_size >= 2 && a.grp_scratch != nullptr) {
-                reflect_group_k<K>(a, s);
+        if constexpr (K >= 16) {   // receiver pairs: ba-RLE lists, 4 hours per lane
+            if (fmt == SGPU_FMT_BARLE && a.grp_size == 2 && a.grp_scratch != nullptr) {
+                if (a.waves == 2) reflect_pair_kw<K, 2>(a, s);
+                else reflect_pair_kw<K, 4>(a, s);
                 return;
             }
         }

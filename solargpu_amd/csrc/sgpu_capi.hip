@@ -229,15 +229,20 @@ void ensure_group_scratch(sgpu_ctx* c) {
     const int64_t waves = (int64_t)std::max(cus, 1) * kMaxWavesPerCu;
     c->grp_scratch = dalloc<int2>((size_t)waves * kGroupCap);
     c->grp_waves = waves;
+    c->grp_own = dalloc<float>((size_t)std::max<int64_t>(c->n_total, 1));
 }
 
 void set_group_args(const sgpu_ctx* c, HourArgs& a, int queue) {
     a.grp_size = 0;
     a.grp_scratch = nullptr;
-    if (queue >= 2 && c->grp_scratch) {
-        a.grp_size = queue;
+    a.grp_own = nullptr;
+    a.grp_limit = 0;
+    if (queue == 2 && c->grp_scratch) {
+        a.grp_size = 2;
         a.grp_scratch = c->grp_scratch;
-        a.grp_cap = c->group_cap > 0 ? c->group_cap : kGroupCap;
+        a.grp_own = c->grp_own;
+        a.grp_cap = kGroupCap;
+        a.grp_limit = c->group_limit;
         a.grp_waves = c->grp_waves;
     }
 }
@@ -1755,13 +1760,11 @@ int sgpu_set_trace_tables(sgpu_ctx* c, int enable, int64_t pending_cap) {
     });
 }
 
-int sgpu_set_group_scratch_cap(sgpu_ctx* c, int entries) {
+int sgpu_set_pair_limit(sgpu_ctx* c, int entries) {
     return guarded([&] {
         need(c != nullptr, SGPU_ERR_ARG, "null context");
-        need(entries == 0 || (entries >= 64 && entries <= kGroupCap), SGPU_ERR_ARG,
-             "group scratch cap must be 0 or 64..4096 entries");
-        static_assert(kGroupCap == 4096, "the message above states the cap");
-        c->group_cap = entries;
+        need(entries >= 0, SGPU_ERR_ARG, "pair limit must be >= 0");
+        c->group_limit = entries;
     });
 }
 

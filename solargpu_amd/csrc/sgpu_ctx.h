@@ -196,9 +196,10 @@ struct sgpu_ctx {
        shape the autotuner picks on the 2048^2 scene, until a scene of its own is tuned */
     sgpu_tuning tune{32, 128, 4, 32, 0, 0, 0.0, 0.0, 1};
     unsigned* reflect_queue = nullptr;                   // kXcds counters of the queued reflection
-    int2* grp_scratch = nullptr;                         // kGroupCap entries per wave, grouped reflection
+    int2* grp_scratch = nullptr;                         // kGroupCap entries per wave, paired reflection
+    float* grp_own = nullptr;                            // own irradiance per position over a pass
     int64_t grp_waves = 0;                               // waves grp_scratch covers
-    int group_cap = 0;                                   // entries per wave used (0: kGroupCap)
+    int group_limit = 0;                                 // kept entries walked as a pair at most (0: any)
     int residency = SGPU_LISTS_DEVICE;          // where sgpu_viewshed leaves the lists
     sgpu::HostLists host;                       // the lists when host-resident
     hipStream_t copy_stream = nullptr;          // H2D tile copies of the streamed hour loop
@@ -249,6 +250,7 @@ struct sgpu_ctx {
         sgpu::dfree(hourly_top);
         sgpu::dfree(reflect_queue);
         sgpu::dfree(grp_scratch);
+        sgpu::dfree(grp_own);
         grp_waves = 0;
         hourly_top_elems = 0;
         scene = false;

@@ -355,17 +355,21 @@ struct HourArgs {
     unsigned* queue;
     int64_t queue_band;
     int queue_blocks_per_cu;   // resident workgroups per CU of the queued grid (0 = as many as fit)
-    /* Grouped receivers (queued, ba-RLE, K >= 16): the queue hands out groups of grp_size (2 or 4)
+    /* Receiver pairs (queued, ba-RLE, K >= 16; grp_size == 2): the queue hands out pairs of
        consecutive positions, each walked by one wave that gathers every kept neighbour row of the
-       group once.  Wave w of the grid keeps the group's (id, weight) entries in its grp_cap entries
-       of grp_scratch, from w * grp_cap; the buffer covers grp_waves waves.  grp_size <= 1 or no
-       scratch: receivers one by one. */
+       pair once.  Kept entries beyond the first 256 of a list go to wave w's grp_cap entries of
+       grp_scratch, from w * grp_cap; the buffer covers grp_waves waves.  grp_own: each position's
+       own irradiance over the pass (by list / sum index, n_total floats), written by the launcher's
+       first kernel.  grp_limit > 0: pairs with more kept entries are walked receiver by receiver.
+       grp_size <= 1 or no scratch: receivers one by one. */
     int2* grp_scratch;
+    float* grp_own;
     int grp_cap;
     int grp_size;
+    int grp_limit;
     int64_t grp_waves;
 };
-/* Entries per wave of the group scratch (8 bytes each), and the most waves a CU holds at once. */
+/* Entries per wave of the pair scratch (8 bytes each), and the most waves a CU holds at once. */
 constexpr int kGroupCap = 4096;
 constexpr int kMaxWavesPerCu = 32;
 /* Rows of the hourly buffer for n_top cells: swizzled rows stay inside aligned blocks of 1024. */

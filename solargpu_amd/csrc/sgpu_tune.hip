@@ -169,8 +169,7 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
             for (int waves : {2, 4}) {
                 for (int band : {0, 64, 256}) trial(K, waves, band, 0, 0);
                 for (int band : {32, 256, 512}) trial(K, waves, band, 0, 1);   // queued receivers
-                if (fmt == SGPU_FMT_BARLE && waves == 4)   // queued groups of 2 or 4 receivers
-                    for (int grp : {2, 4}) trial(K, waves, 512, 0, grp);
+                if (fmt == SGPU_FMT_BARLE && waves == 4) trial(K, waves, 512, 0, 2);   // queued receiver pairs
             }
         const int K = t.hours_per_pass;
         for (int waves : {2, 4})   // the best waves count with receiver groups can differ
@@ -215,8 +214,7 @@ int sgpu_set_tuning(sgpu_ctx* c, const sgpu_tuning* t) {
         need(t->waves_per_group == 2 || t->waves_per_group == 4, SGPU_ERR_ARG, "waves_per_group must be 2 or 4");
         need(t->xcd_band == 0 || (t->xcd_band >= 16 && t->xcd_band <= 4096), SGPU_ERR_ARG,
              "xcd_band must be 0 or 16..4096");
-        need(t->receiver_queue == 0 || t->receiver_queue == 1 || t->receiver_queue == 2 || t->receiver_queue == 4,
-             SGPU_ERR_ARG, "receiver_queue must be 0, 1, 2 or 4");
+        need(t->receiver_queue >= 0 && t->receiver_queue <= 2, SGPU_ERR_ARG, "receiver_queue must be 0, 1 or 2");
         if (t->receiver_queue)
             need(t->group_blocks >= 0 && t->group_blocks <= 64, SGPU_ERR_ARG,
                  "group_blocks (workgroups per CU of the receiver queue) must be 0..64");
