@@ -270,8 +270,7 @@ def main():
     # roofline of the dominant kernel: reflection + Kahan accumulate (ba-RLE decode)
     Kp = inf["hours_per_pass"]
     rq = ctx.tuning()["receiver_queue"]
-    refl_kernel = (f"k_reflect_group<K={Kp}, {rq} receivers per wave, ba-RLE>" if rq >= 2 and Kp >= 16
-                   else f"k_reflect<K={Kp}, ba-RLE>")
+    refl_kernel = f"k_reflect_pair<K={Kp}, ba-RLE>" if rq == 2 and Kp >= 16 else f"k_reflect<K={Kp}, ba-RLE>"
     passes = inf["hour_passes"]
     nh_list = [min(Kp, H - p * Kp) for p in range(passes)]
     bytes_all = sum(reflect_pass_bytes(N, ntop, inf["barle_bytes"], nh) for nh in nh_list)

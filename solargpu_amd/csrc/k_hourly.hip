@@ -1276,11 +1276,29 @@ void reflect_pair_kw(const HourArgs& a, hipStream_t s) {
     if ((int64_t)grid * WAVES > a.grp_waves || a.grp_cap < 0 || a.grp_own == nullptr)
         fail(SGPU_ERR_STATE, "pair scratch holds %lld waves of %d entries; the grid has %lld waves",
              (long long)a.grp_waves, a.grp_cap, (long long)grid * WAVES);
-    const int64_t n = a.order ? a.n_total : a.n_sub;
-    const unsigned own_blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_own_pass<K>, dim3(own_blocks), dim3(256), 0, s, a, a.grp_own);
     hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(kXcds * kQueuesPerXcd), 0, s, a.queue);
     hipLaunchKernelGGL((k_reflect_pair<K, WAVES>), dim3(grid), dim3(64 * WAVES), 0, s, a);
+}
+
+/* The receivers' own irradiance of the pass for the paired walk (launched with the pass's solar
+   term, before the reflection). */
+template <int K>
+void own_pass_k(const HourArgs& a, hipStream_t s) {
+    if constexpr (K >= 16) {
+        const int64_t n = a.order ? a.n_total : a.n_sub;
+        const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_own_pass<K>, dim3(blocks), dim3(256), 0, s, a, a.grp_own);
+    }
+}
+
+/* Whether the reflection pass of these arguments walks receiver pairs: queued (the full scene in
+   processing order, or a subset), ba-RLE lists, K >= 16 and the pair buffers present. */
+bool walks_pairs(const HourArgs& a, int K, unsigned fmt) {
+    const bool queued = a.queue != nullptr && a.dbg_hourly == nullptr &&
+                        ((a.order != nullptr && a.subset == nullptr && a.n_sub == 0) ||
+                         (a.order == nullptr && a.subset != nullptr && a.n_sub > 0));
+    return queued && K >= 16 && fmt == SGPU_FMT_BARLE && a.grp_size == 2 && a.grp_scratch != nullptr &&
+           a.grp_own != nullptr;
 }
 
 /* XCD bands of the queued batched reflection: 65536 positions (what the single-scene tuning picks
@@ -1309,7 +1327,7 @@ void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
                          (a.order == nullptr && a.subset != nullptr && a.n_sub > 0));
     if (queued) {
         if constexpr (K >= 16) {   // receiver pairs: ba-RLE lists, 4 hours per lane
-            if (fmt == SGPU_FMT_BARLE && a.grp_size == 2 && a.grp_scratch != nullptr) {
+            if (walks_pairs(a, K, fmt)) {
                 if (a.waves == 2) reflect_pair_kw<K, 2>(a, s);
                 else reflect_pair_kw<K, 4>(a, s);
                 return;
@@ -1353,6 +1371,18 @@ void launch_solar_top(const HourArgs& a, int K, hipStream_t s) {
         case 128: solar_top_k<128>(a, s); break;
         case 256: solar_top_k<256>(a, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
+    }
+    /* the receivers' own irradiance, when the reflection walks pairs (the stream buffer marks ba-RLE
+       lists) */
+    if (walks_pairs(a, K, a.nb_bytes != nullptr ? SGPU_FMT_BARLE : SGPU_FMT_CSR)) {
+        switch (K) {
+            case 16: own_pass_k<16>(a, s); break;
+            case 32: own_pass_k<32>(a, s); break;
+            case 64: own_pass_k<64>(a, s); break;
+            case 128: own_pass_k<128>(a, s); break;
+            case 256: own_pass_k<256>(a, s); break;
+            default: break;
+        }
     }
     SGPU_HIP(hipGetLastError());
 }
