@@ -750,57 +750,48 @@ __global__ void k_clear_queue(unsigned* q) {
 
 /* ---- receiver pairs: one wave walks two consecutive positions and gathers each shared row once ---- */
 
-/* Rows ids[i] of a merged list added with two weights into two sums (lanes hold hours as in
- * apply_list_at, K >= 16: four hours per lane). */
-template <int K, bool OFF32>
-__device__ __forceinline__ void apply_pair_at(const int* ids, const float* w0, const float* w1, int cnt,
-                                              const float* __restrict__ top, int lane, int swz, float& acc0,
-                                              float& acc1) {
+/* Rows slots[i] (row slots of the hourly buffer) of a merged list, added with NW = 1 or 2 weights
+ * into as many sums (lanes hold hours as in apply_list_at, K >= 16: four hours per lane; a row per
+ * load instruction at K = 256, its slot read into a scalar register). */
+template <int K, bool OFF32, int NW>
+__device__ __forceinline__ void apply_slots_at(const int* slots, const float* w0, const float* w1, int cnt,
+                                               const float* __restrict__ top, int lane, float& acc0, float& acc1) {
     static_assert(K >= 16, "four hours per lane");
     constexpr int LPE = K / 4;
     constexpr int G = 64 / LPE;
     typedef float vec __attribute__((ext_vector_type(4)));
     constexpr int B = 4;   // 16 VGPRs of rows in flight
     const unsigned lane_off = (unsigned)((lane % LPE) * 4) * 4u;
-    const char* lane_top = reinterpret_cast<const char*>(top) + lane_off;
     constexpr int kRowShift = __builtin_ctz(K * 4);
-    auto load = [&](int cell) __attribute__((always_inline)) {
-        const unsigned id = row_slot((unsigned)cell, swz);
-        if constexpr (OFF32) {
-            const unsigned off = (unsigned)id * (unsigned)(K * 4) + lane_off;
-            return *reinterpret_cast<const vec*>(reinterpret_cast<const char*>(top) + off);
-        } else {
-            return *reinterpret_cast<const vec*>(lane_top + ((uint64_t)id << kRowShift));
-        }
+    auto load = [&](int slot) __attribute__((always_inline)) {
+        if constexpr (G == 1) slot = __builtin_amdgcn_readfirstlane(slot);
+        const char* row = reinterpret_cast<const char*>(top) +
+                          (OFF32 ? (uint64_t)((unsigned)slot * (unsigned)(K * 4)) : ((uint64_t)(unsigned)slot << kRowShift));
+        return *reinterpret_cast<const vec*>(row + lane_off);
     };
     vec s0 = {}, s1 = {};
+    auto add = [&](const vec& x, int e) __attribute__((always_inline)) {
+        const float a = w0[e];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) s0[h] = fmaf(a, x[h], s0[h]);
+        if constexpr (NW == 2) {
+            const float b = w1[e];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) s1[h] = fmaf(b, x[h], s1[h]);
+        }
+    };
     int i = lane / LPE;
     for (; i + (B - 1) * G < cnt; i += B * G) {
         vec x[B];
 #pragma unroll
-        for (int u = 0; u < B; ++u) x[u] = load(ids[i + u * G]);
+        for (int u = 0; u < B; ++u) x[u] = load(slots[i + u * G]);
         __builtin_amdgcn_sched_group_barrier(0x0020, B, 0);   // the VMEM reads first
 #pragma unroll
-        for (int u = 0; u < B; ++u) {
-            const float a = w0[i + u * G], b = w1[i + u * G];
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                s0[h] = fmaf(a, x[u][h], s0[h]);
-                s1[h] = fmaf(b, x[u][h], s1[h]);
-            }
-        }
+        for (int u = 0; u < B; ++u) add(x[u], i + u * G);
     }
-    for (; i < cnt; i += G) {
-        const vec x = load(ids[i]);
-        const float a = w0[i], b = w1[i];
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            s0[h] = fmaf(a, x[h], s0[h]);
-            s1[h] = fmaf(b, x[h], s1[h]);
-        }
-    }
+    for (; i < cnt; i += G) add(load(slots[i]), i);
     acc0 += (s0[0] + s0[1]) + (s0[2] + s0[3]);
-    acc1 += (s1[0] + s1[1]) + (s1[2] + s1[3]);
+    if constexpr (NW == 2) acc1 += (s1[0] + s1[1]) + (s1[2] + s1[3]);
 }
 
 /* Each receiver's own irradiance over the pass, for the paired walk: own[q] = the sum over the pass's
@@ -1025,21 +1016,24 @@ void k_reflect_pair(HourArgs a) {
             const uint64_t mr = __ballot(rest);
             const int nr = __popcll(mr);
             wave_lds_sync();   // the searches of both heads are done
+            /* from here the heads hold row slots: the first's in place, the second's own compacted */
+            L.u.mrg.cid[0][lane] = (int)row_slot((unsigned)id0, a.row_swz);
             L.u.mrg.wx[lane] = wx;
             if (rest) {
                 const int at = lanes_below(mr);
-                L.u.mrg.cid[1][at] = id1;
+                L.u.mrg.cid[1][at] = (int)row_slot((unsigned)id1, a.row_swz);
                 L.u.mrg.cw[1][at] = w1;
             }
             wave_lds_sync();
+            float unused = 0.0f;
             if (off32) {
-                apply_pair_at<K, true>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, a.row_swz,
-                                       acc0, acc1);
-                acc1 += apply_list_at<K, true>(L.u.mrg.cid[1], L.u.mrg.cw[1], nr, a.hourly_top, lane, a.row_swz);
+                apply_slots_at<K, true, 2>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, acc0, acc1);
+                apply_slots_at<K, true, 1>(L.u.mrg.cid[1], L.u.mrg.cw[1], nullptr, nr, a.hourly_top, lane, acc1, unused);
             } else {
-                apply_pair_at<K, false>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, a.row_swz,
-                                        acc0, acc1);
-                acc1 += apply_list_at<K, false>(L.u.mrg.cid[1], L.u.mrg.cw[1], nr, a.hourly_top, lane, a.row_swz);
+                apply_slots_at<K, false, 2>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, acc0,
+                                            acc1);
+                apply_slots_at<K, false, 1>(L.u.mrg.cid[1], L.u.mrg.cw[1], nullptr, nr, a.hourly_top, lane, acc1,
+                                            unused);
             }
             h0 += n0;
             h1 += n1;
