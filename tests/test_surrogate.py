@@ -82,16 +82,18 @@ def test_box_raises_the_horizon_towards_it():
     assert north.min() > 60.0 and np.abs(south).max() < 1.0
 
 
-@pytest.mark.parametrize("fallback", [16, sm.FALLBACK_ANNUAL])
-def test_all_fallback_reproduces_plain(fallback):
-    """Every voxel gated: the per-hour fallback loop and the one-walk annual fallback both give
-    the plain method's annual totals (FP64 oracle, 1e-4)."""
+@pytest.mark.parametrize("fallback,queue", [(16, 1), (16, 2), (sm.FALLBACK_ANNUAL, 1)])
+def test_all_fallback_reproduces_plain(fallback, queue):
+    """Every voxel gated: the per-hour fallback loop (its receivers walked one by one, or in pairs
+    over the gated subset) and the one-walk annual fallback all give the plain method's annual
+    totals (FP64 oracle, 1e-4)."""
     sc = synth_scene(11, 48, 40)
     ref = oracle_lists(11, 48, 40)
     az_alt, dni, dhi, bins = sky()
     exact = orc.annual_fp64(sc, bins, dni, dhi, ref["svf"], ref["off"], ref["ids"])
     mlp = sm.MLP(seed=2)
     with _ctx(sc, lists=False) as ctx:
+        ctx.set_tuning(32, 128, 4, 32, 0, queue)
         ctx_lat = SITE[1]
         sm.prepare(ctx, ctx_lat)
         out, fb, st = sm.run(ctx, mlp, sm.Gate(-1.0, 1.0, 1.0, 0.0), fallback)

@@ -187,6 +187,16 @@ def test_8192_barle_only_sampled_oracle_and_surrogate_vs_plain():
         del svf
         rel = np.abs(plain[vox].astype(np.float64) - exact) / np.maximum(np.abs(exact), 1.0)
         assert rel.max() <= ANNUAL_RTOL, rel.max()
+        # receiver pairs at 256 hours per pass: a 69 GB hourly buffer, rows addressed by 64-bit offsets
+        ctx.set_tuning(32, 256, 4, 512, 0, 2)
+        ctx.hour_loop(sg.FMT_BARLE, 256)
+        paired = ctx.irradiance()
+        print(f"8192^2 pairs: hour loop {ctx.info()['t_hours_ms']:.0f} ms", flush=True)
+        rel = np.abs(paired[vox].astype(np.float64) - exact) / np.maximum(np.abs(exact), 1.0)
+        assert rel.max() <= ANNUAL_RTOL, rel.max()
+        assert (np.abs(paired - plain) / np.maximum(np.abs(plain), 1.0)).max() <= 1e-5
+        del paired
+        ctx.set_tuning(*TUNED)
         sm.prepare(ctx, site[1], cell)
         out, fb, st = sm.run(ctx, mlp, gate, sm.FALLBACK_ANNUAL, err_mlp)
     e = np.abs(out.astype(np.float64) - plain) / np.maximum(np.abs(plain.astype(np.float64)), 1.0)
