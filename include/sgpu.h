@@ -384,7 +384,9 @@ int sgpu_batch_viewshed(sgpu_batch* b);
 int sgpu_batch_hour_loop(sgpu_batch* b, unsigned format, int hours_per_pass, int waves, int graph);
 /* Reflection-pass receiver assignment of the batched hour loop (new): 1 = waves take receivers one
    by one from per-XCD queues over every tile's positions; 0 (the default) = static blocks of the
-   concatenated tiles.  Both give bit-identical annual sums. */
+   concatenated tiles; both give bit-identical annual sums.  2 = waves take pairs of consecutive
+   receivers of a tile (ba-RLE lists, 16 or more hours per pass; otherwise as 1), each tile's sums
+   equal to those of a context of its own with receiver_queue 2. */
 int sgpu_batch_set_receiver_queue(sgpu_batch* b, int on);
 int sgpu_batch_get_info(sgpu_batch* b, sgpu_batch_info* out);
 /* Per tile: sizes, then the reference-layout products (offsets rebased to the tile). */

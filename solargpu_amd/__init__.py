@@ -544,9 +544,10 @@ class Batch:
     def viewshed(self) -> None:
         _check(_lib.sgpu_batch_viewshed(self._h))
 
-    def set_receiver_queue(self, on: bool) -> None:
-        """Queued or static (default) receiver assignment of the batched reflection pass."""
-        _check(_lib.sgpu_batch_set_receiver_queue(self._h, int(bool(on))))
+    def set_receiver_queue(self, on) -> None:
+        """Receiver assignment of the batched reflection pass: False / 0 static blocks (default),
+        True / 1 queued receivers, 2 queued receiver pairs."""
+        _check(_lib.sgpu_batch_set_receiver_queue(self._h, int(on)))
 
     def hour_loop(self, fmt: int = FMT_BARLE, hours_per_pass: int = 64, waves: int = 4, graph: bool = True) -> None:
         _check(_lib.sgpu_batch_hour_loop(self._h, ctypes.c_uint(fmt), hours_per_pass, waves, int(graph)))

@@ -4,9 +4,10 @@ Workload: 64 seeded tiles, side log-uniform in [128, 1024] cells of 1 m,
 building density U(10 %, 60 %), latitude U(30, 60), longitude U(-10, 30),
 synthetic weather per tile.  Batched: one persistent viewshed launch per voxel
 chunk over all tiles and a hipGraph hour loop with one solar and one reflection
-launch per pass (static blocks of the concatenated tiles; the assignment from
-per-XCD receiver queues is timed too).  Sequential: a context per tile (viewshed + hour loop), the
-same launch shapes.  Both on the GPU (HIP events); prints one JSON line.
+launch per pass (queued receiver pairs on ba-RLE lists; static blocks of the
+concatenated tiles and receivers one by one from per-XCD queues are timed too).
+Sequential: a context per tile (viewshed + hour loop), the same launch shapes.
+Both on the GPU (HIP events); prints one JSON line.
 
     python -m solargpu_amd.bench_batch [--tiles 64] [--fmt barle]
 """
@@ -62,10 +63,13 @@ def main(argv=None):
             b.add_tile(sc, az, dni, dhi)
         b.viewshed()
         b.hour_loop(fmt, K, waves, graph=True)   # warm-up (graph build, first touch)
-        b.set_receiver_queue(True)                # receivers from per-XCD queues, for comparison
-        b.hour_loop(fmt, K, waves, graph=True)
-        queued_ms = b.info()["t_hours_ms"]
-        b.set_receiver_queue(False)
+        by_mode = {}
+        for mode in (0, 1, 2):   # static blocks, receivers from per-XCD queues, receiver pairs
+            b.set_receiver_queue(mode)
+            b.hour_loop(fmt, K, waves, graph=True)
+            by_mode[mode] = b.info()["t_hours_ms"]
+        mode = 2 if fmt == sg.FMT_BARLE and K >= 16 else 0   # pairs walk ba-RLE lists at K >= 16
+        b.set_receiver_queue(mode)
         b.viewshed()
         b.hour_loop(fmt, K, waves, graph=True)
         bi = b.info()
@@ -77,16 +81,18 @@ def main(argv=None):
            "hours_per_pass": K, "waves": waves,
            "batched": {"viewshed_ms": bi["t_viewshed_ms"], "trace_ms": bi["t_trace_ms"],
                        "lists_ms": bi["t_lists_ms"], "encode_ms": bi["t_encode_ms"], "hours_ms": bi["t_hours_ms"],
+                       "receiver_assignment": {0: "static blocks", 1: "queued receivers", 2: "queued pairs"}[mode],
                        "dda_rays_per_s": n_vox * 2592 / (bi["t_viewshed_ms"] / 1e3),
                        "voxel_hours_per_s": bi["voxel_hours"] / (bi["t_hours_ms"] / 1e3),
-                       "hours_ms_queued_assignment": queued_ms}}
+                       "hours_ms_static_blocks": by_mode[0], "hours_ms_queued_receivers": by_mode[1],
+                       "hours_ms_queued_pairs": by_mode[2]}}
     if not args.no_sequential:
         tv = th = 0.0
         for k, (sc, az, dni, dhi) in enumerate(tiles):
             with sg.Context(0) as ctx:
                 ctx.set_scene(sc)
                 ctx.set_sky(az, dni, dhi)
-                ctx.set_tuning(32, K, waves, 256, 0, 1)   # the batched launches' shapes (queued receivers)
+                ctx.set_tuning(32, K, waves, 256, 0, max(mode, 1))   # the batched launches' shapes (queued)
                 ctx.viewshed(sg.FMT_CSR | sg.FMT_BARLE)
                 tv += ctx.info()["t_viewshed_ms"]
                 ctx.hour_loop(fmt, K)

@@ -815,37 +815,42 @@ __global__ void k_clear_queue(unsigned* q) {
 /* Each receiver's own irradiance over the pass, for the paired walk: own[q] = the sum over the pass's
  * hours, one thread per position (voxel v = p in full-scene order, else subset[p]; q = p). */
 template <int K>
+__device__ __forceinline__ float own_sum(const HourArgs& a, const HourConst* hrs, int64_t v, float inv_w) {
+    const Recv r = receiver_of(a, v, inv_w);
+    const int nh = min(K, a.nh);
+    float s = 0.0f;
+    for (int h = 0; h < nh; ++h) {
+        const HourConst& hc = hrs[h];
+        s += hour_irradiance(hc, r.cs, r.ss, r.ca, r.sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
+    }
+    return s;
+}
+
+template <int K>
 __global__ __launch_bounds__(256) void k_own_pass(HourArgs a, float* __restrict__ own) {
     __shared__ HourConst hrs[K];
     stage_hours<K>(a, hrs);
     __syncthreads();
     const int64_t n = a.order ? a.n_total : a.n_sub;
     const float inv_w = 1.0f / (float)a.W;
-    const int nh = min(K, a.nh);
-    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
-        const int64_t v = a.order ? p : (int64_t)a.subset[p];
-        const Recv r = receiver_of(a, v, inv_w);
-        float s = 0.0f;
-        for (int h = 0; h < nh; ++h) {
-            const HourConst& hc = hrs[h];
-            s += hour_irradiance(hc, r.cs, r.ss, r.ca, r.sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
-        }
-        own[p] = s;
-    }
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256)
+        own[p] = own_sum<K>(a, hrs, a.order ? p : (int64_t)a.subset[p], inv_w);
 }
 
-/* Per-wave stream window of the paired kernel, and the entries of each kept list held in LDS (the
-   rest go to the wave's scratch). */
+/* Per-wave stream window of the paired kernels, and the entries of each kept list held in LDS (the
+   rest go to the wave's scratch): 240 keeps 7 workgroups of 4 waves per CU for the batched kernel,
+   whose waves also hold their tile's arguments in LDS. */
 constexpr int kGroupStageBytes = 1024;
-constexpr int kPairLds = 256;
+constexpr int kPairLds = 240;
 
 /* One wave's LDS in the paired kernel: the first kPairLds entries of both kept lists; while the
  * streams are decoded, the stream window and the decoded ids not yet weighed; while the lists are
  * merged, their 64-entry heads and, per entry of the first head, the second receiver's weight.  A
  * pair walked receiver by receiver uses lid[0] / lw[0] as its compacted list. */
-struct PairLds {
-    int lid[2][kPairLds];
-    float lw[2][kPairLds];
+template <int C>
+struct PairLdsT {
+    int lid[2][C];
+    float lw[2][C];
     union {
         struct {
             __align__(16) uint8_t stage[kGroupStageBytes];
@@ -859,6 +864,7 @@ struct PairLds {
     } u;
 };
 static_assert(kPairLds >= kListCap, "the receiver-by-receiver walk's list fits lid[0]");
+static_assert(sizeof(PairLdsT<kPairLds>) == 16 * kPairLds + 1536, "LDS per wave of the paired kernels");
 
 /* Position p of the full scene in processing order (voxel order[p], lists and sums by voxel) or
    of a subset (voxel subset[p], lists and sums by position): its voxel and its list / sum index. */
@@ -876,12 +882,10 @@ __device__ __forceinline__ int rank_desc(const int* c, int n, int x) {
     return pos;
 }
 
-/* Receiver pairs, queued as k_reflect_queue but by pairs of consecutive positions (XCD bands of
- * queue_band / 2 pairs).  Receivers next to each other in the processing order share most of their
- * kept neighbours (a top cell and its walls, walls stacked on one cell, adjacent cells): a wave
- * that walks the two lists together gathers each shared row once.
+/* One wave's walk of a receiver pair (voxels v0, v1; list / sum indices q0, q1; the second absent
+ * unless `two`), the body of the paired kernels:
  * Phase 1: each receiver's ba-RLE stream is decoded and weighed; its kept (id, weight) entries, in
- *   stream order (descending ids), go to list j: entries k < kPairLds in LDS, the rest to the wave's
+ *   stream order (descending ids), go to list j: entries k < C in LDS, the rest to the wave's
  *   scratch (list j's part from sb_j).
  * Phase 2: the two descending lists are merged 64-entry heads at a time.  With t the larger of the
  *   64th head entries of the lists that have more than 64 left, every entry >= t of either list is
@@ -891,15 +895,165 @@ __device__ __forceinline__ int rank_desc(const int* c, int n, int x) {
  *   head, so every round makes progress.  Rows of the first head's entries are added into both sums
  *   (a zero weight where the second list lacks the id); the second head's remaining rows into the
  *   second receiver's only.
- * A pair whose lists overflow the scratch is walked receiver by receiver (receiver_parts).  Each
- * receiver's own irradiance comes from k_own_pass (own[q]); the annual Kahan add is as in
- * k_reflect_queue.  A receiver's sum differs from the receiver-by-receiver walk only in the order of
- * its additions, and does not depend on which wave takes the pair. */
+ * A pair whose lists overflow the scratch (or exceed a.grp_limit) is walked receiver by receiver
+ * (receiver_parts, the hour constants read from global memory).  Each receiver's own irradiance is
+ * a.grp_own[q]; the pass total enters the annual Kahan sum. */
+template <int K, int C>
+__device__ __forceinline__ void pair_walk(const HourArgs& a, int64_t v0, int64_t q0, int64_t v1, int64_t q1, bool two,
+                                          PairLdsT<C>& L, StreamWindow& win, int2* scr, int cap, int lane) {
+    const float inv_w = 1.0f / (float)a.W;
+    const int cells = a.W * a.H;
+    const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);
+    win.lo = win.hi = 0;   // a previous merge overwrote the window's bytes
+
+    /* phase 1 */
+    int nl[2] = {0, 0}, sb[2] = {0, 0};
+    int spill = 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (j == 1 && !two) continue;
+        const int64_t v = j == 0 ? v0 : v1, q = j == 0 ? q0 : q1;
+        const Recv r = receiver_of(a, v, inv_w);
+        const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
+        win.end = s1;
+        int cnt = 0;
+        sb[j] = spill;
+        auto take = [&](bool valid, int id) __attribute__((always_inline)) {
+            const float w = valid ? neighbour_weight(id, r, a.grid, a.W, cells, inv_w) : 0.0f;
+            const bool keep = w > 0.0f;
+            const uint64_t m = __ballot(keep);
+            const int at = cnt + lanes_below(m);
+            if (keep) {
+                if (at < C) {
+                    L.lid[j][at] = id;
+                    L.lw[j][at] = w;
+                } else if (spill + (at - C) < cap) {
+                    scr[spill + (at - C)] = make_int2(id, __float_as_int(w));
+                }
+            }
+            cnt += __popcll(m);
+        };
+        decode_chunks(win, s0, s1, L.u.dec.pend_id, lane, take);
+        nl[j] = cnt;
+        spill += max(cnt - C, 0);
+    }
+    if (spill > cap || (a.grp_limit > 0 && nl[0] + nl[1] > a.grp_limit)) {
+        /* walked receiver by receiver, as k_reflect_queue does (the hour constants read from
+           global memory) */
+        for (int j = 0; j < (two ? 2 : 1); ++j) {
+            const int64_t v = j == 0 ? v0 : v1, q = j == 0 ? q0 : q1;
+            const Recv r = receiver_of(a, v, inv_w);
+            const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
+            win.end = s1;
+            float own, refl;
+            wave_lds_sync();
+            receiver_parts<K, SGPU_FMT_BARLE>(a, a.hours, r, v, s0, s1, win, L.lid[0], L.lw[0],
+                                              L.u.dec.pend_id, lane, off32, inv_w, cells, own, refl);
+            const float t = wave_sum(own + refl);
+            if (lane == 0) kahan_add(a.sum, a.comp, q, t);
+        }
+        return;
+    }
+
+    /* phase 2 (the scratch entries and LDS lists written by other lanes are visible after the
+       fence of wave_lds_sync) */
+    wave_lds_sync();
+    float acc0 = 0.0f, acc1 = 0.0f;
+    int h0 = 0, h1 = 0;
+    while (h0 < nl[0] || h1 < nl[1]) {
+        int id0 = -1, id1 = -1;
+        float w0 = 0.0f, w1 = 0.0f;
+        {
+            const int k = h0 + lane;
+            if (k < nl[0]) {
+                if (k < C) {
+                    id0 = L.lid[0][k];
+                    w0 = L.lw[0][k];
+                } else {
+                    const int2 e = scr[sb[0] + (k - C)];
+                    id0 = e.x;
+                    w0 = __int_as_float(e.y);
+                }
+            }
+        }
+        {
+            const int k = h1 + lane;
+            if (k < nl[1]) {
+                if (k < C) {
+                    id1 = L.lid[1][k];
+                    w1 = L.lw[1][k];
+                } else {
+                    const int2 e = scr[sb[1] + (k - C)];
+                    id1 = e.x;
+                    w1 = __int_as_float(e.y);
+                }
+            }
+        }
+        int t = -1;
+        if (nl[0] - h0 > 64) t = max(t, __builtin_amdgcn_readlane(id0, 63));
+        if (nl[1] - h1 > 64) t = max(t, __builtin_amdgcn_readlane(id1, 63));
+        const int tmin = max(t, 0);   // t < 0: both lists end within their heads
+        const bool in0 = id0 >= tmin, in1 = id1 >= tmin;   // prefixes of the descending heads
+        const int n0 = __popcll(__ballot(in0)), n1 = __popcll(__ballot(in1));
+        wave_lds_sync();   // the previous round's reads of the heads are done
+        L.u.mrg.cid[0][lane] = id0;
+        L.u.mrg.cid[1][lane] = id1;
+        L.u.mrg.cw[0][lane] = w0;
+        L.u.mrg.cw[1][lane] = w1;
+        wave_lds_sync();
+        float wx = 0.0f;
+        if (in0) {
+            const int pos = rank_desc(L.u.mrg.cid[1], n1, id0);
+            if (pos < n1 && L.u.mrg.cid[1][pos] == id0) wx = L.u.mrg.cw[1][pos];
+        }
+        bool rest = false;   // in the second head only
+        if (in1) {
+            const int pos = rank_desc(L.u.mrg.cid[0], n0, id1);
+            rest = !(pos < n0 && L.u.mrg.cid[0][pos] == id1);
+        }
+        const uint64_t mr = __ballot(rest);
+        const int nr = __popcll(mr);
+        wave_lds_sync();   // the searches of both heads are done
+        /* from here the heads hold row slots: the first's in place, the second's own compacted */
+        L.u.mrg.cid[0][lane] = (int)row_slot((unsigned)id0, a.row_swz);
+        L.u.mrg.wx[lane] = wx;
+        if (rest) {
+            const int at = lanes_below(mr);
+            L.u.mrg.cid[1][at] = (int)row_slot((unsigned)id1, a.row_swz);
+            L.u.mrg.cw[1][at] = w1;
+        }
+        wave_lds_sync();
+        float unused = 0.0f;
+        if (off32) {
+            apply_slots_at<K, true, 2>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, acc0, acc1);
+            apply_slots_at<K, true, 1>(L.u.mrg.cid[1], L.u.mrg.cw[1], nullptr, nr, a.hourly_top, lane, acc1, unused);
+        } else {
+            apply_slots_at<K, false, 2>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, acc0,
+                                        acc1);
+            apply_slots_at<K, false, 1>(L.u.mrg.cid[1], L.u.mrg.cw[1], nullptr, nr, a.hourly_top, lane, acc1,
+                                        unused);
+        }
+        h0 += n0;
+        h1 += n1;
+    }
+    const float r0 = wave_sum(acc0), r1 = wave_sum(acc1);
+    if (lane == 0) {
+        kahan_add(a.sum, a.comp, q0, a.grp_own[q0] + r0);
+        if (two) kahan_add(a.sum, a.comp, q1, a.grp_own[q1] + r1);
+    }
+}
+
+/* Receiver pairs, queued as k_reflect_queue but by pairs of consecutive positions (XCD bands of
+ * queue_band / 2 pairs).  Receivers next to each other in the processing order share most of their
+ * kept neighbours (a top cell and its walls, walls stacked on one cell, adjacent cells): a wave
+ * that walks the two lists together gathers each shared row once (pair_walk).  Each receiver's own
+ * irradiance comes from k_own_pass.  A receiver's sum differs from the receiver-by-receiver walk only
+ * in the order of its additions, and does not depend on which wave takes the pair. */
 template <int K, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(7)))
 void k_reflect_pair(HourArgs a) {
     static_assert(K >= 16, "four hours per lane");
-    __shared__ PairLds PL[WAVES];
+    __shared__ PairLdsT<kPairLds> PL[WAVES];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     const unsigned x = blockIdx.x % kXcds;
     const unsigned c = ((blockIdx.x / kXcds) * WAVES + (unsigned)wave) % kQueuesPerXcd;
@@ -907,12 +1061,9 @@ void k_reflect_pair(HourArgs a) {
     const int64_t n = a.order ? a.n_total : a.n_sub;
     const int64_t ng = (n + 1) / 2;
     const int64_t band = a.queue_band > 0 ? max<int64_t>(a.queue_band / 2, 1) : 0;
-    const float inv_w = 1.0f / (float)a.W;
-    const int cells = a.W * a.H;
-    const bool off32 = a.hourly_top_cap <= ((int64_t)1 << 30);
     const int cap = a.grp_cap;
     int2* scr = a.grp_scratch + ((int64_t)blockIdx.x * WAVES + wave) * cap;
-    PairLds& L = PL[wave];
+    PairLdsT<kPairLds>& L = PL[wave];
     StreamWindow win{L.u.dec.stage, a.nb_bytes, 0, 0, 0, kGroupStageBytes};
     unsigned pending = atomicAdd(counter, lane == 0 ? 1u : 0u);   // every lane joins (see k_reflect_queue)
     for (int64_t turn = 0; turn <= ng; ++turn) {
@@ -922,150 +1073,10 @@ void k_reflect_pair(HourArgs a) {
         pending = atomicAdd(counter, lane == 0 ? 1u : 0u);
         const int64_t p0 = g * 2;
         const bool two = p0 + 1 < n;
-        win.lo = win.hi = 0;   // the merge overwrote the window's bytes
-
-        /* phase 1 */
-        int nl[2] = {0, 0}, sb[2] = {0, 0};
-        int spill = 0;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            if (j == 1 && !two) continue;
-            int64_t v, q;
-            queue_voxel(a, p0 + j, v, q);
-            const Recv r = receiver_of(a, v, inv_w);
-            const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
-            win.end = s1;
-            int cnt = 0;
-            sb[j] = spill;
-            auto take = [&](bool valid, int id) __attribute__((always_inline)) {
-                const float w = valid ? neighbour_weight(id, r, a.grid, a.W, cells, inv_w) : 0.0f;
-                const bool keep = w > 0.0f;
-                const uint64_t m = __ballot(keep);
-                const int at = cnt + lanes_below(m);
-                if (keep) {
-                    if (at < kPairLds) {
-                        L.lid[j][at] = id;
-                        L.lw[j][at] = w;
-                    } else if (spill + (at - kPairLds) < cap) {
-                        scr[spill + (at - kPairLds)] = make_int2(id, __float_as_int(w));
-                    }
-                }
-                cnt += __popcll(m);
-            };
-            decode_chunks(win, s0, s1, L.u.dec.pend_id, lane, take);
-            nl[j] = cnt;
-            spill += max(cnt - kPairLds, 0);
-        }
-        if (spill > cap || (a.grp_limit > 0 && nl[0] + nl[1] > a.grp_limit)) {
-            /* walked receiver by receiver, as k_reflect_queue does (the hour constants read from
-               global memory) */
-            for (int j = 0; j < (two ? 2 : 1); ++j) {
-                int64_t v, q;
-                queue_voxel(a, p0 + j, v, q);
-                const Recv r = receiver_of(a, v, inv_w);
-                const int64_t s0 = uniform64(a.nb_off[q]), s1 = uniform64(a.nb_off[q + 1]);
-                win.end = s1;
-                float own, refl;
-                wave_lds_sync();
-                receiver_parts<K, SGPU_FMT_BARLE>(a, a.hours, r, v, s0, s1, win, L.lid[0], L.lw[0],
-                                                  L.u.dec.pend_id, lane, off32, inv_w, cells, own, refl);
-                const float t = wave_sum(own + refl);
-                if (lane == 0) kahan_add(a.sum, a.comp, q, t);
-            }
-            continue;
-        }
-
-        /* phase 2 (the scratch entries and LDS lists written by other lanes are visible after the
-           fence of wave_lds_sync) */
-        wave_lds_sync();
-        float acc0 = 0.0f, acc1 = 0.0f;
-        int h0 = 0, h1 = 0;
-        while (h0 < nl[0] || h1 < nl[1]) {
-            int id0 = -1, id1 = -1;
-            float w0 = 0.0f, w1 = 0.0f;
-            {
-                const int k = h0 + lane;
-                if (k < nl[0]) {
-                    if (k < kPairLds) {
-                        id0 = L.lid[0][k];
-                        w0 = L.lw[0][k];
-                    } else {
-                        const int2 e = scr[sb[0] + (k - kPairLds)];
-                        id0 = e.x;
-                        w0 = __int_as_float(e.y);
-                    }
-                }
-            }
-            {
-                const int k = h1 + lane;
-                if (k < nl[1]) {
-                    if (k < kPairLds) {
-                        id1 = L.lid[1][k];
-                        w1 = L.lw[1][k];
-                    } else {
-                        const int2 e = scr[sb[1] + (k - kPairLds)];
-                        id1 = e.x;
-                        w1 = __int_as_float(e.y);
-                    }
-                }
-            }
-            int t = -1;
-            if (nl[0] - h0 > 64) t = max(t, __builtin_amdgcn_readlane(id0, 63));
-            if (nl[1] - h1 > 64) t = max(t, __builtin_amdgcn_readlane(id1, 63));
-            const int tmin = max(t, 0);   // t < 0: both lists end within their heads
-            const bool in0 = id0 >= tmin, in1 = id1 >= tmin;   // prefixes of the descending heads
-            const int n0 = __popcll(__ballot(in0)), n1 = __popcll(__ballot(in1));
-            wave_lds_sync();   // the previous round's reads of the heads are done
-            L.u.mrg.cid[0][lane] = id0;
-            L.u.mrg.cid[1][lane] = id1;
-            L.u.mrg.cw[0][lane] = w0;
-            L.u.mrg.cw[1][lane] = w1;
-            wave_lds_sync();
-            float wx = 0.0f;
-            if (in0) {
-                const int pos = rank_desc(L.u.mrg.cid[1], n1, id0);
-                if (pos < n1 && L.u.mrg.cid[1][pos] == id0) wx = L.u.mrg.cw[1][pos];
-            }
-            bool rest = false;   // in the second head only
-            if (in1) {
-                const int pos = rank_desc(L.u.mrg.cid[0], n0, id1);
-                rest = !(pos < n0 && L.u.mrg.cid[0][pos] == id1);
-            }
-            const uint64_t mr = __ballot(rest);
-            const int nr = __popcll(mr);
-            wave_lds_sync();   // the searches of both heads are done
-            /* from here the heads hold row slots: the first's in place, the second's own compacted */
-            L.u.mrg.cid[0][lane] = (int)row_slot((unsigned)id0, a.row_swz);
-            L.u.mrg.wx[lane] = wx;
-            if (rest) {
-                const int at = lanes_below(mr);
-                L.u.mrg.cid[1][at] = (int)row_slot((unsigned)id1, a.row_swz);
-                L.u.mrg.cw[1][at] = w1;
-            }
-            wave_lds_sync();
-            float unused = 0.0f;
-            if (off32) {
-                apply_slots_at<K, true, 2>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, acc0, acc1);
-                apply_slots_at<K, true, 1>(L.u.mrg.cid[1], L.u.mrg.cw[1], nullptr, nr, a.hourly_top, lane, acc1, unused);
-            } else {
-                apply_slots_at<K, false, 2>(L.u.mrg.cid[0], L.u.mrg.cw[0], L.u.mrg.wx, n0, a.hourly_top, lane, acc0,
-                                            acc1);
-                apply_slots_at<K, false, 1>(L.u.mrg.cid[1], L.u.mrg.cw[1], nullptr, nr, a.hourly_top, lane, acc1,
-                                            unused);
-            }
-            h0 += n0;
-            h1 += n1;
-        }
-        const float r0 = wave_sum(acc0), r1 = wave_sum(acc1);
-        if (lane == 0) {
-            int64_t v, q;
-            queue_voxel(a, p0, v, q);
-            kahan_add(a.sum, a.comp, q, a.grp_own[q] + r0);
-            if (two) {
-                queue_voxel(a, p0 + 1, v, q);
-                kahan_add(a.sum, a.comp, q, a.grp_own[q] + r1);
-            }
-        }
+        int64_t v0, q0, v1 = 0, q1 = 0;
+        queue_voxel(a, p0, v0, q0);
+        if (two) queue_voxel(a, p0 + 1, v1, q1);
+        pair_walk<K, kPairLds>(a, v0, q0, v1, q1, two, L, win, scr, cap, lane);
     }
 }
 
@@ -1196,6 +1207,89 @@ void k_reflect_batch_queue(const BatchHour* d, int ntiles, int pass, unsigned* q
     }
 }
 
+/* Batched receiver pairs: each tile's own irradiance of the pass (k_own_pass over its voxels, with
+   its own hour constants); block b belongs to the last tile whose first own block is <= b. */
+template <int K>
+__global__ __launch_bounds__(256) void k_own_pass_batch(const BatchHour* d, int ntiles, int pass) {
+    __shared__ HourConst hrs[K];
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].own_block0 <= (int64_t)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    HourArgs a;
+    if (!tile_pass_args<K>(d[lo], pass, a)) return;   // the whole block leaves together
+    stage_hours<K>(a, hrs);
+    __syncthreads();
+    const int64_t v = ((int64_t)blockIdx.x - d[lo].own_block0) * 256 + threadIdx.x;
+    if (v < a.n_total) a.grp_own[v] = own_sum<K>(a, hrs, v, 1.0f / (float)a.W);
+}
+
+/* The tile owning pair slot g: the last tile whose first slot is <= g (wave-uniform). */
+__device__ __forceinline__ int tile_of_slot(const BatchHour* d, int ntiles, int64_t g) {
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].pslot0 <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+/* Batched receiver pairs: k_reflect_pair over the pair slots of every tile (tile t's slot k = its
+ * positions 2k and 2k + 1 in its processing order, so each tile's pairs are those of a context of
+ * its own and its sums equal that context's bit for bit).  A wave takes a slot from its XCD's queue,
+ * finds the tile (its pass arguments copied into the wave's LDS slot, as k_reflect_batch_queue
+ * does) and walks the pair; tiles whose hours are done are passed over. */
+template <int K, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(7)))
+void k_reflect_batch_pair(const BatchHour* d, int ntiles, int pass, unsigned* queue, int64_t nslots, int64_t band,
+                          int2* scratch) {
+    static_assert(K >= 16, "four hours per lane");
+    __shared__ PairLdsT<kPairLds> PL[WAVES];
+    __shared__ HourArgs targs[WAVES];
+    static_assert(sizeof(HourArgs) % 4 == 0, "copied as words");
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const unsigned x = blockIdx.x % kXcds;
+    const unsigned c = ((blockIdx.x / kXcds) * WAVES + (unsigned)wave) % kQueuesPerXcd;
+    unsigned* counter = queue + (x * kQueuesPerXcd + c) * kQueueStride;
+    int2* scr = scratch + ((int64_t)blockIdx.x * WAVES + wave) * kGroupCap;
+    PairLdsT<kPairLds>& L = PL[wave];
+    StreamWindow win{L.u.dec.stage, d[0].a.nb_bytes, 0, 0, 0, kGroupStageBytes};
+    unsigned pending = atomicAdd(counter, lane == 0 ? 1u : 0u);   // every lane joins (see k_reflect_queue)
+    int64_t s_lo = 0, s_hi = 0;
+    bool live = false;   // the loaded tile has hours in this pass
+    const HourArgs& a = targs[wave];
+    for (int64_t turn = 0; turn <= nslots; ++turn) {
+        const unsigned item = (unsigned)__builtin_amdgcn_readfirstlane((int)pending);
+        const int64_t g = queue_position((uint64_t)item * kQueuesPerXcd + c, x, nslots, band);
+        if (g >= nslots) break;
+        pending = atomicAdd(counter, lane == 0 ? 1u : 0u);
+        if (g < s_lo || g >= s_hi) {
+            const int t = tile_of_slot(d, ntiles, g);
+            s_lo = d[t].pslot0;
+            s_hi = s_lo + (d[t].a.n_total + 1) / 2;
+            live = pass * K < d[t].nsel;
+            wave_lds_sync();   // the wave's reads of the previous tile's arguments are done
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(&d[t].a);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(&targs[wave]);
+            for (int i = lane; i < (int)(sizeof(HourArgs) / 4); i += 64) dst[i] = src[i];
+            wave_lds_sync();
+            if (live && lane == 0) {   // tile_pass_args: this pass's hours
+                targs[wave].hours = d[t].a.hours + pass * K;
+                targs[wave].nh = min(K, d[t].nsel - pass * K);
+            }
+            wave_lds_sync();
+        }
+        if (!live) continue;
+        const int64_t p0 = (g - s_lo) * 2;   // the pair's first position in the tile
+        const bool two = p0 + 1 < a.n_total;
+        const int64_t v0 = a.order[p0], v1 = two ? (int64_t)a.order[p0 + 1] : 0;
+        pair_walk<K, kPairLds>(a, v0, v0, v1, v1, two, L, win, scr, kGroupCap, lane);
+    }
+}
+
 template <int K>
 void solar_top_k(const HourArgs& a, hipStream_t s) {
     const unsigned g = (unsigned)((a.n_top + kSolarBlock - 1) / kSolarBlock);
@@ -1217,10 +1311,20 @@ void reflect_kw(const HourArgs& a, unsigned fmt, hipStream_t s) {
 template <int K, int WAVES>
 void batch_queue_kw(const BatchHour* d, int ntiles, unsigned fmt, int pass, unsigned* queue, int64_t n, hipStream_t s);
 
+template <int K, int WAVES>
+void batch_pair_kw(const BatchHour* d, int ntiles, int pass, unsigned* queue, const BatchPairs& bp, hipStream_t s);
+
 template <int K>
 void batch_k(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, unsigned fmt, int waves, int pass,
-             unsigned* queue, int64_t n_all, hipStream_t s) {
+             unsigned* queue, int64_t n_all, const BatchPairs* pairs, hipStream_t s) {
     hipLaunchKernelGGL(k_solar_top_batch<K>, dim3((unsigned)top_blocks), dim3(kSolarBlock), 0, s, d, ntiles, pass);
+    if constexpr (K >= 16) {
+        if (queue && pairs && fmt == SGPU_FMT_BARLE) {
+            if (waves == 2) batch_pair_kw<K, 2>(d, ntiles, pass, queue, *pairs, s);
+            else batch_pair_kw<K, 4>(d, ntiles, pass, queue, *pairs, s);
+            return;
+        }
+    }
     if (queue) {
         if (waves == 2) batch_queue_kw<K, 2>(d, ntiles, fmt, pass, queue, n_all, s);
         else batch_queue_kw<K, 4>(d, ntiles, fmt, pass, queue, n_all, s);
@@ -1331,6 +1435,21 @@ void batch_queue_kw(const BatchHour* d, int ntiles, unsigned fmt, int pass, unsi
     }
 }
 
+/* Batched receiver pairs: XCD bands of half the batched queue's band of positions, in pair slots. */
+template <int K, int WAVES>
+void batch_pair_kw(const BatchHour* d, int ntiles, int pass, unsigned* queue, const BatchPairs& bp, hipStream_t s) {
+    static const GridShape g = occupancy(k_reflect_batch_pair<K, WAVES>, 64 * WAVES);
+    const unsigned grid = queue_grid(g, 0);
+    if ((int64_t)grid * WAVES > bp.waves)
+        fail(SGPU_ERR_STATE, "pair scratch holds %lld waves; the grid has %lld", (long long)bp.waves,
+             (long long)grid * WAVES);
+    if (bp.own_blocks > 0)
+        hipLaunchKernelGGL(k_own_pass_batch<K>, dim3((unsigned)bp.own_blocks), dim3(256), 0, s, d, ntiles, pass);
+    hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(kXcds * kQueuesPerXcd), 0, s, queue);
+    hipLaunchKernelGGL((k_reflect_batch_pair<K, WAVES>), dim3(grid), dim3(64 * WAVES), 0, s, d, ntiles, pass, queue,
+                       bp.nslots, kBatchQueueBand / 2, bp.scratch);
+}
+
 template <int K>
 void reflect_k(const HourArgs& a, unsigned fmt, hipStream_t s) {
     /* queued: the full scene in processing order, or a subset given by voxel ids */
@@ -1427,20 +1546,21 @@ void launch_reflect_accum(const HourArgs& a, int K, unsigned fmt, hipStream_t s)
 }
 
 void launch_hour_pass_batch(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, int K,
-                            unsigned fmt, int waves, int pass, unsigned* queue, int64_t n_all, hipStream_t s) {
+                            unsigned fmt, int waves, int pass, unsigned* queue, int64_t n_all, const BatchPairs* pairs,
+                            hipStream_t s) {
     if (ntiles <= 0 || blocks <= 0) return;
     if (fmt != SGPU_FMT_CSR && fmt != SGPU_FMT_BARLE) fail(SGPU_ERR_ARG, "format must be CSR or BARLE");
     if (waves != 2 && waves != 4) fail(SGPU_ERR_ARG, "waves per group must be 2 or 4");
     switch (K) {
-        case 1: batch_k<1>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
-        case 2: batch_k<2>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
-        case 4: batch_k<4>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
-        case 8: batch_k<8>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
-        case 16: batch_k<16>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
-        case 32: batch_k<32>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
-        case 64: batch_k<64>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
-        case 128: batch_k<128>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
-        case 256: batch_k<256>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, s); break;
+        case 1: batch_k<1>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
+        case 2: batch_k<2>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
+        case 4: batch_k<4>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
+        case 8: batch_k<8>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
+        case 16: batch_k<16>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
+        case 32: batch_k<32>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
+        case 64: batch_k<64>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
+        case 128: batch_k<128>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
+        case 256: batch_k<256>(d, ntiles, top_blocks, blocks, fmt, waves, pass, queue, n_all, pairs, s); break;
         default: fail(SGPU_ERR_ARG, "unsupported hours per pass %d", K);
     }
     SGPU_HIP(hipGetLastError());

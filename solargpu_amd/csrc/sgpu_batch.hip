@@ -79,8 +79,11 @@ struct sgpu_batch {
     float* hourly_top = nullptr;
     size_t hourly_top_elems = 0;
     unsigned* queue = nullptr;   // receiver-queue counters of the queued reflection (kQueueWords)
-    bool queued = false;         // queued reflection over all tiles' positions; false: static blocks (faster
-                                 // on 64 tiles at K = 256: 6.45 vs 6.59 s)
+    int queue_mode = 0;          // 0: static blocks (faster than 1 on 64 tiles at K = 256: 6.45 vs 6.59 s);
+                                 // 1: queued over all tiles' positions; 2: queued receiver pairs
+    int2* pair_scratch = nullptr;   // receiver pairs: kGroupCap entries per wave
+    int64_t pair_waves = 0;
+    float* own = nullptr;           // receiver pairs: each voxel's own irradiance of the pass (n_all)
     bool irr_ok = false;
     sgpu_batch_info info{};
 
@@ -100,6 +103,8 @@ struct sgpu_batch {
         dfree(comp);
         dfree(hourly_top);
         dfree(queue);
+        dfree(pair_scratch);
+        dfree(own);
         for (auto& t : tiles) t.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -386,8 +391,17 @@ void hour_loop(sgpu_batch* b, unsigned fmt, int K, int waves, bool graph) {
         b->hourly_top = dalloc<float>(elems);
         b->hourly_top_elems = elems;
     }
+    /* receiver pairs: ba-RLE lists, K >= 16 */
+    const bool pairs = b->queue_mode == 2 && fmt == SGPU_FMT_BARLE && K >= 16;
+    if (pairs && !b->pair_scratch) {
+        int cus = 0;
+        SGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+        b->pair_waves = (int64_t)std::max(cus, 1) * kMaxWavesPerCu;
+        b->pair_scratch = dalloc<int2>((size_t)b->pair_waves * kGroupCap);
+    }
+    if (pairs && !b->own) b->own = dalloc<float>((size_t)std::max<int64_t>(b->n_all, 1));
     std::vector<BatchHour> d(nt);
-    int64_t blocks = 0, top_blocks = 0;
+    int64_t blocks = 0, top_blocks = 0, own_blocks = 0, nslots = 0;
     int passes = 0;
     int64_t vh = 0;
     for (int i = 0; i < nt; ++i) {
@@ -424,11 +438,16 @@ void hour_loop(sgpu_batch* b, unsigned fmt, int K, int waves, bool graph) {
            index nb_off, sum and comp) and receiver groups, each tile's blocks whole groups */
         a.order = t.order;
         a.group = kBatchGroup;
+        if (pairs) a.grp_own = b->own + t.vbase;   // by the tile's voxel ids
         d[i].a = a;
         d[i].nsel = (int)t.hours.size();
         d[i].block0 = blocks;
         d[i].pos0 = t.vbase;
         d[i].top_block0 = top_blocks;
+        d[i].own_block0 = own_blocks;
+        d[i].pslot0 = nslots;
+        own_blocks += (t.n_total + 255) / 256;
+        nslots += (t.n_total + 1) / 2;
         blocks += ((t.n_total + 64 * waves - 1) / (64 * waves) + kBatchGroup - 1) / kBatchGroup * kBatchGroup;
         top_blocks += (t.n_top + 255) / 256;
         passes = std::max(passes, (d[i].nsel + K - 1) / K);
@@ -442,10 +461,11 @@ void hour_loop(sgpu_batch* b, unsigned fmt, int K, int waves, bool graph) {
         zero_floats(b->sum, b->n_all, s);
         zero_floats(b->comp, b->n_all, s);
         SGPU_HIP(hipStreamSynchronize(s));
+        const BatchPairs bp{b->pair_scratch, b->pair_waves, own_blocks, nslots};
         auto record = [&] {
             for (int p = 0; p < passes; ++p)
-                launch_hour_pass_batch(dd, nt, top_blocks, blocks, K, fmt, waves, p, b->queued ? b->queue : nullptr,
-                                       b->n_all, s);
+                launch_hour_pass_batch(dd, nt, top_blocks, blocks, K, fmt, waves, p,
+                                       b->queue_mode != 0 ? b->queue : nullptr, b->n_all, pairs ? &bp : nullptr, s);
         };
         EventPair ev;
         if (graph) {
@@ -559,8 +579,8 @@ int sgpu_batch_hour_loop(sgpu_batch* b, unsigned format, int hours_per_pass, int
 int sgpu_batch_set_receiver_queue(sgpu_batch* b, int on) {
     return guarded([&] {
         need(b != nullptr, SGPU_ERR_ARG, "null batch");
-        need(on == 0 || on == 1, SGPU_ERR_ARG, "on must be 0 or 1");
-        b->queued = on != 0;
+        need(on >= 0 && on <= 2, SGPU_ERR_ARG, "on must be 0, 1 or 2");
+        b->queue_mode = on;
     });
 }
 

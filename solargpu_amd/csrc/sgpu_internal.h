@@ -388,11 +388,24 @@ struct BatchHour {
     int nsel;            // the tile's sunlit hours
     int64_t block0, top_block0;
     int64_t pos0;        // the tile's first position in the batch's voxel space (queued reflection)
+    int64_t own_block0;  // receiver pairs: the tile's first block of the own-irradiance pass
+    int64_t pslot0;      // receiver pairs: the tile's first pair slot (its positions 2k, 2k + 1 = slot pslot0 + k)
+};
+/* Receiver pairs of the batched hour loop (ba-RLE, K >= 16): the queue hands out pair slots of every
+   tile (nslots in all), each walked by one wave with kGroupCap entries of `scratch` (covering
+   `waves` waves); each tile's own irradiance (a.grp_own) is written by own_blocks blocks first. */
+struct BatchPairs {
+    int2* scratch;
+    int64_t waves;
+    int64_t own_blocks;
+    int64_t nslots;
 };
 /* queue == NULL: static assignment (blocks of the concatenated tiles); else the queued kernel over
-   the n_all positions of the batch (kQueueWords counters, zeroed by the launcher). */
+   the n_all positions of the batch (kQueueWords counters, zeroed by the launcher), or, with pairs,
+   the paired kernel over the pair slots. */
 void launch_hour_pass_batch(const BatchHour* d, int ntiles, int64_t top_blocks, int64_t blocks, int K,
-                            unsigned fmt, int waves, int pass, unsigned* queue, int64_t n_all, hipStream_t s);
+                            unsigned fmt, int waves, int pass, unsigned* queue, int64_t n_all,
+                            const BatchPairs* pairs, hipStream_t s);
 void launch_reflect_accum(const HourArgs& a, int K, unsigned format, hipStream_t s);
 bool supported_hours_per_pass(int K);
 

@@ -30,11 +30,13 @@ def _tiles():
     return out
 
 
-@pytest.mark.parametrize("fmt,K,waves,queued", [(sg.FMT_BARLE, 64, 4, True), (sg.FMT_CSR, 16, 2, True),
-                                                (sg.FMT_BARLE, 128, 2, False), (sg.FMT_BARLE, 256, 4, False)])
+@pytest.mark.parametrize("fmt,K,waves,queued", [(sg.FMT_BARLE, 64, 4, 1), (sg.FMT_CSR, 16, 2, 1),
+                                                (sg.FMT_BARLE, 128, 2, 0), (sg.FMT_BARLE, 256, 4, 0),
+                                                (sg.FMT_BARLE, 64, 4, 2), (sg.FMT_BARLE, 256, 2, 2)])
 def test_batch_matches_single_tile_contexts(fmt, K, waves, queued):
     """Queued (receivers one by one from per-XCD queues over every tile) and static receiver
-    assignments both give each tile's own annual sums bit for bit."""
+    assignments both give each tile's own annual sums bit for bit; so do queued receiver pairs
+    against contexts that walk pairs too."""
     tiles = _tiles()
     with sg.Batch(0) as b:
         for sc, sky in tiles:
@@ -58,7 +60,7 @@ def test_batch_matches_single_tile_contexts(fmt, K, waves, queued):
                 e_boff, e_data = ctx.barle()
                 np.testing.assert_array_equal(boff, e_boff)
                 np.testing.assert_array_equal(data, e_data)
-                ctx.set_tuning(16, K, waves, 0, 0)
+                ctx.set_tuning(16, K, waves, 0, 0, 2 if queued == 2 else 0)
                 ctx.hour_loop(fmt, K)
                 np.testing.assert_array_equal(b.irradiance(t), ctx.irradiance(), err_msg=f"tile {t} annual")
 
