@@ -235,9 +235,13 @@ def test_annual_within_1e4_of_fp64(fmt, K):
     assert _rel(irr, exact).max() <= ANNUAL_RTOL
 
 
-def test_graph_replays_match_eager_loop():
+@pytest.mark.parametrize("queue", [1, 2])
+def test_graph_replays_match_eager_loop(queue):
+    """Captured hour loops (receivers one by one, or in pairs with the own-irradiance pass) replay
+    to the eager loop's sums bit for bit."""
     sc = synth_scene(11, 48, 40)
     with _ctx(sc) as ctx:
+        ctx.set_tuning(32, 16, 4, 32, 0, queue)
         ctx.hour_loop(sg.FMT_BARLE, 16)
         eager = ctx.irradiance()
         for _ in range(3):
