@@ -232,7 +232,7 @@ int sgpu_set_trace_tables(sgpu_ctx* ctx, int enable, int64_t pending_cap);
 int sgpu_set_trace_table_shape(sgpu_ctx* ctx, int dirs, int steps, int skip);
 /* Diagnostic (new; tests): receiver pairs (receiver_queue 2) whose kept neighbour lists hold more
    than `entries` entries together are walked receiver by receiver (0, the default: only pairs
-   whose lists overflow the pair scratch, 2 x 256 entries in LDS + 4096 per wave); a small limit
+   whose lists overflow the pair scratch, 2 x 240 entries in LDS + 4096 per wave); a small limit
    forces that path. */
 int sgpu_set_pair_limit(sgpu_ctx* ctx, int entries);
 /* Synchronise the context stream. */

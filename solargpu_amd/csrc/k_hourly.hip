@@ -843,7 +843,7 @@ __global__ __launch_bounds__(256) void k_own_pass(HourArgs a, float* __restrict_
 constexpr int kGroupStageBytes = 1024;
 constexpr int kPairLds = 240;
 
-/* One wave's LDS in the paired kernel: the first kPairLds entries of both kept lists; while the
+/* One wave's LDS in the paired kernels: the first C entries of both kept lists; while the
  * streams are decoded, the stream window and the decoded ids not yet weighed; while the lists are
  * merged, their 64-entry heads and, per entry of the first head, the second receiver's weight.  A
  * pair walked receiver by receiver uses lid[0] / lw[0] as its compacted list. */

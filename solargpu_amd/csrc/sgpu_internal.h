@@ -357,7 +357,7 @@ struct HourArgs {
     int queue_blocks_per_cu;   // resident workgroups per CU of the queued grid (0 = as many as fit)
     /* Receiver pairs (queued, ba-RLE, K >= 16; grp_size == 2): the queue hands out pairs of
        consecutive positions, each walked by one wave that gathers every kept neighbour row of the
-       pair once.  Kept entries beyond the first 256 of a list go to wave w's grp_cap entries of
+       pair once.  Kept entries beyond the first 240 of a list go to wave w's grp_cap entries of
        grp_scratch, from w * grp_cap; the buffer covers grp_waves waves.  grp_own: each position's
        own irradiance over the pass (by list / sum index, n_total floats), written by the launcher's
        first kernel.  grp_limit > 0: pairs with more kept entries are walked receiver by receiver.
