@@ -170,7 +170,7 @@ void autotune(sgpu_ctx* c, unsigned fmt, sgpu_tuning* out) {
                 for (int band : {0, 64, 256}) trial(K, waves, band, 0, 0);
                 for (int band : {32, 256, 512}) trial(K, waves, band, 0, 1);   // queued receivers
                 if (fmt == SGPU_FMT_BARLE && waves == 4)   // queued receiver pairs
-                    for (int band : {32, 256, 512}) trial(K, waves, band, 0, 2);
+                    for (int band : {0, 32, 256, 512}) trial(K, waves, band, 0, 2);
             }
         const int K = t.hours_per_pass;
         for (int waves : {2, 4})   // the best waves count with receiver groups can differ
