@@ -819,7 +819,24 @@ __device__ __forceinline__ float own_sum(const HourArgs& a, const HourConst* hrs
     const Recv r = receiver_of(a, v, inv_w);
     const int nh = min(K, a.nh);
     float s = 0.0f;
-    for (int h = 0; h < nh; ++h) {
+    int h = 0;
+    /* eight hours at a time: their SVF bytes requested together (a bin outside the dense set reads
+       byte 0 and is not blocked), then added in hour order */
+    for (; h + 8 <= nh; h += 8) {
+        unsigned byte[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int d = max(hrs[h + u].dense, 0);
+            byte[u] = a.svf[(int64_t)(d >> 3) * a.n_total + v];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const HourConst& hc = hrs[h + u];
+            const bool blocked = hc.dense >= 0 && ((byte[u] >> (hc.dense & 7)) & 1u);
+            s += hour_irradiance(hc, r.cs, r.ss, r.ca, r.sa, blocked);
+        }
+    }
+    for (; h < nh; ++h) {
         const HourConst& hc = hrs[h];
         s += hour_irradiance(hc, r.cs, r.ss, r.ca, r.sa, svf_blocked(a.svf, v, hc.dense, a.n_total));
     }
